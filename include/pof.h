@@ -1,0 +1,154 @@
+/* C ABI of libpof_hip.so: the MI355X (gfx950) parallel-in-time IEKS ODE filter.
+ *
+ * The reference (nathanaelbosch/parallel-in-time-ode-filters) is pure Python
+ * over JAX and has no FFI of its own; its drop-in surface is the Python
+ * function pof.solver.solve (pof/solver.py:11-73).  Each entry point below is
+ * the native counterpart of one reference function on that path; the Python
+ * package `pof` in this repository binds them with ctypes (see
+ * INTEGRATION.md) and keeps the reference's names and argument meanings.
+ *
+ * Conventions: plain pointers and sizes only.  Arrays named *_dev are device
+ * pointers (row-major float64), arrays named *_host are host pointers.
+ * Every function returns POF_OK (0) or a negative POF_E_* status; the message
+ * of the last failure on a context is pof_last_error(ctx).  Work is enqueued
+ * on `stream` (a hipStream_t; NULL = the default stream) and is complete when
+ * the call returns (scalar results are read back).  A context is not
+ * thread-safe: use one per host thread.
+ */
+#ifndef POF_H_
+#define POF_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define POF_OK 0
+#define POF_E_ARG -1        /* invalid argument (shape, kind, size) */
+#define POF_E_UNSUPPORTED -2 /* no kernel instantiation for (D, d) */
+#define POF_E_HIP -3        /* HIP runtime error */
+#define POF_E_INIT -4       /* unknown initial-trajectory method */
+
+/* ODE vector fields available on the device (pof/ivp.py of the reference,
+ * plus fhn_classic, lorenz63, lorenz96). */
+#define POF_LOGISTIC 1
+#define POF_LOTKAVOLTERRA 2
+#define POF_VANDERPOL 3
+#define POF_FITZHUGHNAGUMO 4
+#define POF_FHN_CLASSIC 5
+#define POF_ROBER 6
+#define POF_RIGID_BODY 7
+#define POF_SEIR 8
+#define POF_THREEBODY 9
+#define POF_HENONHEILES 10
+#define POF_LORENZ63 11
+#define POF_LORENZ96 12
+
+/* initial trajectory (pof/convenience.py:76-92) */
+#define POF_INIT_PRIOR 0
+#define POF_INIT_CONSTANT 1
+
+#define POF_MAX_PARAMS 8
+
+/* solve status */
+#define POF_STATUS_CONVERGED 0 /* crit() fired (obj or means) */
+#define POF_STATUS_MAXITER 1   /* k > maxiters */
+#define POF_STATUS_NAN 2       /* obj or nll is NaN */
+#define POF_STATUS_FIXED 3     /* ran exactly fixed_iters iterations */
+
+typedef struct pof_ctx pof_ctx;
+
+/* One IVP y' = f(y) on the grid ts (pof/solver.py:11-13 arguments f, y0, ts, order, init). */
+typedef struct {
+  int32_t kind;                    /* POF_LOGISTIC ... */
+  int32_t d;                       /* ODE dimension (len(y0)) */
+  int32_t q;                       /* order = number of derivatives of the IWP prior, >= 1 */
+  int32_t init;                    /* POF_INIT_PRIOR / POF_INIT_CONSTANT */
+  int64_t n_times;                 /* len(ts) = Ns + 1 >= 2 */
+  const double* ts_host;           /* [n_times]; uniform grid assumed, dt = ts[1] - ts[0] */
+  const double* y0_host;           /* [d] */
+  double params[POF_MAX_PARAMS];   /* field parameters (see pof/ivp.py factories) */
+} pof_problem;
+
+typedef struct {
+  int32_t maxiters;    /* the loop runs while k <= maxiters (reference semantics) */
+  int32_t fixed_iters; /* > 0: run exactly this many iterations, no convergence test */
+  int32_t calibrate;   /* 1: returned factors scaled by sqrt(ssq) twice (solver.py:66-68 + step.py:42-44) */
+  int32_t reserved;
+} pof_opts;
+
+typedef struct {
+  /* caller-owned device outputs (NULL = not wanted) */
+  double* means_dev;       /* [n_times, d]     E0 m_t                    */
+  double* chols_dev;       /* [n_times, d, D]  E0 L_t (calibrated)        */
+  double* full_means_dev;  /* [n_times, D]     preconditioned states      */
+  double* full_chols_dev;  /* [n_times, D, D]  preconditioned factors     */
+  /* info (pof/solver.py:58-64) */
+  double iterations;
+  double nll;
+  double obj;
+  double sigma_squared;
+  int32_t calibrated;
+  int32_t status;          /* POF_STATUS_* */
+} pof_out;
+
+/* Context: device binding and reusable device workspace. */
+int pof_ctx_create(int device, pof_ctx** out);
+void pof_ctx_destroy(pof_ctx* ctx);
+const char* pof_last_error(const pof_ctx* ctx);
+
+/* 1 if a kernel instantiation exists for state dimension D = d (q + 1) and ODE dimension d. */
+int pof_supported(int D, int d);
+
+/* Full parallel IEKS solve.  Replaces pof.solver.solve (pof/solver.py:11-73). */
+int pof_solve(pof_ctx* ctx, const pof_problem* prob, const pof_opts* opts, pof_out* out, void* stream);
+
+/* Solver set-up on the host.  Replaces pof.convenience.set_up_solver (pof/convenience.py:13-45):
+ * F, QL [D, D] (transitions.py:37-50), P, PI diagonals [D] (transitions.py:53-68),
+ * x0 mean [D] and factor [D, D] (preconditioned Taylor-mode init, initialization.py:15-22,
+ * convenience.py:30-31).  Any output may be NULL. */
+int pof_setup(const pof_problem* prob, double* F_host, double* QL_host, double* P_host, double* PI_host,
+              double* x0_mean_host, double* x0_chol_host);
+
+/* IWP prior of order q in d dimensions.  Replaces pof.transitions.preconditioned_discretize
+ * and nordsieck_preconditioner (transitions.py:37-68): F, QL [D, D]; P, PI diagonals [D]. */
+int pof_transitions(int d, int q, double dt, double* F_host, double* QL_host, double* P_host, double* PI_host);
+
+/* Initial trajectory means [n_times, D] on the host.  Replaces
+ * pof.convenience.get_initial_trajectory (convenience.py:76-92) for "constant"
+ * (PI-scaled) and "prior" (not PI-scaled, steps = ts[1:]). */
+int pof_initial_means(const pof_problem* prob, double* means_host);
+
+/* Vector field f(y) at one point on the host (pof/ivp.py f(None, y)). */
+int pof_eval_field(int kind, int d, const double* params, const double* y_host, double* f_host);
+
+/* EK1 linearisation of r(x) = E1 x - f(E0 x) at n device points m [n, D].
+ * Replaces pof.step.linearize_at_previous_states / pof.observations.linearize
+ * (step.py:12-22, observations.py:35-40): H [n, d, D], c [n, d], cholR = 0. */
+int pof_linearize(pof_ctx* ctx, const pof_problem* prob, const double* m_dev, int64_t n, double* H_dev,
+                  double* c_dev, void* stream);
+
+/* One IEKS iteration.  Replaces pof.step.ieks_step (step.py:33-45): linearise at
+ * states_dev [Ns+1, D], parallel filter + smoother, chol *= sqrt(ssq).
+ * Outputs full states [Ns+1, D] / [Ns+1, D, D] and nll, obj, ssq. */
+int pof_ieks_step(pof_ctx* ctx, const pof_problem* prob, const double* states_mean_dev, double* out_mean_dev,
+                  double* out_chol_dev, double* nll, double* obj, double* ssq, void* stream);
+
+/* Noiseless linear filter + smoother with a constant transition (F, QL) and
+ * observations y_k = H_k x_{k+1} + c_k = 0, k < Ns.  Replaces
+ * pof.parallel_filtsmooth.linear_filtsmooth (parallel_filtsmooth/__init__.py:5-10,
+ * filter.py:18-47, smoother.py:8-22).  x0 mean [D] / factor [D, D], F/QL [D, D]
+ * on the host; H [Ns, d, D], c [Ns, d] and outputs on the device.  filt_* (may be
+ * NULL) receive the filtered states [Ns+1, ...]. */
+int pof_linear_filtsmooth(pof_ctx* ctx, int D, int d, int64_t Ns, const double* x0_mean_host,
+                          const double* x0_chol_host, const double* F_host, const double* QL_host,
+                          const double* H_dev, const double* c_dev, double* smooth_mean_dev,
+                          double* smooth_chol_dev, double* filt_mean_dev, double* filt_chol_dev, double* nll,
+                          double* obj, double* ssq, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* POF_H_ */

@@ -1,0 +1,739 @@
+// Device building blocks of the square-root parallel IEKS on CDNA (gfx950).
+//
+// Execution model: one "worker" of W lanes (W = 32 for D <= 16, 64 for
+// D <= 32; two or one workers per 64-wide wavefront) owns one scan element.
+// Lane r of a worker holds row r of every matrix of the element in registers
+// (row-per-lane layout), so the row-local parts of products, triangular
+// solves and Householder updates need no communication.  Rows that other
+// lanes need are staged through the worker's slice of LDS; a worker never
+// synchronises with other workers, only with its own wavefront.
+//
+// Arithmetic follows the reference (pof/parallel_filtsmooth/filter.py:50-142,
+// smoother.py:37-63, utils.py:33-41) in float64.  tria() is a Householder LQ
+// with LAPACK dgeqrf/dlarfg's sign convention (R_ii = -sign(alpha) * norm,
+// tau = 0 when the row tail is zero), because the reference's
+// sigma_squared (whiten() solves L^T r = m, pof/utils.py:110-112) depends on
+// the column signs of the factors it produces.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "pof_problems.hpp"
+#include "pof_types.hpp"
+
+namespace pof {
+
+__device__ __forceinline__ void wsync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  asm volatile("" ::: "memory");
+}
+
+// ------------------------------------------------------------------ layout
+template <int D_>
+struct Geo {
+  static constexpr int D = D_;
+  static constexpr int W = (2 * D_ <= 32) ? 32 : 64;      // lanes per worker
+  static constexpr int LD = D_ + 1;                      // padded LDS row stride
+  static constexpr int TILE = D_ * LD;
+  static constexpr int NT = 6;                           // tiles per worker
+  static constexpr int NV = 8;                           // broadcast vectors per worker
+  static constexpr int BC = 2 * D_ + 2;                  // Householder broadcast row
+  static constexpr int WORKER = BC + NV * 2 * D_ + NT * TILE;
+  static constexpr int CONSTS = 3 * TILE;                // F, LQ, N1z staged per block
+  static constexpr size_t bytes_for(int wpb) { return sizeof(double) * (CONSTS + wpb * WORKER); }
+  // workers per block: as many as fit in 64 KiB of LDS, at most 256 threads
+  static constexpr int WPB = (W == 32 && bytes_for(4) <= 65536) ? 4 : (bytes_for(2) <= 65536 ? 2 : 1);
+  static constexpr int BLOCK = W * WPB;
+  static constexpr size_t lds_bytes() { return bytes_for(WPB); }
+  static_assert(bytes_for(WPB) <= 65536, "state dimension too large for one worker per block");
+};
+
+template <int D>
+struct Ctx {
+  using G = Geo<D>;
+  int lane;           // lane inside the worker
+  int wid;            // global worker index
+  double* w;          // worker LDS slice
+  const double* sF;   // block LDS constants, row stride LD
+  const double* sLQ;
+  const double* sN1;
+  __device__ double* tile(int i) const { return w + G::BC + G::NV * 2 * D + i * G::TILE; }
+  __device__ double* vec(int i) const { return w + G::BC + i * 2 * D; }
+  __device__ double* bc() const { return w; }
+};
+
+// Stage F, LQ, N1z into block LDS and carve out this thread's worker slice.
+// Must be called by every thread of the block before any early exit.
+template <int D>
+__device__ __forceinline__ Ctx<D> make_ctx(const ModelArgs& a, double* smem) {
+  using G = Geo<D>;
+  for (int t = threadIdx.x; t < D * D; t += G::BLOCK) {
+    const int r = t / D, c = t % D;
+    smem[r * G::LD + c] = a.F[t];
+    smem[G::TILE + r * G::LD + c] = a.LQ[t];
+    smem[2 * G::TILE + r * G::LD + c] = a.N1z[t];
+  }
+  __syncthreads();
+  Ctx<D> cx;
+  cx.lane = threadIdx.x % G::W;
+  const int wl = threadIdx.x / G::W;
+  cx.wid = blockIdx.x * G::WPB + wl;
+  cx.w = smem + G::CONSTS + wl * G::WORKER;
+  cx.sF = smem;
+  cx.sLQ = smem + G::TILE;
+  cx.sN1 = smem + 2 * G::TILE;
+  return cx;
+}
+
+// ----------------------------------------------------------------- helpers
+// Opaque copy of a value: keeps the optimiser from correlating the many lane
+// tests of one function (which otherwise duplicates whole regions per branch
+// and multiplies register pressure).
+__device__ __forceinline__ int opaque(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
+// Pin accumulators at a point of the instruction stream.  Within one basic
+// block the instruction selector is free to emit every load of an unrolled
+// product first and the arithmetic last, which keeps each loaded value in a
+// register pair until the end (hundreds of registers, then spills).  Passing
+// the accumulators through an empty volatile asm after each step forces the
+// arithmetic of that step to be emitted before the loads of the next one.
+template <int N>
+__device__ __forceinline__ void pin(double (&x)[N]) {
+#pragma unroll
+  for (int i = 0; i < N; ++i) asm volatile("" : "+v"(x[i])::"memory");
+}
+__device__ __forceinline__ void pin1(double& x) { asm volatile("" : "+v"(x)::"memory"); }
+
+template <int N>
+__device__ __forceinline__ void zero(double (&x)[N]) {
+#pragma unroll
+  for (int i = 0; i < N; ++i) x[i] = 0.0;
+}
+
+template <int N>
+__device__ __forceinline__ void load_row(double (&x)[N], const double* p) {
+#pragma unroll
+  for (int i = 0; i < N; ++i) x[i] = p[i];
+}
+
+template <int N>
+__device__ __forceinline__ void store_row(double* p, const double (&x)[N]) {
+#pragma unroll
+  for (int i = 0; i < N; ++i) p[i] = x[i];
+}
+
+// out[c] = sum_k x[k] * T[k][c]   (row of X @ T), T with row stride LD
+template <int K, int NC, int LD, int NX>
+__device__ __forceinline__ void row_times_tile(const double (&x)[NX], const double* T, double (&out)[NC]) {
+#pragma unroll
+  for (int c = 0; c < NC; ++c) out[c] = 0.0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) out[c] = fma(x[k], T[k * LD + c], out[c]);
+    pin(out);
+  }
+}
+
+// out[c] = sum_k T1[k][r] * T2[k][c]   (row r of T1^T @ T2)
+template <int K, int NC, int LD>
+__device__ __forceinline__ void tile_t_times_tile(int r, const double* T1, const double* T2, double (&out)[NC]) {
+#pragma unroll
+  for (int c = 0; c < NC; ++c) out[c] = 0.0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const double t = T1[k * LD + r];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) out[c] = fma(t, T2[k * LD + c], out[c]);
+    pin(out);
+  }
+}
+
+// Householder LQ of the NR x NC matrix whose row r sits in lane r (r < NR):
+// on return lane r holds row r of the lower-triangular factor L (columns
+// >= min(NR, NC) are zero), L L^T = M M^T.  Same reflectors as LAPACK dgeqrf
+// on M^T.
+template <int NR, int NC>
+__device__ __forceinline__ void lq(double (&a)[NC], int lane_in, double* bc) {
+  constexpr int K = (NR < NC) ? NR : NC;
+  // opaque copy of the lane index: stops the optimiser from threading the
+  // per-pivot lane tests into specialised copies of the surrounding code
+  int lane = lane_in;
+  asm volatile("" : "+v"(lane));
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    if (i == NC - 1) break;  // a length-1 tail: H = I
+    if (lane == i) {
+      const double alpha = a[i];
+      double sig = 0.0;
+#pragma unroll
+      for (int j = i + 1; j < NC; ++j) sig = fma(a[j], a[j], sig);
+      double tau = 0.0, beta = alpha, scal = 0.0;
+      if (sig != 0.0) {
+        const double nrm = sqrt(fma(alpha, alpha, sig));
+        beta = (alpha >= 0.0) ? -nrm : nrm;
+        tau = (beta - alpha) / beta;
+        scal = 1.0 / (alpha - beta);
+      }
+      bc[i] = tau;
+#pragma unroll
+      for (int j = i + 1; j < NC; ++j) bc[j] = a[j] * scal;
+      a[i] = beta;
+#pragma unroll
+      for (int j = i + 1; j < NC; ++j) a[j] = 0.0;
+    }
+    wsync();
+    if (lane > i && lane < NR) {
+      const double tau = bc[i];
+      double w = a[i];
+#pragma unroll
+      for (int j = i + 1; j < NC; ++j) w = fma(a[j], bc[j], w);
+      const double tw = tau * w;
+      a[i] -= tw;
+#pragma unroll
+      for (int j = i + 1; j < NC; ++j) a[j] = fma(-tw, bc[j], a[j]);
+    }
+    wsync();
+  }
+}
+
+// y L = v with L lower triangular (n x n in LDS, stride LD); y holds v on entry.
+template <int n, int LD, int NY>
+__device__ __forceinline__ void rsolve_lower(double (&y)[NY], const double* L) {
+#pragma unroll
+  for (int j = n - 1; j >= 0; --j) {
+    double s = y[j];
+#pragma unroll
+    for (int l = j + 1; l < n; ++l) s = fma(-y[l], L[l * LD + j], s);
+    y[j] = s / L[j * LD + j];
+    pin1(y[j]);
+  }
+}
+
+// y L^T = v with L lower triangular (forward substitution); y holds v on entry.
+template <int n, int LD, int NY>
+__device__ __forceinline__ void rsolve_lowerT(double (&y)[NY], const double* L) {
+#pragma unroll
+  for (int j = 0; j < n; ++j) {
+    double s = y[j];
+#pragma unroll
+    for (int l = 0; l < j; ++l) s = fma(-y[l], L[j * LD + l], s);
+    y[j] = s / L[j * LD + j];
+    pin1(y[j]);
+  }
+}
+
+// x = L^-1 v (forward) with L lower; x holds v on entry.
+template <int n, int LD, int NY>
+__device__ __forceinline__ void lsolve_lower(double (&x)[NY], const double* L) {
+  rsolve_lowerT<n, LD, NY>(x, L);  // L x = v  <=>  x^T L^T = v^T
+}
+
+// x = L^-T v (backward) with L lower; x holds v on entry.
+template <int n, int LD, int NY>
+__device__ __forceinline__ void lsolve_lowerT(double (&x)[NY], const double* L) {
+  rsolve_lower<n, LD, NY>(x, L);   // L^T x = v  <=>  x^T L = v^T
+}
+
+// ---------------------------------------------------------- observation model
+// Affine observation of element k (noiseless): y = H x + c, either from EK1 at
+// lin[k+1] (pof/step.py:12-22, pof/observations.py:35-40: H = E1 - J E0,
+// c = J E0 m - f(E0 m)) or from explicit arrays.  Every lane computes the
+// whole (small) H and c.
+template <int D, int d>
+__device__ __forceinline__ void observation(const ModelArgs& a, int k, double (&H)[d][D], double (&c)[d]) {
+  if (a.lin != nullptr) {
+    constexpr int q1 = D / d;
+    const double* m = a.lin + (size_t)(k + 1) * D;
+    double y[d];
+#pragma unroll
+    for (int i = 0; i < d; ++i) y[i] = a.Pd[i * q1] * m[i * q1];
+    double f[d];
+    double J[d][d];
+    field_and_jacobian<d>(a.kind, a.params, y, f, J);
+#pragma unroll
+    for (int i = 0; i < d; ++i) {
+#pragma unroll
+      for (int j = 0; j < D; ++j) H[i][j] = 0.0;
+      double acc = 0.0;
+#pragma unroll
+      for (int l = 0; l < d; ++l) {
+        H[i][l * q1] = -J[i][l] * a.Pd[l * q1];
+        acc = fma(J[i][l], y[l], acc);
+      }
+      H[i][i * q1 + 1] += a.Pd[i * q1 + 1];
+      c[i] = acc - f[i];
+    }
+  } else {
+    const double* Hk = a.Hx + (size_t)k * d * D;
+#pragma unroll
+    for (int i = 0; i < d; ++i) {
+#pragma unroll
+      for (int j = 0; j < D; ++j) H[i][j] = Hk[i * D + j];
+      c[i] = a.cx[(size_t)k * d + i];
+    }
+  }
+}
+
+// select row `lane` (< d) of a register matrix without dynamic indexing
+template <int d, int D>
+__device__ __forceinline__ void select_row(const double (&H)[d][D], int lane, double (&h)[D]) {
+#pragma unroll
+  for (int j = 0; j < D; ++j) h[j] = H[0][j];
+#pragma unroll
+  for (int i = 1; i < d; ++i) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) h[j] = (lane == i) ? H[i][j] : h[j];
+  }
+}
+
+// ----------------------------------------------------------- filter element
+// _get_element (pof/parallel_filtsmooth/filter.py:50-81) for element k with
+// prior (x0m, x0L) when first, else the zero prior.  Lane r < D returns row r.
+template <int D, int d>
+__device__ void filter_element(const Ctx<D>& cx, const ModelArgs& a, int k, bool first, double (&A)[D],
+                               double& b, double (&U)[D], double& eta, double (&Z)[D]) {
+  using G = Geo<D>;
+  constexpr int LD = G::LD;
+  const int r = opaque(cx.lane);
+  double H[d][D];
+  double cv[d];
+  observation<D, d>(a, k, H, cv);
+
+  double* T0 = cx.tile(0);
+  double* T1 = cx.tile(1);
+  double* T2 = cx.tile(2);
+  double* T3 = cx.tile(3);
+  double* V0 = cx.vec(0);
+  const double* N1 = cx.sN1;
+  double m1r = 0.0;
+  if (first) {
+    // m1 = F m0, N1 = tria([F L0 | LQ])
+    double row[2 * D];
+    if (r < D) {
+      double fr[D];
+      load_row(fr, cx.sF + r * LD);
+      double fl[D];
+      zero(fl);
+#pragma unroll
+      for (int k2 = 0; k2 < D; ++k2) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) fl[c] = fma(fr[k2], a.x0L[k2 * D + c], fl[c]);
+        m1r = fma(fr[k2], a.x0m[k2], m1r);
+        pin(fl);
+      }
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        row[c] = fl[c];
+        row[D + c] = cx.sLQ[r * LD + c];
+      }
+    } else {
+      zero(row);
+    }
+    lq<D, 2 * D>(row, r, cx.bc());
+    if (r < D) {
+#pragma unroll
+      for (int c = 0; c < D; ++c) T0[r * LD + c] = row[c];
+      V0[r] = m1r;
+    }
+    wsync();
+    N1 = T0;
+  }
+
+  // Psi_ = [[H N1, R], [N1, 0]]  (R = 0), rows in lanes 0..d+D-1
+  double psi[D + d];
+  {
+    double h[D];
+    select_row<d, D>(H, r, h);
+    if (r < d) {
+      double hn[D];
+      row_times_tile<D, D, LD>(h, N1, hn);
+#pragma unroll
+      for (int c = 0; c < D; ++c) psi[c] = hn[c];
+    } else if (r < d + D) {
+#pragma unroll
+      for (int c = 0; c < D; ++c) psi[c] = N1[(r - d) * LD + c];
+    } else {
+#pragma unroll
+      for (int c = 0; c < D; ++c) psi[c] = 0.0;
+    }
+#pragma unroll
+    for (int c = 0; c < d; ++c) psi[D + c] = 0.0;
+  }
+  lq<d + D, D + d>(psi, r, cx.bc());
+
+  // Psi11 -> T1
+  if (r < d) {
+#pragma unroll
+    for (int c = 0; c < d; ++c) T1[r * LD + c] = psi[c];
+  }
+  wsync();
+  // K row = Psi21 row Psi11^-1 (lanes d..d+D-1), then move K and U rows to lane r - d
+  if (r >= d && r < d + D) {
+    double kk[d];
+#pragma unroll
+    for (int c = 0; c < d; ++c) kk[c] = psi[c];
+    rsolve_lower<d, LD>(kk, T1);
+#pragma unroll
+    for (int c = 0; c < d; ++c) T3[(r - d) * LD + c] = kk[c];
+#pragma unroll
+    for (int c = 0; c < D; ++c) T2[(r - d) * LD + c] = psi[d + c];
+  }
+  wsync();
+
+  // HF = H F (every lane), hm1 = H m1 + c
+  double HF[d][D];
+  double hm1[d];
+#pragma unroll
+  for (int i = 0; i < d; ++i) {
+#pragma unroll
+    for (int c = 0; c < D; ++c) HF[i][c] = 0.0;
+    hm1[i] = cv[i];
+  }
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    double fj[D];
+    load_row(fj, cx.sF + j * LD);
+    const double m1j = first ? V0[j] : 0.0;
+#pragma unroll
+    for (int i = 0; i < d; ++i) {
+#pragma unroll
+      for (int c = 0; c < D; ++c) HF[i][c] = fma(H[i][j], fj[c], HF[i][c]);
+      hm1[i] = fma(H[i][j], m1j, hm1[i]);
+      pin(HF[i]);
+    }
+  }
+  pin(hm1);
+  // w = Psi11^-1 c
+  double wv[d];
+#pragma unroll
+  for (int i = 0; i < d; ++i) wv[i] = cv[i];
+  lsolve_lower<d, LD>(wv, T1);
+
+  const int rr = (r < D) ? r : 0;
+  {
+    double kr[d];
+#pragma unroll
+    for (int i = 0; i < d; ++i) kr[i] = T3[rr * LD + i];
+    load_row(U, T2 + rr * LD);
+    // A = F - K H F ; b = m1 - K (H m1 + c)
+    double fr[D];
+    load_row(fr, cx.sF + rr * LD);
+    double bb = m1r;
+#pragma unroll
+    for (int c = 0; c < D; ++c) A[c] = fr[c];
+#pragma unroll
+    for (int i = 0; i < d; ++i) {
+#pragma unroll
+      for (int c = 0; c < D; ++c) A[c] = fma(-kr[i], HF[i][c], A[c]);
+      bb = fma(-kr[i], hm1[i], bb);
+    }
+    b = bb;
+  }
+  // Z row = (Psi11^-1 HF[:, r])^T ; eta = -Z Psi11^-1 c
+  double z[d];
+#pragma unroll
+  for (int i = 0; i < d; ++i) {
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) acc = fma(H[i][j], cx.sF[j * LD + rr], acc);
+    z[i] = acc;
+    pin1(z[i]);
+  }
+  lsolve_lower<d, LD>(z, T1);
+  {
+    double e = 0.0;
+#pragma unroll
+    for (int i = 0; i < d; ++i) e = fma(-z[i], wv[i], e);
+    eta = e;
+  }
+  if constexpr (D > d) {
+#pragma unroll
+    for (int c = 0; c < D; ++c) Z[c] = (c < d) ? z[(c < d) ? c : 0] : 0.0;
+  } else {
+    // nx <= ny: Z = tria(Z) with Z of shape D x d
+    lq<D, d>(z, r, cx.bc());
+#pragma unroll
+    for (int c = 0; c < D; ++c) Z[c] = z[c];
+  }
+  wsync();
+}
+
+// ---------------------------------------------------------- filter operators
+// Full combine (pof/parallel_filtsmooth/filter.py:117-142): e = e1 o e2, e1
+// earlier.  Reads e1 at index i1 of `x1`, e2 at index i2 of `x2`, writes index
+// o of `y`.  If `state_only`, e1 is a prefix state (b1, U1) read from
+// (s1.m, s1.L) and only (b, U) of the result are produced, into (so.m, so.L).
+template <int D, bool state_only>
+__device__ void filter_combine(const Ctx<D>& cx, const FiltArrays& x1, const StateArrays& s1, size_t i1,
+                               const FiltArrays& x2, size_t i2, const FiltArrays& y, const StateArrays& so,
+                               size_t o) {
+  using G = Geo<D>;
+  constexpr int LD = G::LD;
+  const int r = opaque(cx.lane);
+  const bool act = r < D;
+  const int rr = act ? r : 0;  // clamped row for branch-free loads
+  double* TU1 = cx.tile(0);
+  double* TZ2 = cx.tile(1);
+  double* TX11 = cx.tile(2);
+  double* TX21 = cx.tile(3);
+  double* TX22 = cx.tile(4);
+  double* TA1 = cx.tile(5);
+  double* Veta2 = cx.vec(0);
+  double* Vb1 = cx.vec(1);
+  double* Vu = cx.vec(2);
+  double* Vs = cx.vec(3);
+
+  const double* U1p = state_only ? s1.L : x1.U;
+  const double* b1p = state_only ? s1.m : x1.b;
+  const double b1r = b1p[i1 * D + rr];
+  const double eta2r = x2.eta[i2 * D + rr];
+  {
+    double u1[D], z2[D];
+    load_row(u1, U1p + (i1 * D + rr) * D);
+    load_row(z2, x2.Z + (i2 * D + rr) * D);
+    if (act) {
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        TU1[r * LD + c] = u1[c];
+        TZ2[r * LD + c] = z2[c];
+      }
+      Vb1[r] = b1r;
+      Veta2[r] = eta2r;
+    }
+  }
+  wsync();
+  // Xi_ = [[U1^T Z2, I], [Z2, 0]]
+  double xr[2 * D];
+  if (r < D) {
+    double t[D];
+    tile_t_times_tile<D, D, LD>(r, TU1, TZ2, t);
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      xr[c] = t[c];
+      xr[D + c] = (c == r) ? 1.0 : 0.0;
+    }
+  } else if (r < 2 * D) {
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      xr[c] = TZ2[(r - D) * LD + c];
+      xr[D + c] = 0.0;
+    }
+  } else {
+    zero(xr);
+  }
+  lq<2 * D, 2 * D>(xr, r, cx.bc());
+  if (r < D) {
+#pragma unroll
+    for (int c = 0; c < D; ++c) TX11[r * LD + c] = xr[c];
+  } else if (r < 2 * D) {
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      TX21[(r - D) * LD + c] = xr[c];
+      TX22[(r - D) * LD + c] = xr[D + c];
+    }
+  }
+  wsync();
+
+  // MT = A2 U1 Xi11^-T ; W = A2 - MT Xi21^T
+  double a2[D], mt[D], wr[D];
+  load_row(a2, x2.A + (i2 * D + rr) * D);
+  row_times_tile<D, D, LD>(a2, TU1, mt);
+  rsolve_lowerT<D, LD>(mt, TX11);
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    double acc = a2[c];
+#pragma unroll
+    for (int j = 0; j < D; ++j) acc = fma(-mt[j], TX21[c * LD + j], acc);
+    wr[c] = acc;
+    pin1(wr[c]);
+  }
+
+  // b = W (b1 + U1 U1^T eta2) + b2
+  {
+    double u = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) u = fma(TU1[k * LD + rr], Veta2[k], u);
+    if (act) Vu[r] = u;
+    wsync();
+    double sv = b1r;
+#pragma unroll
+    for (int j = 0; j < D; ++j) sv = fma(TU1[rr * LD + j], Vu[j], sv);
+    if (act) Vs[r] = sv;
+    wsync();
+    double bb = x2.b[i2 * D + rr];
+#pragma unroll
+    for (int k = 0; k < D; ++k) bb = fma(wr[k], Vs[k], bb);
+    double* bo = state_only ? so.m : y.b;
+    if (act) bo[o * D + r] = bb;
+  }
+  // U = tria([MT | U2])
+  {
+    double ur[2 * D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) ur[c] = mt[c];
+#pragma unroll
+    for (int c = 0; c < D; ++c) ur[D + c] = x2.U[(i2 * D + rr) * D + c];
+    lq<D, 2 * D>(ur, r, cx.bc());
+    if (act) {
+      double* Uo = state_only ? so.L : y.U;
+#pragma unroll
+      for (int c = 0; c < D; ++c) Uo[(o * D + r) * D + c] = ur[c];
+    }
+  }
+  if constexpr (!state_only) {
+    double* Vt = cx.vec(4);
+    double* Vs2 = cx.vec(5);
+    double* Vu2 = cx.vec(6);
+    double* Vs3 = cx.vec(7);
+    {
+      double a1[D];
+      load_row(a1, x1.A + (i1 * D + rr) * D);
+      if (act) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) TA1[r * LD + c] = a1[c];
+      }
+    }
+    wsync();
+    // A = W A1
+    if (act) {
+      double ar[D];
+      row_times_tile<D, D, LD>(wr, TA1, ar);
+      store_row(y.A + (o * D + r) * D, ar);
+    }
+    // eta = A1^T (I - Xi21 Xi11^-1 U1^T)(eta2 - Z2 Z2^T b1) + eta1
+    if (act) {
+      double t = 0.0;
+#pragma unroll
+      for (int k = 0; k < D; ++k) t = fma(TZ2[k * LD + r], Vb1[k], t);
+      Vt[r] = t;
+    }
+    wsync();
+    double s2 = 0.0;
+    if (act) {
+      s2 = eta2r;
+#pragma unroll
+      for (int j = 0; j < D; ++j) s2 = fma(-TZ2[r * LD + j], Vt[j], s2);
+      Vs2[r] = s2;
+    }
+    wsync();
+    if (act) {
+      double u2 = 0.0;
+#pragma unroll
+      for (int k = 0; k < D; ++k) u2 = fma(TU1[k * LD + r], Vs2[k], u2);
+      Vu2[r] = u2;
+    }
+    wsync();
+    double w2[D];
+    load_row(w2, Vu2);
+    lsolve_lower<D, LD>(w2, TX11);
+    if (act) {
+      double v2 = 0.0;
+#pragma unroll
+      for (int j = 0; j < D; ++j) v2 = fma(TX21[r * LD + j], w2[j], v2);
+      Vs3[r] = s2 - v2;
+    }
+    wsync();
+    if (act) {
+      double e = x1.eta[i1 * D + rr];
+#pragma unroll
+      for (int k = 0; k < D; ++k) e = fma(TA1[k * LD + r], Vs3[k], e);
+      y.eta[o * D + r] = e;
+    }
+    // Z = tria([A1^T Xi22 | Z1])
+    double zr[2 * D];
+    {
+      double t[D];
+      tile_t_times_tile<D, D, LD>(rr, TA1, TX22, t);
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        zr[c] = t[c];
+        zr[D + c] = x1.Z[(i1 * D + rr) * D + c];
+      }
+    }
+    lq<D, 2 * D>(zr, r, cx.bc());
+    if (act) store_row(y.Z + (o * D + r) * D, *reinterpret_cast<double(*)[D]>(zr));
+  }
+  wsync();
+}
+
+// ------------------------------------------------------- smoother operators
+// (pof/parallel_filtsmooth/smoother.py:53-63) under reverse=True: e1 is the
+// aggregate later in time.  full: (g, E, D) = (E2 g1 + g2, E2 E1,
+// tria([E2 D1 | D2])).  state_only: (m, L) = (E2 m1 + g2, tria([E2 L1 | D2])).
+// with_chol = false skips every D/L term (means-only iterations).
+template <int D, bool state_only>
+__device__ void smoother_combine(const Ctx<D>& cx, const SmoArrays& x1, const StateArrays& s1, size_t i1,
+                                 const SmoArrays& x2, size_t i2, const SmoArrays& y, const StateArrays& so,
+                                 size_t o, bool with_chol) {
+  using G = Geo<D>;
+  constexpr int LD = G::LD;
+  const int r = opaque(cx.lane);
+  const bool act = r < D;
+  const int rr = act ? r : 0;
+  double* TE1 = cx.tile(0);
+  double* TD1 = cx.tile(1);
+  double* Vg1 = cx.vec(0);
+  const double* g1p = state_only ? s1.m : x1.g;
+  const double* D1p = state_only ? s1.L : x1.Dm;
+  {
+    const double g1 = g1p[i1 * D + rr];
+    if (act) Vg1[r] = g1;
+    if (!state_only) {
+      double e1[D];
+      load_row(e1, x1.E + (i1 * D + rr) * D);
+      if (act) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) TE1[r * LD + c] = e1[c];
+      }
+    }
+    if (with_chol) {
+      double d1[D];
+      load_row(d1, D1p + (i1 * D + rr) * D);
+      if (act) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) TD1[r * LD + c] = d1[c];
+      }
+    }
+  }
+  wsync();
+  double e2[D];
+  load_row(e2, x2.E + (i2 * D + rr) * D);
+  if (act) {
+    double gg = x2.g[i2 * D + r];
+#pragma unroll
+    for (int k = 0; k < D; ++k) gg = fma(e2[k], Vg1[k], gg);
+    (state_only ? so.m : y.g)[o * D + r] = gg;
+    if (!state_only) {
+      double er[D];
+      row_times_tile<D, D, LD>(e2, TE1, er);
+      store_row(y.E + (o * D + r) * D, er);
+    }
+  }
+  if (with_chol) {
+    double dr[2 * D];
+    {
+      double t[D];
+      row_times_tile<D, D, LD>(e2, TD1, t);
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        dr[c] = t[c];
+        dr[D + c] = x2.Dm[(i2 * D + rr) * D + c];
+      }
+    }
+    lq<D, 2 * D>(dr, r, cx.bc());
+    if (act) {
+      double* Do = state_only ? so.L : y.Dm;
+#pragma unroll
+      for (int c = 0; c < D; ++c) Do[(o * D + r) * D + c] = dr[c];
+    }
+  }
+  wsync();
+}
+
+}  // namespace pof

@@ -1,0 +1,63 @@
+"""Public API: the parallel IEKS ODE solve (reference ``pof/solver.py``)."""
+import ctypes
+
+from . import _lib
+from .convenience import ProblemSpec
+from .utils import MVNSqrt
+
+
+def solve(*, f, y0, ts, order, init="prior", calibrate=True, maxiters=10_000, sequential=False,
+          fixed_iters=None, return_full_states=False):
+    """Parallel-in-time IEKS solve on the GPU (``pof/solver.py:11-73``).
+
+    Same keyword arguments and result as the reference:
+    ``(MVNSqrt(means[len(ts), d], chol_covs[len(ts), d, D]), info)`` with
+    ``info = {"iterations", "nll", "obj", "sigma_squared", "calibrated"}``.
+    ``f`` must come from ``pof.ivp`` (a device vector field).  Extra:
+    ``fixed_iters`` runs exactly that many iterations without the convergence
+    test (benchmark mode); ``return_full_states`` also returns the
+    preconditioned states in ``info["states"]``.  ``info["status"]`` tells
+    how the loop ended.
+    """
+    import torch
+
+    if sequential:
+        raise NotImplementedError("sequential=True is not implemented on the GPU")
+    spec = ProblemSpec(f, y0, ts, order, init)
+    N, d, D = spec.ts.shape[0], spec.d, spec.D
+    dev = torch.device("cuda", torch.cuda.current_device())
+    means = torch.empty((N, d), dtype=torch.float64, device=dev)
+    chols = torch.empty((N, d, D), dtype=torch.float64, device=dev)
+    fm = torch.empty((N, D), dtype=torch.float64, device=dev) if return_full_states else None
+    fL = torch.empty((N, D, D), dtype=torch.float64, device=dev) if return_full_states else None
+    opts = _lib.Opts()
+    opts.maxiters = int(maxiters)
+    opts.fixed_iters = int(fixed_iters or 0)
+    opts.calibrate = 1 if calibrate else 0
+    out = _lib.Out()
+    out.means_dev = means.data_ptr()
+    out.chols_dev = chols.data_ptr()
+    out.full_means_dev = fm.data_ptr() if fm is not None else None
+    out.full_chols_dev = fL.data_ptr() if fL is not None else None
+    prob = spec.cstruct()
+    ctx = _lib.context(dev.index)
+    _lib.check(
+        _lib.load().pof_solve(ctx, ctypes.byref(prob), ctypes.byref(opts), ctypes.byref(out), _lib.stream_ptr(dev)),
+        ctx, "pof_solve",
+    )
+    info = {
+        "iterations": float(out.iterations),
+        "nll": float(out.nll),
+        "obj": float(out.obj),
+        "sigma_squared": float(out.sigma_squared),
+        "calibrated": bool(out.calibrated),
+        "status": _lib.STATUS.get(out.status, str(out.status)),
+    }
+    if return_full_states:
+        info["states"] = MVNSqrt(fm, fL)
+    return MVNSqrt(means, chols), info
+
+
+def sequential_eks_solve(*, f, y0, ts, order, return_full_states=False, calibrate=True):
+    """Single-pass sequential EKS (``pof/solver.py:76-96``): not implemented on the GPU."""
+    raise NotImplementedError("sequential_eks_solve is not implemented on the GPU")
