@@ -1,0 +1,201 @@
+"""Benchmark: parallel IEKS ODE solves on one MI355X.
+
+Workload (one "step" = one solve): Lorenz-63 (sigma, rho, beta) = (10, 28, 8/3),
+y0 = (1, 1, 1), t in [0, 10], Ns = 65536 uniform steps (len(ts) = 65537),
+IWP order q = 3 (state dimension D = 12), float64 throughout, constant
+initial trajectory, exactly 10 IEKS iterations per solve.  Inputs are tiny
+(y0, ts); everything else lives in HBM.
+
+Prints one JSON line: solves/s over K timed solves, the roofline of the
+dominant kernel class (algorithmic flops per launch over its mean launch
+time, measured with HIP events on the launch stream in an instrumented
+solve after the timed region), and the CPU baseline: the FP64 NumPy
+restatement of the reference algorithm (the oracle) timed on the host on
+one full-size IEKS iteration (solves/s = 1 / (10 x that time)).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+for p in (ROOT / "parallel-in-time-ode-filters_amd", ROOT):
+    if str(p) not in sys.path:
+        sys.path.insert(0, str(p))
+
+METRIC = "ODE solves/s + p50/p99 latency, batched vs plain GPU scan at 1e-6 rel err"
+NS = 65536
+Q = 3
+ITERS = 10
+
+
+def gpu_run(steps, warmup):
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    import pof.ivp
+    import pof.solver
+    from pof import _lib, roofline
+
+    torch.cuda.set_device(0)
+    ivp = pof.ivp.lorenz63()
+    ts = np.linspace(0.0, 10.0, NS + 1)
+
+    def solve():
+        return pof.solver.solve(f=ivp.f, y0=ivp.y0, ts=ts, order=Q, init="constant", fixed_iters=ITERS)
+
+    for _ in range(warmup):
+        solve()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ys, info = solve()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    finite = bool(torch.isfinite(ys.mean).all())
+
+    # instrumented solve: per-kernel-class event timing on the launch stream
+    lib = _lib.load()
+    ctx = _lib.context(0)
+    _lib.check(lib.pof_profile_enable(ctx, 1), ctx, "pof_profile_enable")
+    solve()
+    torch.cuda.synchronize()
+    n = len(roofline.KERNEL_CLASSES)
+    ms = (ctypes.c_double * n)()
+    launches = (ctypes.c_longlong * n)()
+    work = (ctypes.c_longlong * n)()
+    lib.pof_profile_read(ctx, n, ms, launches, work)
+    _lib.check(lib.pof_profile_enable(ctx, 0), ctx, "pof_profile_enable")
+    d, D = 3, 3 * (Q + 1)
+    per_class = {}
+    for i, name in enumerate(roofline.KERNEL_CLASSES):
+        if launches[i] == 0:
+            continue
+        fl = roofline.flops_per_unit(name, D, d) * work[i]
+        by = roofline.bytes_per_unit(name, D, d) * work[i]
+        per_class[name] = {
+            "ms": ms[i],
+            "launches": int(launches[i]),
+            "mean_launch_us": 1e3 * ms[i] / launches[i],
+            "tflops": fl / (ms[i] * 1e-3) / 1e12 if ms[i] > 0 else None,
+            "gbs": by / (ms[i] * 1e-3) / 1e9 if ms[i] > 0 else None,
+        }
+    dom = max(per_class, key=lambda k: per_class[k]["ms"])
+    c = per_class[dom]
+    flops_per_launch = roofline.flops_per_unit(dom, D, d) * work[roofline.KERNEL_CLASSES.index(dom)] / c["launches"]
+    achieved = flops_per_launch / (c["mean_launch_us"] * 1e-6) / 1e12
+    total_ms = sum(v["ms"] for v in per_class.values())
+    iter_flops = roofline.iteration_flops(NS, D, d)
+    return {
+        "seconds": dt,
+        "finite": finite,
+        "iterations": info["iterations"],
+        "sigma_squared": info["sigma_squared"],
+        "per_class": per_class,
+        "dominant": dom,
+        "roofline": {
+            "bound": "mfma",
+            "achieved": achieved,
+            "peak": roofline.PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / roofline.PEAK_FP64_TFLOPS,
+            "traffic": None,
+            "kernel": dom,
+            "flops_per_launch": flops_per_launch,
+            "mean_launch_us": c["mean_launch_us"],
+        },
+        "instrumented_iteration_ms": total_ms / ITERS,
+        "iteration_tflops_algorithmic": iter_flops / (total_ms / ITERS * 1e-3) / 1e12,
+    }
+
+
+def cpu_baseline():
+    """The oracle (NumPy FP64 parallel-scan IEKS) on one full-size iteration, one thread."""
+    import numpy as np
+
+    try:
+        from threadpoolctl import threadpool_limits
+    except ImportError:  # pragma: no cover
+        threadpool_limits = None
+    from oracle import model as om
+    from oracle import problems as op
+    from oracle import solver as osol
+
+    p = op.lorenz63()
+    ts = np.linspace(0.0, 10.0, NS + 1)
+    setup = om.set_up_solver(p, ts, Q)
+    states = om.get_initial_trajectory(setup, "constant")
+
+    def run():
+        t0 = time.perf_counter()
+        osol.ieks_step(setup, states)
+        return time.perf_counter() - t0
+
+    if threadpool_limits is not None:
+        with threadpool_limits(limits=1):
+            t = run()
+    else:
+        t = run()
+    return {
+        "value": 1.0 / (ITERS * t),
+        "unit": "solves/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"one IEKS iteration of the Lorenz-63 Ns={NS} q={Q} f64 workload ({t:.2f} s, BLAS/LAPACK on 1 "
+                  f"thread); solves/s = 1 / ({ITERS} x iteration time)",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--verbose", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part of this build",
+              file=sys.stderr)
+        sys.exit(2)
+    g = gpu_run(args.steps, args.warmup)
+    if args.verbose:
+        print(json.dumps(g["per_class"], indent=1), file=sys.stderr)
+    cpu = None if args.no_cpu_baseline else cpu_baseline()
+    ms_step = 1e3 * g["seconds"] / args.steps
+    out = {
+        "metric": METRIC,
+        "value": args.steps / g["seconds"],
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain parallel IEKS, Lorenz-63 t=[0,10] Ns={NS} q={Q} (D=12), f64, {ITERS} fixed "
+                        f"iterations per solve, constant init",
+            "Ns": NS,
+            "q": Q,
+            "iterations_per_solve": ITERS,
+            "batch": 1,
+        },
+        "ms_per_iteration": ms_step / ITERS,
+        "roofline": g["roofline"],
+        "cpu_baseline": cpu,
+        "kernel_classes_ms_per_solve": {k: round(v["ms"], 4) for k, v in g["per_class"].items()},
+        "finite": g["finite"],
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -101,6 +101,15 @@ const char* pof_last_error(const pof_ctx* ctx);
 /* 1 if a kernel instantiation exists for state dimension D = d (q + 1) and ODE dimension d. */
 int pof_supported(int D, int d);
 
+/* Per-kernel timing with HIP events around every launch of the solve loop
+ * (instrumented runs only; no reference counterpart).  Classes, in order:
+ * filter_leaf, filter_up, filter_down, filter_post, smooth_up, smooth_down,
+ * smooth_post, reduce.  pof_profile_enable(ctx, 1) resets the counters;
+ * pof_profile_read fills up to n_classes entries of milliseconds, launch
+ * counts and elements processed, and returns the number of classes. */
+int pof_profile_enable(pof_ctx* ctx, int on);
+int pof_profile_read(const pof_ctx* ctx, int n_classes, double* ms, long long* launches, long long* work);
+
 /* Full parallel IEKS solve.  Replaces pof.solver.solve (pof/solver.py:11-73). */
 int pof_solve(pof_ctx* ctx, const pof_problem* prob, const pof_opts* opts, pof_out* out, void* stream);
 

@@ -154,7 +154,7 @@ static void tria_host(const std::vector<double>& M, int nr, int nc, std::vector<
     double tau = 0.0, beta = alpha, scal = 0.0;
     if (sig != 0.0) {
       const double nrm = std::sqrt(std::fma(alpha, alpha, sig));
-      beta = (alpha >= 0.0) ? -nrm : nrm;
+      beta = -copysign(nrm, alpha);  // Fortran SIGN: honours the sign of -0.0
       tau = (beta - alpha) / beta;
       scal = 1.0 / (alpha - beta);
     }
@@ -338,6 +338,12 @@ struct pof_ctx {
   pof::Workspace ws;
   size_t ws_cap[32] = {};  // capacities (bytes) of ws allocations, by slot
   void* ws_ptr[32] = {};
+  // per-kernel-class timing (pof_profile_*)
+  bool profiling = false;
+  pof::KernelTimer timer;
+  double prof_ms[pof::kNumKernelClasses] = {};
+  long long prof_launches[pof::kNumKernelClasses] = {};
+  long long prof_work[pof::kNumKernelClasses] = {};
 };
 
 namespace {
@@ -446,6 +452,24 @@ int prepare_workspace(pof_ctx* ctx, int D, int d, int Ns, const std::vector<doub
   return POF_OK;
 }
 
+// after a timed iteration: accumulate event intervals per kernel class
+hipError_t collect_timing(pof_ctx* ctx, hipStream_t st) {
+  if (!ctx->profiling) return hipSuccess;
+  hipError_t e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return e;
+  KernelTimer& t = ctx->timer;
+  for (int i = 0; i < t.n; ++i) {
+    float ms = 0.0f;
+    e = hipEventElapsedTime(&ms, t.ev[i], t.ev[i + 1]);
+    if (e != hipSuccess) return e;
+    ctx->prof_ms[t.cls[i]] += ms;
+    ctx->prof_launches[t.cls[i]] += 1;
+    ctx->prof_work[t.cls[i]] += t.work[i];
+  }
+  t.n = 0;
+  return hipSuccess;
+}
+
 IterOpts iter_opts(const pof_problem* pr, int cur) {
   IterOpts o;
   o.cur = cur;
@@ -491,12 +515,43 @@ void pof_ctx_destroy(pof_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   for (int i = 0; i < 32; ++i)
     if (ctx->ws_ptr[i]) (void)hipFree(ctx->ws_ptr[i]);
+  if (ctx->timer.ready)
+    for (int i = 0; i <= pof::kMaxTimed; ++i) (void)hipEventDestroy(ctx->timer.ev[i]);
   delete ctx;
 }
 
 const char* pof_last_error(const pof_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
 int pof_supported(int D, int d) { return pof::find_engine(D, d) != nullptr ? 1 : 0; }
+
+int pof_profile_enable(pof_ctx* ctx, int on) {
+  if (!ctx) return POF_E_ARG;
+  if (on && !ctx->timer.ready) {
+    (void)hipSetDevice(ctx->device);
+    for (int i = 0; i <= pof::kMaxTimed; ++i) {
+      hipError_t e = hipEventCreate(&ctx->timer.ev[i]);
+      if (e != hipSuccess) return set_err(ctx, POF_E_HIP, "hipEventCreate: %s", hipGetErrorString(e));
+    }
+    ctx->timer.ready = true;
+  }
+  ctx->profiling = on != 0;
+  for (int i = 0; i < pof::kNumKernelClasses; ++i) {
+    ctx->prof_ms[i] = 0.0;
+    ctx->prof_launches[i] = 0;
+    ctx->prof_work[i] = 0;
+  }
+  return POF_OK;
+}
+
+int pof_profile_read(const pof_ctx* ctx, int n_classes, double* ms, long long* launches, long long* work) {
+  if (!ctx || n_classes < 0) return POF_E_ARG;
+  for (int i = 0; i < n_classes && i < pof::kNumKernelClasses; ++i) {
+    if (ms) ms[i] = ctx->prof_ms[i];
+    if (launches) launches[i] = ctx->prof_launches[i];
+    if (work) work[i] = ctx->prof_work[i];
+  }
+  return pof::kNumKernelClasses;
+}
 
 int pof_eval_field(int kind, int d, const double* params, const double* y_host, double* f_host) {
   const int need = pof::problem_dim(kind);
@@ -598,7 +653,13 @@ int pof_solve(pof_ctx* ctx, const pof_problem* prob, const pof_opts* opts, pof_o
       if (!(k <= opts->maxiters)) { status = POF_STATUS_MAXITER; break; }
       if (conv) { status = isnan_ ? POF_STATUS_NAN : POF_STATUS_CONVERGED; break; }
     }
-    HIP_TRY(e->iteration(ws, iter_opts(prob, cur), st));
+    IterOpts io = iter_opts(prob, cur);
+    io.timer = ctx->profiling ? &ctx->timer : nullptr;
+    HIP_TRY(e->iteration(ws, io, st));
+    HIP_TRY(collect_timing(ctx, st));
+    cur = 1 - cur;
+    ++k;
+    if (opts->fixed_iters > 0 && k < opts->fixed_iters) continue;  // no host round trip
     HIP_TRY(hipMemcpyAsync(sc, ws.scal, 8 * sizeof(double), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     nll_old = nll;
@@ -608,8 +669,6 @@ int pof_solve(pof_ctx* ctx, const pof_problem* prob, const pof_opts* opts, pof_o
     ssq = sc[2];
     means_close = sc[3] != 0.0;
     isnan_ = std::isnan(obj) || std::isnan(nll);
-    cur = 1 - cur;
-    ++k;
   }
   (void)nll_old;
   // double calibration (step.py:42-44 inside the loop + solver.py:66-68)

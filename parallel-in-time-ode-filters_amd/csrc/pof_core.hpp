@@ -176,7 +176,7 @@ __device__ __forceinline__ void lq(double (&a)[NC], int lane_in, double* bc) {
       double tau = 0.0, beta = alpha, scal = 0.0;
       if (sig != 0.0) {
         const double nrm = sqrt(fma(alpha, alpha, sig));
-        beta = (alpha >= 0.0) ? -nrm : nrm;
+        beta = -copysign(nrm, alpha);  // Fortran SIGN: honours the sign of -0.0
         tau = (beta - alpha) / beta;
         scal = 1.0 / (alpha - beta);
       }

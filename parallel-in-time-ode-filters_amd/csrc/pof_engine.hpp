@@ -49,12 +49,35 @@ struct Workspace {
   std::vector<void*> allocs;
 };
 
+// Optional per-kernel timing with HIP events (instrumented runs only).
+constexpr int kMaxTimed = 256;
+enum KernelClass : int {
+  kKFilterLeaf = 0,
+  kKFilterUp,
+  kKFilterDown,
+  kKFilterPost,
+  kKSmoothUp,
+  kKSmoothDown,
+  kKSmoothPost,
+  kKReduce,
+  kKOutput,
+  kNumKernelClasses
+};
+struct KernelTimer {
+  int n = 0;
+  hipEvent_t ev[kMaxTimed + 1];
+  int cls[kMaxTimed];
+  long long work[kMaxTimed];  // elements processed by the launch
+  bool ready = false;
+};
+
 struct IterOpts {
   int cur = 0;           // means[cur] are the linearisation points; means[1-cur] receive the result
   int affine = 0;        // 1: observations from Hx/cx instead of EK1
   int kind = 0;
   int q = 0;
   double params[kMaxParams] = {};
+  KernelTimer* timer = nullptr;  // non-null: record an event after every launch
 };
 
 // Kernel entry points per (D, d) instantiation.
@@ -74,6 +97,5 @@ struct EngineFns {
 };
 
 const EngineFns* find_engine(int D, int d);
-size_t lds_bytes_for(int D);
 
 }  // namespace pof

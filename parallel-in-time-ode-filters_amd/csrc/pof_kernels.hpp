@@ -1,4 +1,5 @@
-// HIP kernels of one parallel IEKS iteration for gfx950.
+// HIP kernels of one parallel IEKS iteration for gfx950 (templates; the
+// instantiations per ODE dimension live in pof_inst_d*.hip).
 //
 // Filter: JAX's odd/even associative scan (pof/parallel_filtsmooth/filter.py:35)
 // reproduced level by level -- up-sweep level l pairs (2i, 2i+1), down-sweep
@@ -6,6 +7,8 @@
 // of Householder LQs as the reference.  Smoother: the same scan structure
 // on the elements stored in reversed time order (smoother.py:15-17,
 // reverse=True).  All arithmetic is float64.
+#pragma once
+
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -354,7 +357,7 @@ __global__ void k_smooth_post(const double* F, const double* LQ, StateArrays spr
 
 // One block: fixed-order sums -> scal[0] = nll, scal[1] = obj, scal[2] = ssq,
 // scal[3] = 1 if all means are close (no violations), scal[4] = NaN flag.
-__global__ void k_reduce(const double* t_ssq, const double* t_nll, const double* t_obj, const double* t_viol,
+static __global__ void k_reduce(const double* t_ssq, const double* t_nll, const double* t_obj, const double* t_viol,
                          int Ns, int d, double* scal) {
   __shared__ double sh[4][256];
   const int tid = threadIdx.x;
@@ -460,26 +463,51 @@ inline unsigned grid_workers(size_t nworkers) {
   return (unsigned)((nworkers + Geo<D>::WPB - 1) / Geo<D>::WPB);
 }
 
+namespace {
+// With a timer, event i+1 is recorded after launch i (event 0 before the first).
+struct Rec {
+  KernelTimer* t;
+  hipStream_t st;
+  Rec(KernelTimer* t_, hipStream_t s) : t(t_), st(s) {
+    if (t) {
+      t->n = 0;
+      (void)hipEventRecord(t->ev[0], st);
+    }
+  }
+  void operator()(int cls, long long work) {
+    if (!t || t->n >= kMaxTimed) return;
+    t->cls[t->n] = cls;
+    t->work[t->n] = work;
+    (void)hipEventRecord(t->ev[t->n + 1], st);
+    ++t->n;
+  }
+};
+}  // namespace
+
 template <int D, int d>
 hipError_t run_iteration(const Workspace& ws, const IterOpts& o, hipStream_t st) {
   using G = Geo<D>;
   const ModelArgs a = model_args(ws, o);
   const size_t lds = G::lds_bytes();
   const int Ns = ws.Ns;
+  Rec rec(o.timer, st);
   // ---- filter: leaf elements, up-sweep, down-sweep
   {
     FiltArrays l0 = at(ws.fel, ws.f_off[0], D);
     hipLaunchKernelGGL((k_filter_leaf<D, d>), dim3(grid_workers<D>(Ns)), dim3(G::BLOCK), lds, st, a, l0, Ns);
+    rec(kKFilterLeaf, Ns);
     for (int l = 0; l + 1 < ws.f_levels; ++l) {
       const int nout = (int)ws.f_n[l + 1];
       hipLaunchKernelGGL((k_filter_up<D>), dim3(grid_workers<D>(nout)), dim3(G::BLOCK), lds, st, a,
                          at(ws.fel, ws.f_off[l], D), at(ws.fel, ws.f_off[l + 1], D), nout);
+      rec(kKFilterUp, nout);
     }
     for (int l = ws.f_levels - 1; l >= 0; --l) {
       const int n = (int)ws.f_n[l];
       StateArrays up = (l + 1 < ws.f_levels) ? at(ws.fpre, ws.f_off[l + 1], D) : StateArrays{};
       hipLaunchKernelGGL((k_filter_down<D>), dim3(grid_workers<D>((n + 1) / 2)), dim3(G::BLOCK), lds, st, a,
                          at(ws.fel, ws.f_off[l], D), up, at(ws.fpre, ws.f_off[l], D), n);
+      rec(kKFilterDown, (n + 1) / 2 - 1);
     }
   }
   // ---- smoother elements + observation statistics
@@ -487,6 +515,7 @@ hipError_t run_iteration(const Workspace& ws, const IterOpts& o, hipStream_t st)
     SmoArrays s0 = at(ws.sel, ws.s_off[0], D);
     hipLaunchKernelGGL((k_filter_post<D, d>), dim3(grid_workers<D>(Ns + 1)), dim3(G::BLOCK), lds, st, a,
                        at(ws.fpre, ws.f_off[0], D), s0, ws.t_ssq, ws.t_nll);
+    rec(kKFilterPost, Ns);
   }
   // ---- smoother scan (reversed order)
   {
@@ -495,12 +524,14 @@ hipError_t run_iteration(const Workspace& ws, const IterOpts& o, hipStream_t st)
       const int nout = (int)ws.s_n[l + 1];
       hipLaunchKernelGGL((k_smooth_up<D>), dim3(grid_workers<D>(nout)), dim3(G::BLOCK), lds, st, a,
                          at(ws.sel, ws.s_off[l], D), at(ws.sel, ws.s_off[l + 1], D), nout, wc);
+      rec(kKSmoothUp, nout);
     }
     for (int l = ws.s_levels - 1; l >= 0; --l) {
       const int n = (int)ws.s_n[l];
       StateArrays up = (l + 1 < ws.s_levels) ? at(ws.spre, ws.s_off[l + 1], D) : StateArrays{};
       hipLaunchKernelGGL((k_smooth_down<D>), dim3(grid_workers<D>((n + 1) / 2)), dim3(G::BLOCK), lds, st, a,
                          at(ws.sel, ws.s_off[l], D), up, at(ws.spre, ws.s_off[l], D), n, wc);
+      rec(kKSmoothDown, (n + 1) / 2 - 1);
     }
   }
   // ---- smoothed means, objective, convergence partials, reduction
@@ -508,8 +539,10 @@ hipError_t run_iteration(const Workspace& ws, const IterOpts& o, hipStream_t st)
     const int nt = Ns + 1;
     hipLaunchKernelGGL((k_smooth_post<D>), dim3((nt + 127) / 128), dim3(128), 0, st, ws.F, ws.LQ,
                        at(ws.spre, ws.s_off[0], D), ws.means[o.cur], ws.means[1 - o.cur], ws.t_obj, ws.t_viol, Ns);
+    rec(kKSmoothPost, Ns + 1);
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, st, ws.t_ssq, ws.t_nll, ws.t_obj, ws.t_viol, Ns, d,
                        ws.scal);
+    rec(kKReduce, Ns);
   }
   return hipGetLastError();
 }
@@ -535,37 +568,6 @@ hipError_t run_linearize(const Workspace& ws, const IterOpts& o, const double* m
   const ModelArgs a = model_args(ws, o);
   hipLaunchKernelGGL((k_linearize<D, d>), dim3((n + 127) / 128), dim3(128), 0, st, a, m, n, H, c);
   return hipGetLastError();
-}
-
-// ------------------------------------------------------------ registry
-#define POF_ENGINE(DD, dd) EngineFns{DD, dd, &run_iteration<DD, dd>, &run_output<DD, dd>, &run_linearize<DD, dd>}
-
-static const EngineFns kEngines[] = {
-    // d = 1: q = 0..5
-    POF_ENGINE(1, 1), POF_ENGINE(2, 1), POF_ENGINE(3, 1), POF_ENGINE(4, 1), POF_ENGINE(5, 1), POF_ENGINE(6, 1),
-    // d = 2: q = 1..5
-    POF_ENGINE(4, 2), POF_ENGINE(6, 2), POF_ENGINE(8, 2), POF_ENGINE(10, 2), POF_ENGINE(12, 2),
-    // d = 3: q = 1..5
-    POF_ENGINE(6, 3), POF_ENGINE(9, 3), POF_ENGINE(12, 3), POF_ENGINE(15, 3), POF_ENGINE(18, 3),
-    // d = 4: q = 1..4
-    POF_ENGINE(8, 4), POF_ENGINE(12, 4), POF_ENGINE(16, 4), POF_ENGINE(20, 4),
-};
-
-const EngineFns* find_engine(int D, int d) {
-  for (const auto& e : kEngines)
-    if (e.D == D && e.d == d) return &e;
-  return nullptr;
-}
-
-size_t lds_bytes_for(int D) {
-  switch (D) {
-#define POF_LDS(DD) \
-  case DD: return Geo<DD>::lds_bytes();
-    POF_LDS(1) POF_LDS(2) POF_LDS(3) POF_LDS(4) POF_LDS(5) POF_LDS(6) POF_LDS(8) POF_LDS(9) POF_LDS(10)
-    POF_LDS(12) POF_LDS(15) POF_LDS(16) POF_LDS(18) POF_LDS(20)
-#undef POF_LDS
-    default: return 0;
-  }
 }
 
 }  // namespace pof

@@ -6,7 +6,8 @@ from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 CSRC = PKG.parent / "csrc"
-SOURCES = ["pof_kernels.hip", "pof_abi.hip"]
+SOURCES = ["pof_inst_d1.hip", "pof_inst_d2.hip", "pof_inst_d3.hip", "pof_inst_d4.hip", "pof_registry.hip",
+           "pof_abi.hip"]
 FLAGS = [
     "--offload-arch=gfx950",
     "-O3",

@@ -35,6 +35,8 @@ INIT_PRIOR = 0
 INIT_CONSTANT = 1
 MAX_PARAMS = 8
 STATUS = {0: "converged", 1: "maxiter", 2: "nan", 3: "fixed"}
+KERNEL_CLASSES = ["filter_leaf", "filter_up", "filter_down", "filter_post", "smooth_up", "smooth_down",
+                  "smooth_post", "reduce"]
 
 
 class Problem(ctypes.Structure):
@@ -85,6 +87,8 @@ SIGNATURES = {
     "pof_ctx_destroy": (None, [_vp]),
     "pof_last_error": (ctypes.c_char_p, [_vp]),
     "pof_supported": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    "pof_profile_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pof_profile_read": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _P(ctypes.c_longlong), _P(ctypes.c_longlong)]),
     "pof_solve": (ctypes.c_int, [_vp, _P(Problem), _P(Opts), _P(Out), _vp]),
     "pof_setup": (ctypes.c_int, [_P(Problem), _dp, _dp, _dp, _dp, _dp, _dp]),
     "pof_transitions": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _d, _dp, _dp, _dp, _dp]),

@@ -1,0 +1,323 @@
+"""Pins of the CPU oracle itself: closed forms, brute force, reference-held numbers.
+
+The oracle (``oracle/``) is the checker of the GPU path, so it is checked here
+against what does not depend on it: textbook closed forms, brute-force
+Gaussian conditioning on tiny models, the reference's own tests, the
+reference's published work-precision CSV values and its README figure.
+"""
+import csv
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+from scipy.integrate import solve_ivp
+
+from oracle import model as om
+from oracle import parallel as opar
+from oracle import problems as op
+from oracle import sequential as oseq
+from oracle import solver as osol
+from oracle.linalg import solve_triangular, tria
+
+GOLDEN = Path(__file__).parent / "golden"
+
+
+# ---------------------------------------------------------------- transitions
+@pytest.mark.parametrize("q", [0, 1, 2, 3, 4, 5])
+def test_iwp_closed_forms(q):
+    A, L = om.preconditioned_discretize_1d(q)
+    for i in range(q + 1):
+        for j in range(q + 1):
+            want_A = math.comb(q - i, j - i) if j >= i else 0
+            assert A[i, j] == want_A
+            assert abs((L @ L.T)[i, j] - 1.0 / (2 * q + 1 - i - j)) < 1e-15
+    dt = 0.37
+    P, PI = om.nordsieck_preconditioner_1d(q, dt)
+    T = P @ A @ PI
+    Qc = (P @ L) @ (P @ L).T
+    for i in range(q + 1):
+        for j in range(q + 1):
+            want = dt ** (j - i) / math.factorial(j - i) if j >= i else 0.0
+            assert abs(T[i, j] - want) < 1e-13
+            want_q = dt ** (2 * q + 1 - i - j) / ((2 * q + 1 - i - j) * math.factorial(q - i) * math.factorial(q - j))
+            assert abs(Qc[i, j] - want_q) < 1e-13 * max(1.0, abs(want_q))
+
+
+@pytest.mark.parametrize("dim", [1, 3, 5])
+@pytest.mark.parametrize("order", [1, 3, 5])
+def test_projection_layout(dim, order):
+    """Reference tests/test_transtions.py:19-32: interleaved layout."""
+    iwp = om.IWP(dim, order)
+    D = dim * (order + 1)
+    E0, E1 = om.projection_matrix(iwp, 0), om.projection_matrix(iwp, 1)
+    assert E0.shape == (dim, D) and E1.shape == (dim, D)
+    x = np.random.default_rng(dim * 10 + order).uniform(size=D)
+    assert np.all(E0 @ x == x[0 :: order + 1])
+    assert np.all(E1 @ x == x[1 :: order + 1])
+    F, QL = om.non_preconditioned_discretize(om.IWP(2, 3), 0.1)
+    assert F.shape == QL.shape == (8, 8)
+
+
+# ----------------------------------------------------------------- Taylor mode
+def test_taylor_logistic_closed_form():
+    y = 0.01
+    d1 = y * (1 - y)
+    d2 = d1 * (1 - 2 * y)
+    d3 = d2 * (1 - 2 * y) - 2 * d1**2
+    d4 = d3 * (1 - 2 * y) - 6 * d1 * d2
+    got = op.taylor_mode(op.logistic(), 4)[:, 0]
+    np.testing.assert_allclose(got, [y, d1, d2, d3, d4], rtol=1e-14, atol=1e-18)
+
+
+@pytest.mark.parametrize("name", ["fitzhughnagumo", "lorenz63", "rigid_body", "henonheiles", "vanderpol",
+                                  "lotkavolterra", "fhn_classic", "threebody", "seir"])
+def test_taylor_first_two_derivatives(name):
+    """y' = f(y0) and y'' = J(y0) f(y0) for every field."""
+    p = getattr(op, name)()
+    t = op.taylor_mode(p, 3)
+    f0 = p.f(None, p.y0)
+    np.testing.assert_allclose(t[0], p.y0)
+    np.testing.assert_allclose(t[1], f0, rtol=1e-14, atol=1e-14)
+    np.testing.assert_allclose(t[2], p.jac(p.y0) @ f0, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("name", ["lorenz63", "fitzhughnagumo", "rigid_body", "threebody"])
+def test_taylor_third_derivative(name):
+    """Third derivative d/dt (J f) = D(J f)(y0) f(y0), by central differences of J f along f."""
+    p = getattr(op, name)()
+    t = op.taylor_mode(p, 3)
+    f0 = p.f(None, p.y0)
+    g = lambda y: p.jac(y) @ p.f(None, y)
+    eps = 1e-5 / max(1.0, np.abs(f0).max())
+    d3 = (g(p.y0 + eps * f0) - g(p.y0 - eps * f0)) / (2 * eps)
+    np.testing.assert_allclose(t[3], d3, rtol=1e-7, atol=1e-7 * np.abs(d3).max())
+
+
+# --------------------------------------------------------------- EK1 linearise
+@pytest.mark.parametrize("name,q", [("lorenz63", 3), ("fitzhughnagumo", 2), ("threebody", 2), ("logistic", 4)])
+def test_linearize_invariants(name, q):
+    p = getattr(op, name)()
+    setup = om.set_up_solver(p, np.linspace(p.t0, p.tmax, 11), q)
+    rng = np.random.default_rng(1)
+    m = rng.standard_normal((5, p.d * (q + 1))) * 0.3
+    if name == "threebody":
+        m += setup.PI @ np.kron(np.ones(p.d), np.eye(q + 1)[0]) * 0.5
+    H, b, R = om.linearize(setup, m)
+    # reference tests/test_observations.py:33: H m + b == f(m)
+    np.testing.assert_allclose(np.einsum("nij,nj->ni", H, m) + b, om.residual(setup, m), rtol=1e-12, atol=1e-10)
+    assert np.all(R == 0) and R.shape == (5, p.d, p.d)
+    # finite differences of the residual
+    eps = 1e-6
+    for j in range(m.shape[1]):
+        e = np.zeros(m.shape[1])
+        e[j] = eps
+        fd = (om.residual(setup, m + e) - om.residual(setup, m - e)) / (2 * eps)
+        np.testing.assert_allclose(H[:, :, j], fd, rtol=1e-5, atol=1e-5 * max(1.0, np.abs(H).max()))
+
+
+# ------------------------------------------------------- associative scan order
+def test_associative_scan_matches_fold():
+    rng = np.random.default_rng(0)
+    for n in range(1, 20):
+        M = rng.standard_normal((n, 3, 3))
+        (fw,) = opar.associative_scan(lambda a, b: (b[0] @ a[0],), (M,))
+        (bw,) = opar.associative_scan(lambda a, b: (b[0] @ a[0],), (M,), reverse=True)
+        acc = M[0]
+        for k in range(n):
+            if k:
+                acc = M[k] @ acc
+            np.testing.assert_allclose(fw[k], acc, rtol=1e-10)
+        acc = M[-1]
+        for k in range(n - 1, -1, -1):
+            if k < n - 1:
+                acc = M[k] @ acc
+            np.testing.assert_allclose(bw[k], acc, rtol=1e-10)
+
+
+# ------------------------------------------------------ brute-force conditioning
+def _random_model(rng, d, q, Ns, prior_chol=True):
+    D = d * (q + 1)
+    F, QL = om.preconditioned_discretize(om.IWP(d, q))
+    H = rng.standard_normal((Ns, d, D))
+    c = rng.standard_normal((Ns, d))
+    m0 = rng.standard_normal(D)
+    L0 = np.tril(rng.standard_normal((D, D))) if prior_chol else np.zeros((D, D))
+    return F, QL, om.AffineModel(H, c, np.zeros((Ns, d, d))), om.MVNSqrt(m0, L0)
+
+
+def _brute(F, QL, obs, x0):
+    """Dense joint Gaussian over x_0..x_Ns, conditioned on H_k x_{k+1} + c_k = 0."""
+    H, c, _ = obs
+    Ns, d, D = H.shape
+    n = (Ns + 1) * D
+    # x = B w + mu with w ~ N(0, I): x_0 = m0 + L0 w0, x_{k+1} = F x_k + QL w_{k+1}
+    B = np.zeros((n, n))
+    mu = np.zeros(n)
+    mu[:D] = x0.mean
+    B[:D, :D] = x0.chol
+    for k in range(Ns):
+        s, t = k * D, (k + 1) * D
+        mu[t : t + D] = F @ mu[s : s + D]
+        B[t : t + D] = F @ B[s : s + D]
+        B[t : t + D, t : t + D] += QL
+    S = B @ B.T
+    G = np.zeros((Ns * d, n))
+    for k in range(Ns):
+        G[k * d : (k + 1) * d, (k + 1) * D : (k + 2) * D] = H[k]
+    y = G @ mu + c.reshape(-1)
+
+    def condition(rows):
+        Gs = G[rows]
+        Syy = Gs @ S @ Gs.T
+        K = S @ Gs.T @ np.linalg.inv(Syy)
+        return mu - K @ y[rows], S - K @ Gs @ S
+
+    filt_m, filt_P = [x0.mean], [x0.chol @ x0.chol.T]
+    for k in range(1, Ns + 1):
+        mk, Pk = condition(np.arange(k * d))
+        filt_m.append(mk[k * D : (k + 1) * D])
+        filt_P.append(Pk[k * D : (k + 1) * D, k * D : (k + 1) * D])
+    ms, Ps = condition(np.arange(Ns * d))
+    sm_m = [ms[k * D : (k + 1) * D] for k in range(Ns + 1)]
+    sm_P = [Ps[k * D : (k + 1) * D, k * D : (k + 1) * D] for k in range(Ns + 1)]
+    Syy = G @ S @ G.T
+    nll = 0.5 * y @ np.linalg.solve(Syy, y) + 0.5 * np.linalg.slogdet(2 * np.pi * Syy)[1]
+    return np.array(filt_m), np.array(filt_P), np.array(sm_m), np.array(sm_P), nll
+
+
+@pytest.mark.parametrize("d,q,Ns,seed", [(1, 2, 5, 0), (2, 1, 6, 1), (2, 2, 4, 2), (1, 3, 4, 3)])
+def test_parallel_filtsmooth_brute_force(d, q, Ns, seed):
+    rng = np.random.default_rng(seed)
+    F, QL, obs, x0 = _random_model(rng, d, q, Ns)
+    fm, fP, sm, sP, nll = _brute(F, QL, obs, x0)
+    filt, pnll, _, pssq = opar.linear_noiseless_filtering(x0, F, QL, obs)
+    out, obj = opar.smoothing(F, QL, filt)
+    cov = lambda L: np.einsum("nij,nkj->nik", L, L)
+    scale = np.abs(sP).max()
+    np.testing.assert_allclose(filt.mean, fm, rtol=1e-8, atol=1e-8 * np.abs(fm).max())
+    np.testing.assert_allclose(cov(filt.chol), fP, atol=1e-8 * np.abs(fP).max())
+    np.testing.assert_allclose(out.mean, sm, rtol=1e-8, atol=1e-8 * np.abs(sm).max())
+    np.testing.assert_allclose(cov(out.chol), sP, atol=1e-8 * scale)
+    assert abs(pnll - nll) < 1e-8 * max(1.0, abs(nll))
+    if d == 1:
+        # sigma^2: mean of squared whitened innovations (sign-free for d = 1)
+        H, c, _ = obs
+        pm = np.einsum("ij,nj->ni", F, fm[:-1])
+        yhat = np.einsum("nij,nj->ni", H, pm) + c
+        Spred = np.einsum("nij,njk,nlk->nil", H, np.einsum("ij,njk,lk->nil", F, fP[:-1], F) + QL @ QL.T, H)
+        ssq = np.mean(yhat[:, 0] ** 2 / Spred[:, 0, 0])
+        assert abs(pssq - ssq) < 1e-8 * ssq
+
+
+@pytest.mark.parametrize("d,q,Ns,seed", [(2, 2, 23, 5), (3, 1, 17, 6), (1, 3, 30, 7)])
+def test_parallel_equals_sequential(d, q, Ns, seed):
+    rng = np.random.default_rng(seed)
+    F, QL, obs, x0 = _random_model(rng, d, q, Ns)
+    p, pnll, pobj, pssq = opar.linear_filtsmooth(x0, F, QL, obs)
+    s, snll, sobj, sssq = oseq.linear_filtsmooth(x0, F, QL, obs)
+    cov = lambda L: np.einsum("nij,nkj->nik", L, L)
+    np.testing.assert_allclose(p.mean, s.mean, rtol=1e-9, atol=1e-9 * np.abs(s.mean).max())
+    np.testing.assert_allclose(cov(p.chol), cov(s.chol), atol=1e-9 * np.abs(cov(s.chol)).max())
+    assert abs(pnll + snll) < 1e-8 * abs(snll)  # sign of the nll slot differs by path
+    assert abs(pobj - sobj) < 1e-8 * abs(sobj)
+
+
+def test_reference_degenerate_equality():
+    """Reference tests/test_filtsmooth.py:16-62 (q = 0, d = 1, all-zero model): exact equality."""
+    iwp = om.IWP(1, 0)
+    dtm = om.discretize_transitions(iwp, np.arange(10))
+    F, QL = dtm.F[0], dtm.QL[0]
+    Ns = dtm.F.shape[0]
+    obs = om.AffineModel(np.ones((Ns, 1, 1)), np.zeros((Ns, 1)), np.zeros((Ns, 1, 1)))
+    x0 = om.MVNSqrt(np.zeros(1), np.zeros((1, 1)))
+    pf, _, _, _ = opar.linear_noiseless_filtering(x0, F, QL, obs)
+    ps, _ = opar.smoothing(F, QL, pf)
+    sf, _, _ = oseq.linear_noiseless_filter(x0, F, QL, obs)
+    ss, _ = oseq.smoothing(F, QL, sf)
+    assert np.all(pf.mean == sf.mean) and np.all(pf.chol == sf.chol)
+    assert np.all(ps.mean == ss.mean) and np.all(ps.chol == ss.chol)
+
+
+def test_tria_sign_convention_negative_zero():
+    """dlarfg uses Fortran SIGN(norm, alpha): a -0.0 pivot gives a positive diagonal."""
+    L = tria(np.array([[-0.0, 1.0], [1.0, 2.0]]))
+    assert L[0, 0] == 1.0
+    L = tria(np.array([[0.0, 1.0], [1.0, 2.0]]))
+    assert L[0, 0] == -1.0
+
+
+# ------------------------------------------------------ reference-held numbers
+def _pins():
+    with open(GOLDEN / "work_precision_pins.csv") as f:
+        return list(csv.DictReader(f))
+
+
+PROB = {
+    "logistic": lambda: op.logistic(),
+    "fhn": lambda: op.fitzhughnagumo(),
+    "rigidbody": lambda: op.rigid_body(),
+    "vdp0": lambda: op.vanderpol(stiffness_constant=1.0),
+    "henonheiles": lambda: op.henonheiles(tmax=10.0),
+}
+# rows whose parallel and sequential iteration counts already differ in the
+# reference's own run are rounding-sensitive and are not pinned
+PINS = [r for r in _pins() if r["iterations"] == r["seq_iterations"] and float(r["iterations"]) <= 200]
+
+
+@pytest.mark.parametrize("row", PINS, ids=[f"{r['problem']}-N{r['N']}-q{r['q']}" for r in PINS])
+def test_work_precision_csv_pins(row):
+    p = PROB[row["problem"]]()
+    N, q = int(row["N"]), int(row["q"])
+    ts = np.linspace(p.t0, p.tmax, N)
+    ys, info = osol.solve(p, ts, q, init="constant", maxiters=1000)
+    assert info["iterations"] == float(row["iterations"])
+    sol = solve_ivp(lambda _, y: p.f(None, y), p.t_span, p.y0, method="DOP853", rtol=1e-13, atol=1e-15,
+                    dense_output=True)
+    yref = sol.sol(ts).T
+    rf = np.linalg.norm(ys.mean[-1] - yref[-1])
+    rt = np.linalg.norm(ys.mean - yref, axis=1).mean()
+    # the reference solution of the CSVs carries ~1e-11 error of its own
+    for got, want in ((rf, float(row["rmse_final"])), (rt, float(row["rmse_traj"]))):
+        if want > 1e-8:
+            assert abs(got - want) <= 1e-5 * want + 1e-11, (got, want)
+
+
+def test_readme_figure_pins():
+    """readmefig.svg of the reference: parallel solve and sequential EKS on FHN (decoded to ~5e-8)."""
+    z = np.load(GOLDEN / "readme_fhn.npz")
+    p = op.fitzhughnagumo()
+    ts = np.linspace(0, 100, 100)
+    ys, info = osol.solve(p, ts, 3, init="constant")
+    tol = 2e-7
+    np.testing.assert_allclose(ys.mean, z["par_means"], atol=tol)
+    std = np.sqrt(np.einsum("nij,nij->ni", ys.chol, ys.chol))
+    lo, hi = np.sort(z["par_bands"], axis=-1)[..., 0], np.sort(z["par_bands"], axis=-1)[..., 1]
+    np.testing.assert_allclose(lo, ys.mean - 2 * std, atol=tol)
+    np.testing.assert_allclose(hi, ys.mean + 2 * std, atol=tol)
+    ts3 = np.linspace(0, 100, 300)
+    ys3, _ = osol.sequential_eks_solve(p, ts3, 3)
+    np.testing.assert_allclose(ys3.mean, z["seq_means"], atol=tol)
+    std3 = np.sqrt(np.einsum("nij,nij->ni", ys3.chol, ys3.chol))
+    b3 = np.sort(z["seq_bands"], axis=-1)
+    np.testing.assert_allclose(b3[..., 0], ys3.mean - 2 * std3, atol=tol)
+    np.testing.assert_allclose(b3[..., 1], ys3.mean + 2 * std3, atol=tol)
+
+
+def test_converged_residual_is_zero():
+    """Noiseless observations: E1 m_k = f(E0 m_k) at every k >= 1 at the IEKS fixed point."""
+    p = op.logistic()
+    ts = np.linspace(0, 10, 64)
+    ys, info = osol.solve(p, ts, 3, init="constant", return_history=True)
+    setup = info["setup"]
+    m = info["states"].mean
+    r = om.residual(setup, m[1:])
+    scale = np.abs(np.einsum("ij,nj->ni", setup.E1, m[1:])).max()
+    assert np.abs(r).max() < 1e-7 * scale
+
+
+def test_maxiters_semantics():
+    """The loop runs while k <= maxiters: maxiters + 1 iterations when never converged."""
+    p = op.fitzhughnagumo()
+    ys, info = osol.solve(p, np.linspace(0, 100, 64), 1, init="constant", maxiters=3)
+    assert info["iterations"] == 4.0
