@@ -435,6 +435,7 @@ int prepare_workspace(pof_ctx* ctx, int D, int d, int Ns, const std::vector<doub
   HIP_TRY(ensure(ctx, slot++, ((size_t)Ns + 1) * B, &ws.t_obj));
   HIP_TRY(ensure(ctx, slot++, ((size_t)Ns + 1) * B, &ws.t_viol));
   HIP_TRY(ensure(ctx, slot++, 8 * B, &ws.scal));
+  HIP_TRY(ensure(ctx, slot++, 4 * 128 * B, &ws.part));
   HIP_TRY(ensure(ctx, slot++, DD * B, &ws.F));
   HIP_TRY(ensure(ctx, slot++, DD * B, &ws.LQ));
   HIP_TRY(ensure(ctx, slot++, DD * B, &ws.N1z));

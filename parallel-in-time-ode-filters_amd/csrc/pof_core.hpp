@@ -170,9 +170,11 @@ __device__ __forceinline__ void lq(double (&a)[NC], int lane_in, double* bc) {
     if (i == NC - 1) break;  // a length-1 tail: H = I
     if (lane == i) {
       const double alpha = a[i];
-      double sig = 0.0;
+      // sum of squares of the tail in four independent chains (latency)
+      double sq[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int j = i + 1; j < NC; ++j) sig = fma(a[j], a[j], sig);
+      for (int j = i + 1; j < NC; ++j) sq[(j - i - 1) & 3] = fma(a[j], a[j], sq[(j - i - 1) & 3]);
+      const double sig = (sq[0] + sq[1]) + (sq[2] + sq[3]);
       double tau = 0.0, beta = alpha, scal = 0.0;
       if (sig != 0.0) {
         const double nrm = sqrt(fma(alpha, alpha, sig));
@@ -190,10 +192,10 @@ __device__ __forceinline__ void lq(double (&a)[NC], int lane_in, double* bc) {
     wsync();
     if (lane > i && lane < NR) {
       const double tau = bc[i];
-      double w = a[i];
+      double wp[4] = {a[i], 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int j = i + 1; j < NC; ++j) w = fma(a[j], bc[j], w);
-      const double tw = tau * w;
+      for (int j = i + 1; j < NC; ++j) wp[(j - i) & 3] = fma(a[j], bc[j], wp[(j - i) & 3]);
+      const double tw = tau * ((wp[0] + wp[1]) + (wp[2] + wp[3]));
       a[i] -= tw;
 #pragma unroll
       for (int j = i + 1; j < NC; ++j) a[j] = fma(-tw, bc[j], a[j]);
@@ -207,10 +209,13 @@ template <int n, int LD, int NY>
 __device__ __forceinline__ void rsolve_lower(double (&y)[NY], const double* L) {
 #pragma unroll
   for (int j = n - 1; j >= 0; --j) {
-    double s = y[j];
+    double s0 = y[j], s1 = 0.0;
 #pragma unroll
-    for (int l = j + 1; l < n; ++l) s = fma(-y[l], L[l * LD + j], s);
-    y[j] = s / L[j * LD + j];
+    for (int l = j + 1; l < n; ++l) {
+      if ((l - j) & 1) s0 = fma(-y[l], L[l * LD + j], s0);
+      else s1 = fma(-y[l], L[l * LD + j], s1);
+    }
+    y[j] = (s0 + s1) / L[j * LD + j];
     pin1(y[j]);
   }
 }
@@ -220,10 +225,13 @@ template <int n, int LD, int NY>
 __device__ __forceinline__ void rsolve_lowerT(double (&y)[NY], const double* L) {
 #pragma unroll
   for (int j = 0; j < n; ++j) {
-    double s = y[j];
+    double s0 = y[j], s1 = 0.0;
 #pragma unroll
-    for (int l = 0; l < j; ++l) s = fma(-y[l], L[j * LD + l], s);
-    y[j] = s / L[j * LD + j];
+    for (int l = 0; l < j; ++l) {
+      if (l & 1) s1 = fma(-y[l], L[j * LD + l], s1);
+      else s0 = fma(-y[l], L[j * LD + l], s0);
+    }
+    y[j] = (s0 + s1) / L[j * LD + j];
     pin1(y[j]);
   }
 }
@@ -491,17 +499,30 @@ __device__ void filter_combine(const Ctx<D>& cx, const FiltArrays& x1, const Sta
 
   const double* U1p = state_only ? s1.L : x1.U;
   const double* b1p = state_only ? s1.m : x1.b;
+  // every global read of the combine is issued here, in one batch, so that
+  // the memory latency is paid once rather than once per phase
   const double b1r = b1p[i1 * D + rr];
   const double eta2r = x2.eta[i2 * D + rr];
+  const double b2r = x2.b[i2 * D + rr];
+  double a2[D], u2[D], z1[D];
+  double eta1r = 0.0;
   {
-    double u1[D], z2[D];
+    double u1[D], z2[D], a1[D];
     load_row(u1, U1p + (i1 * D + rr) * D);
     load_row(z2, x2.Z + (i2 * D + rr) * D);
+    load_row(a2, x2.A + (i2 * D + rr) * D);
+    load_row(u2, x2.U + (i2 * D + rr) * D);
+    if constexpr (!state_only) {
+      load_row(a1, x1.A + (i1 * D + rr) * D);
+      load_row(z1, x1.Z + (i1 * D + rr) * D);
+      eta1r = x1.eta[i1 * D + rr];
+    }
     if (act) {
 #pragma unroll
       for (int c = 0; c < D; ++c) {
         TU1[r * LD + c] = u1[c];
         TZ2[r * LD + c] = z2[c];
+        if constexpr (!state_only) TA1[r * LD + c] = a1[c];
       }
       Vb1[r] = b1r;
       Veta2[r] = eta2r;
@@ -541,8 +562,7 @@ __device__ void filter_combine(const Ctx<D>& cx, const FiltArrays& x1, const Sta
   wsync();
 
   // MT = A2 U1 Xi11^-T ; W = A2 - MT Xi21^T
-  double a2[D], mt[D], wr[D];
-  load_row(a2, x2.A + (i2 * D + rr) * D);
+  double mt[D], wr[D];
   row_times_tile<D, D, LD>(a2, TU1, mt);
   rsolve_lowerT<D, LD>(mt, TX11);
 #pragma unroll
@@ -566,7 +586,7 @@ __device__ void filter_combine(const Ctx<D>& cx, const FiltArrays& x1, const Sta
     for (int j = 0; j < D; ++j) sv = fma(TU1[rr * LD + j], Vu[j], sv);
     if (act) Vs[r] = sv;
     wsync();
-    double bb = x2.b[i2 * D + rr];
+    double bb = b2r;
 #pragma unroll
     for (int k = 0; k < D; ++k) bb = fma(wr[k], Vs[k], bb);
     double* bo = state_only ? so.m : y.b;
@@ -578,7 +598,7 @@ __device__ void filter_combine(const Ctx<D>& cx, const FiltArrays& x1, const Sta
 #pragma unroll
     for (int c = 0; c < D; ++c) ur[c] = mt[c];
 #pragma unroll
-    for (int c = 0; c < D; ++c) ur[D + c] = x2.U[(i2 * D + rr) * D + c];
+    for (int c = 0; c < D; ++c) ur[D + c] = u2[c];
     lq<D, 2 * D>(ur, r, cx.bc());
     if (act) {
       double* Uo = state_only ? so.L : y.U;
@@ -591,15 +611,6 @@ __device__ void filter_combine(const Ctx<D>& cx, const FiltArrays& x1, const Sta
     double* Vs2 = cx.vec(5);
     double* Vu2 = cx.vec(6);
     double* Vs3 = cx.vec(7);
-    {
-      double a1[D];
-      load_row(a1, x1.A + (i1 * D + rr) * D);
-      if (act) {
-#pragma unroll
-        for (int c = 0; c < D; ++c) TA1[r * LD + c] = a1[c];
-      }
-    }
-    wsync();
     // A = W A1
     if (act) {
       double ar[D];
@@ -640,7 +651,7 @@ __device__ void filter_combine(const Ctx<D>& cx, const FiltArrays& x1, const Sta
     }
     wsync();
     if (act) {
-      double e = x1.eta[i1 * D + rr];
+      double e = eta1r;
 #pragma unroll
       for (int k = 0; k < D; ++k) e = fma(TA1[k * LD + r], Vs3[k], e);
       y.eta[o * D + r] = e;
@@ -653,7 +664,7 @@ __device__ void filter_combine(const Ctx<D>& cx, const FiltArrays& x1, const Sta
 #pragma unroll
       for (int c = 0; c < D; ++c) {
         zr[c] = t[c];
-        zr[D + c] = x1.Z[(i1 * D + rr) * D + c];
+        zr[D + c] = z1[c];
       }
     }
     lq<D, 2 * D>(zr, r, cx.bc());
@@ -681,31 +692,31 @@ __device__ void smoother_combine(const Ctx<D>& cx, const SmoArrays& x1, const St
   double* Vg1 = cx.vec(0);
   const double* g1p = state_only ? s1.m : x1.g;
   const double* D1p = state_only ? s1.L : x1.Dm;
+  // all global reads first (one memory latency)
+  double e2[D], d2[D];
+  const double g2r = x2.g[i2 * D + rr];
+  load_row(e2, x2.E + (i2 * D + rr) * D);
+  if (with_chol) load_row(d2, x2.Dm + (i2 * D + rr) * D);
   {
     const double g1 = g1p[i1 * D + rr];
-    if (act) Vg1[r] = g1;
-    if (!state_only) {
-      double e1[D];
-      load_row(e1, x1.E + (i1 * D + rr) * D);
-      if (act) {
+    double e1[D], d1[D];
+    if (!state_only) load_row(e1, x1.E + (i1 * D + rr) * D);
+    if (with_chol) load_row(d1, D1p + (i1 * D + rr) * D);
+    if (act) {
+      Vg1[r] = g1;
+      if (!state_only) {
 #pragma unroll
         for (int c = 0; c < D; ++c) TE1[r * LD + c] = e1[c];
       }
-    }
-    if (with_chol) {
-      double d1[D];
-      load_row(d1, D1p + (i1 * D + rr) * D);
-      if (act) {
+      if (with_chol) {
 #pragma unroll
         for (int c = 0; c < D; ++c) TD1[r * LD + c] = d1[c];
       }
     }
   }
   wsync();
-  double e2[D];
-  load_row(e2, x2.E + (i2 * D + rr) * D);
   if (act) {
-    double gg = x2.g[i2 * D + r];
+    double gg = g2r;
 #pragma unroll
     for (int k = 0; k < D; ++k) gg = fma(e2[k], Vg1[k], gg);
     (state_only ? so.m : y.g)[o * D + r] = gg;
@@ -723,7 +734,7 @@ __device__ void smoother_combine(const Ctx<D>& cx, const SmoArrays& x1, const St
 #pragma unroll
       for (int c = 0; c < D; ++c) {
         dr[c] = t[c];
-        dr[D + c] = x2.Dm[(i2 * D + rr) * D + c];
+        dr[D + c] = d2[c];
       }
     }
     lq<D, 2 * D>(dr, r, cx.bc());

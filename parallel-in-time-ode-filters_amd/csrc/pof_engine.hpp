@@ -37,6 +37,7 @@ struct Workspace {
   double* t_obj = nullptr;   // [Ns]
   double* t_viol = nullptr;  // [Ns+1]
   double* scal = nullptr;    // [8] device scalars: nll, obj, ssq, means_converged
+  double* part = nullptr;    // [4 * 128] partial sums of the reduction
   // model constants
   double* F = nullptr;
   double* LQ = nullptr;

@@ -152,10 +152,20 @@ __global__ __launch_bounds__(Geo<D>::BLOCK) void k_filter_post(ModelArgs a, Stat
   double* TD = cx.tile(3);
   double* Vm = cx.vec(0);
   double* Vfm = cx.vec(1);
-  if (act) {
+  // global reads first: the filtered state and the linearisation point of obs model s
+  double H[d][D];
+  double cv[d];
+  observation<D, d>(a, s, H, cv);
+  {
+    const int rr = act ? r : 0;
+    double lr[D];
+    load_row(lr, Lp + rr * D);
+    const double mr = mp[rr];
+    if (act) {
 #pragma unroll
-    for (int c = 0; c < D; ++c) TL[r * LD + c] = Lp[r * D + c];
-    Vm[r] = mp[r];
+      for (int c = 0; c < D; ++c) TL[r * LD + c] = lr[c];
+      Vm[r] = mr;
+    }
   }
   wsync();
   // Phi_ = [[F L, LQ], [L, 0]]
@@ -214,9 +224,6 @@ __global__ __launch_bounds__(Geo<D>::BLOCK) void k_filter_post(ModelArgs a, Stat
     for (int c = 0; c < D; ++c) sel.Dm[(ro * D + r) * D + c] = TD[r * LD + c];
   }
   // observation statistics with obs model s
-  double H[d][D];
-  double cv[d];
-  observation<D, d>(a, s, H, cv);
   double sr[D + d];
   {
     double h[D];
@@ -355,25 +362,52 @@ __global__ void k_smooth_post(const double* F, const double* LQ, StateArrays spr
   }
 }
 
-// One block: fixed-order sums -> scal[0] = nll, scal[1] = obj, scal[2] = ssq,
-// scal[3] = 1 if all means are close (no violations), scal[4] = NaN flag.
-static __global__ void k_reduce(const double* t_ssq, const double* t_nll, const double* t_obj, const double* t_viol,
-                         int Ns, int d, double* scal) {
+// Fixed-order reduction of the per-index terms, stage 1: block b sums the
+// contiguous segment [b * seg, (b + 1) * seg) into part[b * 4 + 0..3]
+// (ssq, nll, obj: sums; violations: max).
+constexpr int kRedBlocks = 128;
+static __global__ void k_reduce_part(const double* t_ssq, const double* t_nll, const double* t_obj,
+                                     const double* t_viol, int Ns, double* part) {
   __shared__ double sh[4][256];
   const int tid = threadIdx.x;
+  const int seg = (Ns + 1 + kRedBlocks - 1) / kRedBlocks;
+  const int lo = blockIdx.x * seg;
+  const int hi = min(lo + seg, Ns + 1);
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  for (int i = tid; i < Ns; i += 256) {
-    s0 += t_ssq[i];
-    s1 += t_nll[i];
-    s2 += t_obj[i];
+  for (int i = lo + tid; i < hi; i += 256) {
+    if (i < Ns) {
+      s0 += t_ssq[i];
+      s1 += t_nll[i];
+      s2 += t_obj[i];
+    }
+    s3 = fmax(s3, t_viol[i]);
   }
-  for (int i = tid; i <= Ns; i += 256) s3 = fmax(s3, t_viol[i]);
   sh[0][tid] = s0;
   sh[1][tid] = s1;
   sh[2][tid] = s2;
   sh[3][tid] = s3;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) {
+      sh[0][tid] += sh[0][tid + w];
+      sh[1][tid] += sh[1][tid + w];
+      sh[2][tid] += sh[2][tid + w];
+      sh[3][tid] = fmax(sh[3][tid], sh[3][tid + w]);
+    }
+    __syncthreads();
+  }
+  if (tid < 4) part[blockIdx.x * 4 + tid] = sh[tid][0];
+}
+
+// Stage 2 (one block of kRedBlocks threads): scal[0] = nll, scal[1] = obj,
+// scal[2] = ssq, scal[3] = 1 if all means are close, scal[4] = NaN flag.
+static __global__ void k_reduce_final(const double* part, int Ns, int d, double* scal) {
+  __shared__ double sh[4][kRedBlocks];
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) sh[k][tid] = part[tid * 4 + k];
+  __syncthreads();
+  for (int w = kRedBlocks / 2; w > 0; w >>= 1) {
     if (tid < w) {
       sh[0][tid] += sh[0][tid + w];
       sh[1][tid] += sh[1][tid + w];
@@ -540,8 +574,9 @@ hipError_t run_iteration(const Workspace& ws, const IterOpts& o, hipStream_t st)
     hipLaunchKernelGGL((k_smooth_post<D>), dim3((nt + 127) / 128), dim3(128), 0, st, ws.F, ws.LQ,
                        at(ws.spre, ws.s_off[0], D), ws.means[o.cur], ws.means[1 - o.cur], ws.t_obj, ws.t_viol, Ns);
     rec(kKSmoothPost, Ns + 1);
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, st, ws.t_ssq, ws.t_nll, ws.t_obj, ws.t_viol, Ns, d,
-                       ws.scal);
+    hipLaunchKernelGGL(k_reduce_part, dim3(kRedBlocks), dim3(256), 0, st, ws.t_ssq, ws.t_nll, ws.t_obj,
+                       ws.t_viol, Ns, ws.part);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedBlocks), 0, st, ws.part, Ns, d, ws.scal);
     rec(kKReduce, Ns);
   }
   return hipGetLastError();
