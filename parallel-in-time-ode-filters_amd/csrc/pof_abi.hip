@@ -644,6 +644,7 @@ int pof_solve(pof_ctx* ctx, const pof_problem* prob, const pof_opts* opts, pof_o
   bool means_close = false, isnan_ = false;
   int k = 0;
   int status = POF_STATUS_CONVERGED;
+  int last_with_chol = 0;
   double sc[8];
   for (;;) {
     if (opts->fixed_iters > 0) {
@@ -656,6 +657,10 @@ int pof_solve(pof_ctx* ctx, const pof_problem* prob, const pof_opts* opts, pof_o
     }
     IterOpts io = iter_opts(prob, cur);
     io.timer = ctx->profiling ? &ctx->timer : nullptr;
+    // the smoothed factors are needed only from the last iteration: means-only
+    // smoother scans until then (known in advance only with fixed_iters)
+    io.with_chol = (opts->fixed_iters > 0 && k + 1 == opts->fixed_iters) ? 1 : 0;
+    last_with_chol = io.with_chol;
     HIP_TRY(e->iteration(ws, io, st));
     HIP_TRY(collect_timing(ctx, st));
     cur = 1 - cur;
@@ -672,6 +677,13 @@ int pof_solve(pof_ctx* ctx, const pof_problem* prob, const pof_opts* opts, pof_o
     isnan_ = std::isnan(obj) || std::isnan(nll);
   }
   (void)nll_old;
+  if (!last_with_chol) {
+    // smoothed factors of the last iteration's smoother elements
+    IterOpts io = iter_opts(prob, cur);
+    io.timer = ctx->profiling ? &ctx->timer : nullptr;
+    HIP_TRY(e->smooth_chol(ws, io, st));
+    HIP_TRY(collect_timing(ctx, st));
+  }
   // double calibration (step.py:42-44 inside the loop + solver.py:66-68)
   const double scale = opts->calibrate ? ssq : std::sqrt(ssq);
   HIP_TRY(e->output(ws, cur, scale, out->means_dev, out->chols_dev, out->full_means_dev, out->full_chols_dev,

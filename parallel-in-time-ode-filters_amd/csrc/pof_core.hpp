@@ -31,20 +31,34 @@ __device__ __forceinline__ void wsync() {
 }
 
 // ------------------------------------------------------------------ layout
-template <int D_>
+// LDS layout kinds: how many tiles / broadcast vectors a worker of each
+// kernel needs and whether the block stages the model constants F, LQ, N1z.
+enum LdsKind : int { kLdsFull = 0, kLdsState, kLdsSmooth, kLdsLeaf, kLdsPost };
+
+template <int K>
+struct LdsNeeds {
+  static constexpr int NT = (K == kLdsFull) ? 6 : (K == kLdsState || K == kLdsLeaf) ? 4 : (K == kLdsPost) ? 5 : 2;
+  static constexpr int NV = (K == kLdsFull) ? 8 : (K == kLdsState) ? 4 : (K == kLdsPost) ? 2 : 1;
+  static constexpr bool CONSTS = (K == kLdsLeaf || K == kLdsPost);
+};
+
+template <int D_, int K = kLdsFull>
 struct Geo {
   static constexpr int D = D_;
   static constexpr int W = (2 * D_ <= 32) ? 32 : 64;      // lanes per worker
   static constexpr int LD = D_ + 1;                      // padded LDS row stride
   static constexpr int TILE = D_ * LD;
-  static constexpr int NT = 6;                           // tiles per worker
-  static constexpr int NV = 8;                           // broadcast vectors per worker
+  static constexpr int NT = LdsNeeds<K>::NT;             // tiles per worker
+  static constexpr int NV = LdsNeeds<K>::NV;             // broadcast vectors per worker
   static constexpr int BC = 2 * D_ + 2;                  // Householder broadcast row
   static constexpr int WORKER = BC + NV * 2 * D_ + NT * TILE;
-  static constexpr int CONSTS = 3 * TILE;                // F, LQ, N1z staged per block
+  static constexpr int CONSTS = LdsNeeds<K>::CONSTS ? 3 * TILE : 0;  // F, LQ, N1z
   static constexpr size_t bytes_for(int wpb) { return sizeof(double) * (CONSTS + wpb * WORKER); }
   // workers per block: as many as fit in 64 KiB of LDS, at most 256 threads
-  static constexpr int WPB = (W == 32 && bytes_for(4) <= 65536) ? 4 : (bytes_for(2) <= 65536 ? 2 : 1);
+  static constexpr int WPB = (W == 32 && bytes_for(8) <= 65536)   ? 8
+                             : (W == 32 && bytes_for(4) <= 65536) ? 4
+                             : (bytes_for(2) <= 65536)            ? 2
+                                                                  : 1;
   static constexpr int BLOCK = W * WPB;
   static constexpr size_t lds_bytes() { return bytes_for(WPB); }
   static_assert(bytes_for(WPB) <= 65536, "state dimension too large for one worker per block");
@@ -52,35 +66,43 @@ struct Geo {
 
 template <int D>
 struct Ctx {
-  using G = Geo<D>;
+  static constexpr int LD = D + 1;
+  static constexpr int TILE = D * LD;
   int lane;           // lane inside the worker
   int wid;            // global worker index
-  double* w;          // worker LDS slice
-  const double* sF;   // block LDS constants, row stride LD
+  double* w;          // worker LDS slice (starts with the Householder broadcast row)
+  double* vbase;      // broadcast vectors
+  double* tbase;      // tiles
+  const double* sF;   // block LDS constants, row stride LD (kernels that stage them)
   const double* sLQ;
   const double* sN1;
-  __device__ double* tile(int i) const { return w + G::BC + G::NV * 2 * D + i * G::TILE; }
-  __device__ double* vec(int i) const { return w + G::BC + i * 2 * D; }
+  __device__ double* tile(int i) const { return tbase + i * TILE; }
+  __device__ double* vec(int i) const { return vbase + i * 2 * D; }
   __device__ double* bc() const { return w; }
 };
 
-// Stage F, LQ, N1z into block LDS and carve out this thread's worker slice.
-// Must be called by every thread of the block before any early exit.
-template <int D>
+// Stage F, LQ, N1z into block LDS (kinds that need them) and carve out this
+// thread's worker slice.  Must be called by every thread of the block before
+// any early exit.
+template <int D, int K>
 __device__ __forceinline__ Ctx<D> make_ctx(const ModelArgs& a, double* smem) {
-  using G = Geo<D>;
-  for (int t = threadIdx.x; t < D * D; t += G::BLOCK) {
-    const int r = t / D, c = t % D;
-    smem[r * G::LD + c] = a.F[t];
-    smem[G::TILE + r * G::LD + c] = a.LQ[t];
-    smem[2 * G::TILE + r * G::LD + c] = a.N1z[t];
+  using G = Geo<D, K>;
+  if constexpr (G::CONSTS > 0) {
+    for (int t = threadIdx.x; t < D * D; t += G::BLOCK) {
+      const int r = t / D, c = t % D;
+      smem[r * G::LD + c] = a.F[t];
+      smem[G::TILE + r * G::LD + c] = a.LQ[t];
+      smem[2 * G::TILE + r * G::LD + c] = a.N1z[t];
+    }
+    __syncthreads();
   }
-  __syncthreads();
   Ctx<D> cx;
   cx.lane = threadIdx.x % G::W;
   const int wl = threadIdx.x / G::W;
   cx.wid = blockIdx.x * G::WPB + wl;
   cx.w = smem + G::CONSTS + wl * G::WORKER;
+  cx.vbase = cx.w + G::BC;
+  cx.tbase = cx.vbase + G::NV * 2 * D;
   cx.sF = smem;
   cx.sLQ = smem + G::TILE;
   cx.sN1 = smem + 2 * G::TILE;
@@ -306,8 +328,7 @@ __device__ __forceinline__ void select_row(const double (&H)[d][D], int lane, do
 template <int D, int d>
 __device__ void filter_element(const Ctx<D>& cx, const ModelArgs& a, int k, bool first, double (&A)[D],
                                double& b, double (&U)[D], double& eta, double (&Z)[D]) {
-  using G = Geo<D>;
-  constexpr int LD = G::LD;
+  constexpr int LD = D + 1;
   const int r = opaque(cx.lane);
   double H[d][D];
   double cv[d];
@@ -481,8 +502,7 @@ template <int D, bool state_only>
 __device__ void filter_combine(const Ctx<D>& cx, const FiltArrays& x1, const StateArrays& s1, size_t i1,
                                const FiltArrays& x2, size_t i2, const FiltArrays& y, const StateArrays& so,
                                size_t o) {
-  using G = Geo<D>;
-  constexpr int LD = G::LD;
+  constexpr int LD = D + 1;
   const int r = opaque(cx.lane);
   const bool act = r < D;
   const int rr = act ? r : 0;  // clamped row for branch-free loads
@@ -490,8 +510,9 @@ __device__ void filter_combine(const Ctx<D>& cx, const FiltArrays& x1, const Sta
   double* TZ2 = cx.tile(1);
   double* TX11 = cx.tile(2);
   double* TX21 = cx.tile(3);
-  double* TX22 = cx.tile(4);
-  double* TA1 = cx.tile(5);
+  // tiles 4-5 and vectors 4-7 exist only in the full-combine LDS layout
+  double* TX22 = state_only ? nullptr : cx.tile(4);
+  double* TA1 = state_only ? nullptr : cx.tile(5);
   double* Veta2 = cx.vec(0);
   double* Vb1 = cx.vec(1);
   double* Vu = cx.vec(2);
@@ -556,7 +577,7 @@ __device__ void filter_combine(const Ctx<D>& cx, const FiltArrays& x1, const Sta
 #pragma unroll
     for (int c = 0; c < D; ++c) {
       TX21[(r - D) * LD + c] = xr[c];
-      TX22[(r - D) * LD + c] = xr[D + c];
+      if constexpr (!state_only) TX22[(r - D) * LD + c] = xr[D + c];  // tile 4 exists in the full layout only
     }
   }
   wsync();
@@ -682,8 +703,7 @@ template <int D, bool state_only>
 __device__ void smoother_combine(const Ctx<D>& cx, const SmoArrays& x1, const StateArrays& s1, size_t i1,
                                  const SmoArrays& x2, size_t i2, const SmoArrays& y, const StateArrays& so,
                                  size_t o, bool with_chol) {
-  using G = Geo<D>;
-  constexpr int LD = G::LD;
+  constexpr int LD = D + 1;
   const int r = opaque(cx.lane);
   const bool act = r < D;
   const int rr = act ? r : 0;

@@ -79,6 +79,7 @@ struct IterOpts {
   int q = 0;
   double params[kMaxParams] = {};
   KernelTimer* timer = nullptr;  // non-null: record an event after every launch
+  int with_chol = 1;             // smoother scan also propagates the factors D / L
 };
 
 // Kernel entry points per (D, d) instantiation.
@@ -87,6 +88,9 @@ struct EngineFns {
   // one IEKS iteration: filter scan, smoother elements + S statistics, smoother scan,
   // smoothed means, objective, convergence partials and the reduction into ws.scal
   hipError_t (*iteration)(const Workspace& ws, const IterOpts& o, hipStream_t s);
+  // smoother scan with factors over the smoother elements of the last iteration
+  // (after means-only iterations): fills the smoothed factors read by output()
+  hipError_t (*smooth_chol)(const Workspace& ws, const IterOpts& o, hipStream_t s);
   // project smoothed states: means_out[t] = E0 m_t, chol_out[t] = scale * E0 L_t;
   // full_means / full_chols (optional) receive the preconditioned states.
   hipError_t (*output)(const Workspace& ws, int cur, double scale, double* means_out, double* chol_out,

@@ -5,11 +5,11 @@ namespace pof {
 
 const EngineFns* engines_d2(int D) {
   static const EngineFns table[] = {
-      EngineFns{4, 2, &run_iteration<4, 2>, &run_output<4, 2>, &run_linearize<4, 2>},
-      EngineFns{6, 2, &run_iteration<6, 2>, &run_output<6, 2>, &run_linearize<6, 2>},
-      EngineFns{8, 2, &run_iteration<8, 2>, &run_output<8, 2>, &run_linearize<8, 2>},
-      EngineFns{10, 2, &run_iteration<10, 2>, &run_output<10, 2>, &run_linearize<10, 2>},
-      EngineFns{12, 2, &run_iteration<12, 2>, &run_output<12, 2>, &run_linearize<12, 2>},
+      EngineFns{4, 2, &run_iteration<4, 2>, &run_smooth_chol<4, 2>, &run_output<4, 2>, &run_linearize<4, 2>},
+      EngineFns{6, 2, &run_iteration<6, 2>, &run_smooth_chol<6, 2>, &run_output<6, 2>, &run_linearize<6, 2>},
+      EngineFns{8, 2, &run_iteration<8, 2>, &run_smooth_chol<8, 2>, &run_output<8, 2>, &run_linearize<8, 2>},
+      EngineFns{10, 2, &run_iteration<10, 2>, &run_smooth_chol<10, 2>, &run_output<10, 2>, &run_linearize<10, 2>},
+      EngineFns{12, 2, &run_iteration<12, 2>, &run_smooth_chol<12, 2>, &run_output<12, 2>, &run_linearize<12, 2>},
   };
   for (const auto& e : table)
     if (e.D == D) return &e;
@@ -18,11 +18,11 @@ const EngineFns* engines_d2(int D) {
 
 size_t lds_bytes_d2(int D) {
   switch (D) {
-    case 4: return Geo<4>::lds_bytes();
-    case 6: return Geo<6>::lds_bytes();
-    case 8: return Geo<8>::lds_bytes();
-    case 10: return Geo<10>::lds_bytes();
-    case 12: return Geo<12>::lds_bytes();
+    case 4: return Geo<4, kLdsFull>::lds_bytes();
+    case 6: return Geo<6, kLdsFull>::lds_bytes();
+    case 8: return Geo<8, kLdsFull>::lds_bytes();
+    case 10: return Geo<10, kLdsFull>::lds_bytes();
+    case 12: return Geo<12, kLdsFull>::lds_bytes();
     default: return 0;
   }
 }

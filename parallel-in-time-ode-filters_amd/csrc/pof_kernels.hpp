@@ -55,9 +55,9 @@ inline StateArrays at(const StateArrays& x, size_t off, int D) {
 // ------------------------------------------------------------------ kernels
 // Leaf elements: worker k builds filter element k (k = 0 carries x0).
 template <int D, int d>
-__global__ __launch_bounds__(Geo<D>::BLOCK) void k_filter_leaf(ModelArgs a, FiltArrays out, int n) {
+__global__ __launch_bounds__((Geo<D, kLdsLeaf>::BLOCK)) void k_filter_leaf(ModelArgs a, FiltArrays out, int n) {
   extern __shared__ double smem[];
-  Ctx<D> cx = make_ctx<D>(a, smem);
+  Ctx<D> cx = make_ctx<D, kLdsLeaf>(a, smem);
   const int k = cx.wid;
   if (k >= n) return;
   double A[D], U[D], Z[D];
@@ -76,9 +76,9 @@ __global__ __launch_bounds__(Geo<D>::BLOCK) void k_filter_leaf(ModelArgs a, Filt
 
 // Up-sweep: out[i] = in[2i] o in[2i+1]
 template <int D>
-__global__ __launch_bounds__(Geo<D>::BLOCK) void k_filter_up(ModelArgs a, FiltArrays in, FiltArrays out, int n_out) {
+__global__ __launch_bounds__((Geo<D, kLdsFull>::BLOCK)) void k_filter_up(ModelArgs a, FiltArrays in, FiltArrays out, int n_out) {
   extern __shared__ double smem[];
-  Ctx<D> cx = make_ctx<D>(a, smem);
+  Ctx<D> cx = make_ctx<D, kLdsFull>(a, smem);
   const int i = cx.wid;
   if (i >= n_out) return;
   StateArrays none{};
@@ -89,10 +89,10 @@ __global__ __launch_bounds__(Geo<D>::BLOCK) void k_filter_up(ModelArgs a, FiltAr
 //   p[0] = state(e[0]); p[2i+1] = up[i]; p[2i] = up[i-1] o e[2i]  (i >= 1)
 // With top != 0 (a level of one element) only p[0] = state(e[0]) is written.
 template <int D>
-__global__ __launch_bounds__(Geo<D>::BLOCK) void k_filter_down(ModelArgs a, FiltArrays el, StateArrays up,
+__global__ __launch_bounds__((Geo<D, kLdsState>::BLOCK)) void k_filter_down(ModelArgs a, FiltArrays el, StateArrays up,
                                                                StateArrays p, int n) {
   extern __shared__ double smem[];
-  Ctx<D> cx = make_ctx<D>(a, smem);
+  Ctx<D> cx = make_ctx<D, kLdsState>(a, smem);
   const int i = cx.wid;
   const int j0 = 2 * i;
   if (j0 >= n) return;
@@ -121,12 +121,11 @@ __global__ __launch_bounds__(Geo<D>::BLOCK) void k_filter_down(ModelArgs a, Filt
 //   S = tria([H_s tria([F L_s | LQ]) | 0]), yhat = H_s F m_s + c_s,
 //   ssq_s = |S^-T yhat|^2 (utils.py:110-112), nll_s = -loglik(yhat, S) (utils.py:22-30).
 template <int D, int d>
-__global__ __launch_bounds__(Geo<D>::BLOCK) void k_filter_post(ModelArgs a, StateArrays pre0, SmoArrays sel,
+__global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK)) void k_filter_post(ModelArgs a, StateArrays pre0, SmoArrays sel,
                                                                double* t_ssq, double* t_nll) {
   extern __shared__ double smem[];
-  Ctx<D> cx = make_ctx<D>(a, smem);
-  using G = Geo<D>;
-  constexpr int LD = G::LD;
+  Ctx<D> cx = make_ctx<D, kLdsPost>(a, smem);
+  constexpr int LD = D + 1;
   const int s = cx.wid;
   const int Ns = a.Ns;
   if (s > Ns) return;
@@ -275,10 +274,10 @@ __global__ __launch_bounds__(Geo<D>::BLOCK) void k_filter_post(ModelArgs a, Stat
 
 // Smoother up-sweep
 template <int D>
-__global__ __launch_bounds__(Geo<D>::BLOCK) void k_smooth_up(ModelArgs a, SmoArrays in, SmoArrays out, int n_out,
+__global__ __launch_bounds__((Geo<D, kLdsSmooth>::BLOCK)) void k_smooth_up(ModelArgs a, SmoArrays in, SmoArrays out, int n_out,
                                                              int with_chol) {
   extern __shared__ double smem[];
-  Ctx<D> cx = make_ctx<D>(a, smem);
+  Ctx<D> cx = make_ctx<D, kLdsSmooth>(a, smem);
   const int i = cx.wid;
   if (i >= n_out) return;
   StateArrays none{};
@@ -287,10 +286,10 @@ __global__ __launch_bounds__(Geo<D>::BLOCK) void k_smooth_up(ModelArgs a, SmoArr
 }
 
 template <int D>
-__global__ __launch_bounds__(Geo<D>::BLOCK) void k_smooth_down(ModelArgs a, SmoArrays el, StateArrays up,
+__global__ __launch_bounds__((Geo<D, kLdsSmooth>::BLOCK)) void k_smooth_down(ModelArgs a, SmoArrays el, StateArrays up,
                                                                StateArrays p, int n, int with_chol) {
   extern __shared__ double smem[];
-  Ctx<D> cx = make_ctx<D>(a, smem);
+  Ctx<D> cx = make_ctx<D, kLdsSmooth>(a, smem);
   const int i = cx.wid;
   const int j0 = 2 * i;
   if (j0 >= n) return;
@@ -492,10 +491,13 @@ __global__ void k_linearize(ModelArgs a, const double* m, int n, double* H, doub
 }
 
 // ------------------------------------------------------------ launch logic
-template <int D>
+template <int D, int K>
 inline unsigned grid_workers(size_t nworkers) {
-  return (unsigned)((nworkers + Geo<D>::WPB - 1) / Geo<D>::WPB);
+  return (unsigned)((nworkers + Geo<D, K>::WPB - 1) / Geo<D, K>::WPB);
 }
+#define POF_LAUNCH(kernel, K, nworkers, ...)                                                        \
+  hipLaunchKernelGGL(kernel, dim3((grid_workers<D, K>(nworkers))), dim3((Geo<D, K>::BLOCK)),        \
+                     (Geo<D, K>::lds_bytes()), __VA_ARGS__)
 
 namespace {
 // With a timer, event i+1 is recorded after launch i (event 0 before the first).
@@ -518,28 +520,51 @@ struct Rec {
 };
 }  // namespace
 
+template <int D>
+void smooth_scan(const Workspace& ws, const ModelArgs& a, int wc, hipStream_t st, Rec& rec) {
+  for (int l = 0; l + 1 < ws.s_levels; ++l) {
+    const int nout = (int)ws.s_n[l + 1];
+    POF_LAUNCH((k_smooth_up<D>), kLdsSmooth, nout, st, a,
+                       at(ws.sel, ws.s_off[l], D), at(ws.sel, ws.s_off[l + 1], D), nout, wc);
+    rec(kKSmoothUp, nout);
+  }
+  for (int l = ws.s_levels - 1; l >= 0; --l) {
+    const int n = (int)ws.s_n[l];
+    StateArrays up = (l + 1 < ws.s_levels) ? at(ws.spre, ws.s_off[l + 1], D) : StateArrays{};
+    POF_LAUNCH((k_smooth_down<D>), kLdsSmooth, (n + 1) / 2, st, a,
+                       at(ws.sel, ws.s_off[l], D), up, at(ws.spre, ws.s_off[l], D), n, wc);
+    rec(kKSmoothDown, (n + 1) / 2 - 1);
+  }
+}
+
+template <int D, int d>
+hipError_t run_smooth_chol(const Workspace& ws, const IterOpts& o, hipStream_t st) {
+  const ModelArgs a = model_args(ws, o);
+  Rec rec(o.timer, st);
+  smooth_scan<D>(ws, a, 1, st, rec);
+  return hipGetLastError();
+}
+
 template <int D, int d>
 hipError_t run_iteration(const Workspace& ws, const IterOpts& o, hipStream_t st) {
-  using G = Geo<D>;
   const ModelArgs a = model_args(ws, o);
-  const size_t lds = G::lds_bytes();
   const int Ns = ws.Ns;
   Rec rec(o.timer, st);
   // ---- filter: leaf elements, up-sweep, down-sweep
   {
     FiltArrays l0 = at(ws.fel, ws.f_off[0], D);
-    hipLaunchKernelGGL((k_filter_leaf<D, d>), dim3(grid_workers<D>(Ns)), dim3(G::BLOCK), lds, st, a, l0, Ns);
+    POF_LAUNCH((k_filter_leaf<D, d>), kLdsLeaf, Ns, st, a, l0, Ns);
     rec(kKFilterLeaf, Ns);
     for (int l = 0; l + 1 < ws.f_levels; ++l) {
       const int nout = (int)ws.f_n[l + 1];
-      hipLaunchKernelGGL((k_filter_up<D>), dim3(grid_workers<D>(nout)), dim3(G::BLOCK), lds, st, a,
+      POF_LAUNCH((k_filter_up<D>), kLdsFull, nout, st, a,
                          at(ws.fel, ws.f_off[l], D), at(ws.fel, ws.f_off[l + 1], D), nout);
       rec(kKFilterUp, nout);
     }
     for (int l = ws.f_levels - 1; l >= 0; --l) {
       const int n = (int)ws.f_n[l];
       StateArrays up = (l + 1 < ws.f_levels) ? at(ws.fpre, ws.f_off[l + 1], D) : StateArrays{};
-      hipLaunchKernelGGL((k_filter_down<D>), dim3(grid_workers<D>((n + 1) / 2)), dim3(G::BLOCK), lds, st, a,
+      POF_LAUNCH((k_filter_down<D>), kLdsState, (n + 1) / 2, st, a,
                          at(ws.fel, ws.f_off[l], D), up, at(ws.fpre, ws.f_off[l], D), n);
       rec(kKFilterDown, (n + 1) / 2 - 1);
     }
@@ -547,27 +572,12 @@ hipError_t run_iteration(const Workspace& ws, const IterOpts& o, hipStream_t st)
   // ---- smoother elements + observation statistics
   {
     SmoArrays s0 = at(ws.sel, ws.s_off[0], D);
-    hipLaunchKernelGGL((k_filter_post<D, d>), dim3(grid_workers<D>(Ns + 1)), dim3(G::BLOCK), lds, st, a,
+    POF_LAUNCH((k_filter_post<D, d>), kLdsPost, Ns + 1, st, a,
                        at(ws.fpre, ws.f_off[0], D), s0, ws.t_ssq, ws.t_nll);
     rec(kKFilterPost, Ns);
   }
   // ---- smoother scan (reversed order)
-  {
-    const int wc = 1;
-    for (int l = 0; l + 1 < ws.s_levels; ++l) {
-      const int nout = (int)ws.s_n[l + 1];
-      hipLaunchKernelGGL((k_smooth_up<D>), dim3(grid_workers<D>(nout)), dim3(G::BLOCK), lds, st, a,
-                         at(ws.sel, ws.s_off[l], D), at(ws.sel, ws.s_off[l + 1], D), nout, wc);
-      rec(kKSmoothUp, nout);
-    }
-    for (int l = ws.s_levels - 1; l >= 0; --l) {
-      const int n = (int)ws.s_n[l];
-      StateArrays up = (l + 1 < ws.s_levels) ? at(ws.spre, ws.s_off[l + 1], D) : StateArrays{};
-      hipLaunchKernelGGL((k_smooth_down<D>), dim3(grid_workers<D>((n + 1) / 2)), dim3(G::BLOCK), lds, st, a,
-                         at(ws.sel, ws.s_off[l], D), up, at(ws.spre, ws.s_off[l], D), n, wc);
-      rec(kKSmoothDown, (n + 1) / 2 - 1);
-    }
-  }
+  smooth_scan<D>(ws, a, o.with_chol, st, rec);
   // ---- smoothed means, objective, convergence partials, reduction
   {
     const int nt = Ns + 1;
