@@ -595,7 +595,9 @@ __device__ void filter_combine(const Ctx<D>& cx, const FiltArrays& x1, const Sta
     pin1(wr[c]);
   }
 
-  // b = W (b1 + U1 U1^T eta2) + b2
+  // b = W (b1 + U1 U1^T eta2) + b2   (all global stores are issued at the end:
+  // a store in flight would make the next wait on a load also wait for it)
+  double bb;
   {
     double u = 0.0;
 #pragma unroll
@@ -607,13 +609,12 @@ __device__ void filter_combine(const Ctx<D>& cx, const FiltArrays& x1, const Sta
     for (int j = 0; j < D; ++j) sv = fma(TU1[rr * LD + j], Vu[j], sv);
     if (act) Vs[r] = sv;
     wsync();
-    double bb = b2r;
+    bb = b2r;
 #pragma unroll
     for (int k = 0; k < D; ++k) bb = fma(wr[k], Vs[k], bb);
-    double* bo = state_only ? so.m : y.b;
-    if (act) bo[o * D + r] = bb;
   }
   // U = tria([MT | U2])
+  double uo[D];
   {
     double ur[2 * D];
 #pragma unroll
@@ -621,11 +622,8 @@ __device__ void filter_combine(const Ctx<D>& cx, const FiltArrays& x1, const Sta
 #pragma unroll
     for (int c = 0; c < D; ++c) ur[D + c] = u2[c];
     lq<D, 2 * D>(ur, r, cx.bc());
-    if (act) {
-      double* Uo = state_only ? so.L : y.U;
 #pragma unroll
-      for (int c = 0; c < D; ++c) Uo[(o * D + r) * D + c] = ur[c];
-    }
+    for (int c = 0; c < D; ++c) uo[c] = ur[c];
   }
   if constexpr (!state_only) {
     double* Vt = cx.vec(4);
@@ -633,50 +631,41 @@ __device__ void filter_combine(const Ctx<D>& cx, const FiltArrays& x1, const Sta
     double* Vu2 = cx.vec(6);
     double* Vs3 = cx.vec(7);
     // A = W A1
-    if (act) {
-      double ar[D];
-      row_times_tile<D, D, LD>(wr, TA1, ar);
-      store_row(y.A + (o * D + r) * D, ar);
-    }
+    double ar[D];
+    row_times_tile<D, D, LD>(wr, TA1, ar);
     // eta = A1^T (I - Xi21 Xi11^-1 U1^T)(eta2 - Z2 Z2^T b1) + eta1
-    if (act) {
+    {
       double t = 0.0;
 #pragma unroll
-      for (int k = 0; k < D; ++k) t = fma(TZ2[k * LD + r], Vb1[k], t);
-      Vt[r] = t;
+      for (int k = 0; k < D; ++k) t = fma(TZ2[k * LD + rr], Vb1[k], t);
+      if (act) Vt[r] = t;
     }
     wsync();
-    double s2 = 0.0;
-    if (act) {
-      s2 = eta2r;
+    double s2 = eta2r;
 #pragma unroll
-      for (int j = 0; j < D; ++j) s2 = fma(-TZ2[r * LD + j], Vt[j], s2);
-      Vs2[r] = s2;
-    }
+    for (int j = 0; j < D; ++j) s2 = fma(-TZ2[rr * LD + j], Vt[j], s2);
+    if (act) Vs2[r] = s2;
     wsync();
-    if (act) {
+    {
       double u2 = 0.0;
 #pragma unroll
-      for (int k = 0; k < D; ++k) u2 = fma(TU1[k * LD + r], Vs2[k], u2);
-      Vu2[r] = u2;
+      for (int k = 0; k < D; ++k) u2 = fma(TU1[k * LD + rr], Vs2[k], u2);
+      if (act) Vu2[r] = u2;
     }
     wsync();
     double w2[D];
     load_row(w2, Vu2);
     lsolve_lower<D, LD>(w2, TX11);
-    if (act) {
+    {
       double v2 = 0.0;
 #pragma unroll
-      for (int j = 0; j < D; ++j) v2 = fma(TX21[r * LD + j], w2[j], v2);
-      Vs3[r] = s2 - v2;
+      for (int j = 0; j < D; ++j) v2 = fma(TX21[rr * LD + j], w2[j], v2);
+      if (act) Vs3[r] = s2 - v2;
     }
     wsync();
-    if (act) {
-      double e = eta1r;
+    double eo = eta1r;
 #pragma unroll
-      for (int k = 0; k < D; ++k) e = fma(TA1[k * LD + r], Vs3[k], e);
-      y.eta[o * D + r] = e;
-    }
+    for (int k = 0; k < D; ++k) eo = fma(TA1[k * LD + rr], Vs3[k], eo);
     // Z = tria([A1^T Xi22 | Z1])
     double zr[2 * D];
     {
@@ -689,7 +678,15 @@ __device__ void filter_combine(const Ctx<D>& cx, const FiltArrays& x1, const Sta
       }
     }
     lq<D, 2 * D>(zr, r, cx.bc());
-    if (act) store_row(y.Z + (o * D + r) * D, *reinterpret_cast<double(*)[D]>(zr));
+    if (act) {
+      store_row(y.A + (o * D + r) * D, ar);
+      y.eta[o * D + r] = eo;
+      store_row(y.Z + (o * D + r) * D, *reinterpret_cast<double(*)[D]>(zr));
+    }
+  }
+  if (act) {
+    (state_only ? so.m : y.b)[o * D + r] = bb;
+    store_row((state_only ? so.L : y.U) + (o * D + r) * D, uo);
   }
   wsync();
 }
@@ -735,19 +732,15 @@ __device__ void smoother_combine(const Ctx<D>& cx, const SmoArrays& x1, const St
     }
   }
   wsync();
-  if (act) {
-    double gg = g2r;
+  // results stay in registers; every global store is issued at the end so
+  // that no wait on outstanding memory operations lands inside the LQ below
+  double gg = g2r;
 #pragma unroll
-    for (int k = 0; k < D; ++k) gg = fma(e2[k], Vg1[k], gg);
-    (state_only ? so.m : y.g)[o * D + r] = gg;
-    if (!state_only) {
-      double er[D];
-      row_times_tile<D, D, LD>(e2, TE1, er);
-      store_row(y.E + (o * D + r) * D, er);
-    }
-  }
+  for (int k = 0; k < D; ++k) gg = fma(e2[k], Vg1[k], gg);
+  double er[state_only ? 1 : D];
+  if constexpr (!state_only) row_times_tile<D, D, LD>(e2, TE1, er);
+  double dr[2 * D];
   if (with_chol) {
-    double dr[2 * D];
     {
       double t[D];
       row_times_tile<D, D, LD>(e2, TD1, t);
@@ -758,7 +751,11 @@ __device__ void smoother_combine(const Ctx<D>& cx, const SmoArrays& x1, const St
       }
     }
     lq<D, 2 * D>(dr, r, cx.bc());
-    if (act) {
+  }
+  if (act) {
+    (state_only ? so.m : y.g)[o * D + r] = gg;
+    if constexpr (!state_only) store_row(y.E + (o * D + r) * D, er);
+    if (with_chol) {
       double* Do = state_only ? so.L : y.Dm;
 #pragma unroll
       for (int c = 0; c < D; ++c) Do[(o * D + r) * D + c] = dr[c];

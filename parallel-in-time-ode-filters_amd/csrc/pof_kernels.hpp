@@ -211,18 +211,8 @@ __global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK)) void k_filter_post(Model
     }
   }
   wsync();
-  if (act) {
-    double e[D];
-    load_row(e, TE + r * LD);
-    double gg = Vm[r];
-#pragma unroll
-    for (int k = 0; k < D; ++k) gg = fma(-e[k], Vfm[k], gg);
-    sel.g[ro * D + r] = gg;
-    store_row(sel.E + (ro * D + r) * D, e);
-#pragma unroll
-    for (int c = 0; c < D; ++c) sel.Dm[(ro * D + r) * D + c] = TD[r * LD + c];
-  }
-  // observation statistics with obs model s
+  // observation statistics with obs model s (the smoother element stays in
+  // LDS until the end so that no store is outstanding during the LQ below)
   double sr[D + d];
   {
     double h[D];
@@ -268,6 +258,17 @@ __global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK)) void k_filter_post(Model
     t_ssq[s] = ss;
     // -loglik = 0.5 |S^-1 y|^2 + sum log|S_ii| + d log(2 pi) / 2
     t_nll[s] = 0.5 * nn + (ld + d * log(2.0 * M_PI) / 2.0);
+  }
+  if (act) {
+    double e[D];
+    load_row(e, TE + r * LD);
+    double gg = Vm[r];
+#pragma unroll
+    for (int k = 0; k < D; ++k) gg = fma(-e[k], Vfm[k], gg);
+    sel.g[ro * D + r] = gg;
+    store_row(sel.E + (ro * D + r) * D, e);
+#pragma unroll
+    for (int c = 0; c < D; ++c) sel.Dm[(ro * D + r) * D + c] = TD[r * LD + c];
   }
   wsync();
 }
