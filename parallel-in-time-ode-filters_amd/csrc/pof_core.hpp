@@ -50,7 +50,7 @@ struct Geo {
   static constexpr int TILE = D_ * LD;
   static constexpr int NT = LdsNeeds<K>::NT;             // tiles per worker
   static constexpr int NV = LdsNeeds<K>::NV;             // broadcast vectors per worker
-  static constexpr int BC = 2 * D_ + 2;                  // Householder broadcast row
+  static constexpr int BC = 4 * D_ + 2;                  // Householder broadcast rows (two matrices)
   static constexpr int WORKER = BC + NV * 2 * D_ + NT * TILE;
   static constexpr int CONSTS = LdsNeeds<K>::CONSTS ? 3 * TILE : 0;  // F, LQ, N1z
   static constexpr size_t bytes_for(int wpb) { return sizeof(double) * (CONSTS + wpb * WORKER); }
@@ -176,54 +176,84 @@ __device__ __forceinline__ void tile_t_times_tile(int r, const double* T1, const
   }
 }
 
-// Householder LQ of the NR x NC matrix whose row r sits in lane r (r < NR):
-// on return lane r holds row r of the lower-triangular factor L (columns
-// >= min(NR, NC) are zero), L L^T = M M^T.  Same reflectors as LAPACK dgeqrf
-// on M^T.
-template <int NR, int NC>
+// 1/x to about one ulp: the hardware estimate refined by two Newton steps
+// (x is a nonzero norm-sized value here; no denormal or overflow handling).
+__device__ __forceinline__ double rcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// Householder LQ of NG independent NR x NC matrices; row r of matrix g sits
+// in lane g * NR + r.  On return that lane holds row r of the lower-
+// triangular factor L of its matrix (columns > r are zero), L L^T = M M^T:
+// the reflectors of LAPACK dgeqrf on M^T (dlarfg: beta = -sign(alpha) *
+// norm, tau = (beta - alpha) / beta, v = [1, x / (alpha - beta)], tau = 0
+// when the tail x is zero, no reflector for a length-1 tail).
+//
+// Per pivot step the pivot lane broadcasts its raw row through LDS and every
+// lane derives (beta, tau, scale) itself, so there is no divergent pivot
+// section; the dot products with the unscaled tail overlap the square root
+// and reciprocals.  Finished rows are not cleared per step; the strict upper
+// part is zeroed once at the end.
+template <int NR, int NC, int NG = 1>
 __device__ __forceinline__ void lq(double (&a)[NC], int lane_in, double* bc) {
+  static_assert(NG == 1 || NG == 2, "one or two matrices");
   constexpr int K = (NR < NC) ? NR : NC;
   // opaque copy of the lane index: stops the optimiser from threading the
   // per-pivot lane tests into specialised copies of the surrounding code
   int lane = lane_in;
   asm volatile("" : "+v"(lane));
+  const int g = (NG == 2 && lane >= NR) ? 1 : 0;
+  const int li = lane - g * NR;
+  const bool valid = lane < NG * NR;
+  double* row = bc + g * NC;  // broadcast row of this lane's matrix
 #pragma unroll
   for (int i = 0; i < K; ++i) {
     if (i == NC - 1) break;  // a length-1 tail: H = I
-    if (lane == i) {
-      const double alpha = a[i];
-      // sum of squares of the tail in four independent chains (latency)
-      double sq[4] = {0.0, 0.0, 0.0, 0.0};
+    if (valid && li == i) {
 #pragma unroll
-      for (int j = i + 1; j < NC; ++j) sq[(j - i - 1) & 3] = fma(a[j], a[j], sq[(j - i - 1) & 3]);
-      const double sig = (sq[0] + sq[1]) + (sq[2] + sq[3]);
-      double tau = 0.0, beta = alpha, scal = 0.0;
-      if (sig != 0.0) {
-        const double nrm = sqrt(fma(alpha, alpha, sig));
-        beta = -copysign(nrm, alpha);  // Fortran SIGN: honours the sign of -0.0
-        tau = (beta - alpha) / beta;
-        scal = 1.0 / (alpha - beta);
-      }
-      bc[i] = tau;
-#pragma unroll
-      for (int j = i + 1; j < NC; ++j) bc[j] = a[j] * scal;
-      a[i] = beta;
-#pragma unroll
-      for (int j = i + 1; j < NC; ++j) a[j] = 0.0;
+      for (int j = i; j < NC; ++j) row[j] = a[j];
     }
     wsync();
-    if (lane > i && lane < NR) {
-      const double tau = bc[i];
-      double wp[4] = {a[i], 0.0, 0.0, 0.0};
+    double p[NC];
 #pragma unroll
-      for (int j = i + 1; j < NC; ++j) wp[(j - i) & 3] = fma(a[j], bc[j], wp[(j - i) & 3]);
-      const double tw = tau * ((wp[0] + wp[1]) + (wp[2] + wp[3]));
+    for (int j = i; j < NC; ++j) p[j] = row[j];
+    const double alpha = p[i];
+    // sum of squares of the pivot tail and this row's dot with it, in four
+    // independent chains each
+    double sq[4] = {0.0, 0.0, 0.0, 0.0};
+    double dt[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = i + 1; j < NC; ++j) {
+      sq[(j - i - 1) & 3] = fma(p[j], p[j], sq[(j - i - 1) & 3]);
+      dt[(j - i - 1) & 3] = fma(a[j], p[j], dt[(j - i - 1) & 3]);
+    }
+    const double sig = (sq[0] + sq[1]) + (sq[2] + sq[3]);
+    const double dot = (dt[0] + dt[1]) + (dt[2] + dt[3]);
+    const double nrm = sqrt(fma(alpha, alpha, sig));
+    const double bc_ = -copysign(nrm, alpha);  // Fortran SIGN: honours the sign of -0.0
+    const double den = alpha - bc_;
+    const double sc_ = rcp(den);
+    const double tc_ = -den * rcp(bc_);
+    const bool nz = sig != 0.0;
+    const double beta = nz ? bc_ : alpha;
+    const double scal = nz ? sc_ : 0.0;
+    const double tau = nz ? tc_ : 0.0;
+    if (valid && li == i) a[i] = beta;
+    if (valid && li > i) {
+      const double tw = tau * fma(scal, dot, a[i]);
       a[i] -= tw;
+      const double tws = tw * scal;
 #pragma unroll
-      for (int j = i + 1; j < NC; ++j) a[j] = fma(-tw, bc[j], a[j]);
+      for (int j = i + 1; j < NC; ++j) a[j] = fma(-tws, p[j], a[j]);
     }
-    wsync();
   }
+#pragma unroll
+  for (int j = 1; j < NC; ++j) a[j] = (valid && j > li) ? 0.0 : a[j];
+  wsync();
 }
 
 // y L = v with L lower triangular (n x n in LDS, stride LD); y holds v on entry.
@@ -506,6 +536,9 @@ __device__ void filter_combine(const Ctx<D>& cx, const FiltArrays& x1, const Sta
   const int r = opaque(cx.lane);
   const bool act = r < D;
   const int rr = act ? r : 0;  // clamped row for branch-free loads
+  // the full combine factors U (lanes 0..D-1) and Z (lanes D..2D-1) together
+  const bool zl = !state_only && r >= D && r < 2 * D;
+  const int rz = zl ? r - D : 0;
   double* TU1 = cx.tile(0);
   double* TZ2 = cx.tile(1);
   double* TX11 = cx.tile(2);
@@ -535,7 +568,7 @@ __device__ void filter_combine(const Ctx<D>& cx, const FiltArrays& x1, const Sta
     load_row(u2, x2.U + (i2 * D + rr) * D);
     if constexpr (!state_only) {
       load_row(a1, x1.A + (i1 * D + rr) * D);
-      load_row(z1, x1.Z + (i1 * D + rr) * D);
+      load_row(z1, x1.Z + (i1 * D + rz) * D);  // Z1 rows feed the lanes D..2D-1
       eta1r = x1.eta[i1 * D + rr];
     }
     if (act) {
@@ -613,27 +646,15 @@ __device__ void filter_combine(const Ctx<D>& cx, const FiltArrays& x1, const Sta
 #pragma unroll
     for (int k = 0; k < D; ++k) bb = fma(wr[k], Vs[k], bb);
   }
-  // U = tria([MT | U2])
-  double uo[D];
-  {
-    double ur[2 * D];
-#pragma unroll
-    for (int c = 0; c < D; ++c) ur[c] = mt[c];
-#pragma unroll
-    for (int c = 0; c < D; ++c) ur[D + c] = u2[c];
-    lq<D, 2 * D>(ur, r, cx.bc());
-#pragma unroll
-    for (int c = 0; c < D; ++c) uo[c] = ur[c];
-  }
+  // A = W A1 and eta = A1^T (I - Xi21 Xi11^-1 U1^T)(eta2 - Z2 Z2^T b1) + eta1
+  double ar[state_only ? 1 : D];
+  double eo = 0.0;
   if constexpr (!state_only) {
     double* Vt = cx.vec(4);
     double* Vs2 = cx.vec(5);
     double* Vu2 = cx.vec(6);
     double* Vs3 = cx.vec(7);
-    // A = W A1
-    double ar[D];
     row_times_tile<D, D, LD>(wr, TA1, ar);
-    // eta = A1^T (I - Xi21 Xi11^-1 U1^T)(eta2 - Z2 Z2^T b1) + eta1
     {
       double t = 0.0;
 #pragma unroll
@@ -647,10 +668,10 @@ __device__ void filter_combine(const Ctx<D>& cx, const FiltArrays& x1, const Sta
     if (act) Vs2[r] = s2;
     wsync();
     {
-      double u2 = 0.0;
+      double uu = 0.0;
 #pragma unroll
-      for (int k = 0; k < D; ++k) u2 = fma(TU1[k * LD + rr], Vs2[k], u2);
-      if (act) Vu2[r] = u2;
+      for (int k = 0; k < D; ++k) uu = fma(TU1[k * LD + rr], Vs2[k], uu);
+      if (act) Vu2[r] = uu;
     }
     wsync();
     double w2[D];
@@ -663,30 +684,41 @@ __device__ void filter_combine(const Ctx<D>& cx, const FiltArrays& x1, const Sta
       if (act) Vs3[r] = s2 - v2;
     }
     wsync();
-    double eo = eta1r;
+    eo = eta1r;
 #pragma unroll
     for (int k = 0; k < D; ++k) eo = fma(TA1[k * LD + rr], Vs3[k], eo);
-    // Z = tria([A1^T Xi22 | Z1])
-    double zr[2 * D];
-    {
-      double t[D];
-      tile_t_times_tile<D, D, LD>(rr, TA1, TX22, t);
-#pragma unroll
-      for (int c = 0; c < D; ++c) {
-        zr[c] = t[c];
-        zr[D + c] = z1[c];
-      }
-    }
-    lq<D, 2 * D>(zr, r, cx.bc());
-    if (act) {
-      store_row(y.A + (o * D + r) * D, ar);
-      y.eta[o * D + r] = eo;
-      store_row(y.Z + (o * D + r) * D, *reinterpret_cast<double(*)[D]>(zr));
-    }
   }
+  // U = tria([MT | U2]) in lanes 0..D-1 and, for the full combine,
+  // Z = tria([A1^T Xi22 | Z1]) in lanes D..2D-1, as one paired LQ
+  double jr[2 * D];
+  if (act) {
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      jr[c] = mt[c];
+      jr[D + c] = u2[c];
+    }
+  } else if (zl) {
+    double t[D];
+    tile_t_times_tile<D, D, LD>(rz, TA1, TX22, t);
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      jr[c] = t[c];
+      jr[D + c] = z1[c];
+    }
+  } else {
+    zero(jr);
+  }
+  lq<D, 2 * D, state_only ? 1 : 2>(jr, r, cx.bc());
+  const double(&lo)[D] = *reinterpret_cast<const double(*)[D]>(jr);
   if (act) {
     (state_only ? so.m : y.b)[o * D + r] = bb;
-    store_row((state_only ? so.L : y.U) + (o * D + r) * D, uo);
+    store_row((state_only ? so.L : y.U) + (o * D + r) * D, lo);
+    if constexpr (!state_only) {
+      store_row(y.A + (o * D + r) * D, ar);
+      y.eta[o * D + r] = eo;
+    }
+  } else if (zl) {
+    store_row(y.Z + (o * D + rz) * D, lo);
   }
   wsync();
 }
