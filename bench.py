@@ -29,6 +29,7 @@ METRIC = "ODE solves/s + p50/p99 latency, batched vs plain GPU scan at 1e-6 rel 
 NS = 65536
 Q = 3
 ITERS = 10
+TRAFFIC_SUMMARY = ROOT / "profiles" / "ieks_lorenz63_Ns65536_q3_f64_hbm_traffic.md"
 
 
 def gpu_run(steps, warmup):
@@ -89,6 +90,11 @@ def gpu_run(steps, warmup):
     flops_per_launch = roofline.flops_per_unit(dom, D, d) * work[roofline.KERNEL_CLASSES.index(dom)] / c["launches"]
     achieved = flops_per_launch / (c["mean_launch_us"] * 1e-6) / 1e12
     total_ms = sum(v["ms"] for v in per_class.values())
+    # HBM bytes per launch of the dominant kernel, from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE summary of this workload (null if absent)
+    from pof.profiling import read_traffic
+
+    traffic = read_traffic(str(TRAFFIC_SUMMARY), f"k_{dom}<{D}")
     iter_flops = roofline.iteration_flops(NS, D, d)
     return {
         "seconds": dt,
@@ -103,10 +109,13 @@ def gpu_run(steps, warmup):
             "peak": roofline.PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / roofline.PEAK_FP64_TFLOPS,
-            "traffic": None,
+            "traffic": traffic,
             "kernel": dom,
             "flops_per_launch": flops_per_launch,
             "mean_launch_us": c["mean_launch_us"],
+            "algorithmic_bytes_per_launch": roofline.bytes_per_unit(dom, D, d)
+            * work[roofline.KERNEL_CLASSES.index(dom)] / c["launches"],
+            "traffic_source": TRAFFIC_SUMMARY.name if traffic is not None else None,
         },
         "instrumented_iteration_ms": total_ms / ITERS,
         "iteration_tflops_algorithmic": iter_flops / (total_ms / ITERS * 1e-3) / 1e12,
@@ -150,6 +159,23 @@ def cpu_baseline():
     }
 
 
+def copy_probe():
+    """A plain device copy of known size (2^25 float64 = 256 MiB read + 256 MiB written per launch).
+
+    Run under ``rocprofv3 --pmc FETCH_SIZE`` / ``WRITE_SIZE`` to calibrate those
+    counters against a kernel whose HBM traffic is known.
+    """
+    import torch
+
+    n = 1 << 25
+    x = torch.rand(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": 8 * n, "copy_probe_bytes_written": 8 * n, "launches": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -157,11 +183,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true", help="only run the HBM counter calibration copy")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part of this build",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe()
+        return
     g = gpu_run(args.steps, args.warmup)
     if args.verbose:
         print(json.dumps(g["per_class"], indent=1), file=sys.stderr)

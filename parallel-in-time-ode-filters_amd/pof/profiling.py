@@ -1,0 +1,138 @@
+"""Summaries of rocprofv3 output for the IEKS kernels.
+
+* ``stats``: the ``--kernel-trace --stats`` per-kernel table as Markdown.
+* ``traffic``: HBM bytes per launch of every kernel from two counter runs
+  (``--pmc FETCH_SIZE`` and ``--pmc WRITE_SIZE``, KiB per dispatch), each
+  scaled by what the same counter reads for a plain device copy of known
+  size (``bench.py --copy-probe``: 2^25 float64 read and written per launch).
+
+``read_traffic`` returns the bytes per launch of one kernel from a committed
+traffic summary, for ``bench.py``'s roofline record.
+
+Usage::
+
+    python -m pof.profiling stats RUN_kernel_stats.csv --command "..." > summary.md
+    python -m pof.profiling traffic --fetch F.csv --write W.csv --copy-fetch CF.csv \
+        --copy-write CW.csv --copy-bytes 268435456 --command "..." > traffic.md
+"""
+import argparse
+import collections
+import csv
+import re
+import sys
+
+COPY_KERNEL = "__amd_rocclr_copyBuffer"
+
+
+def short_name(name):
+    return name.split("(")[0].replace("void ", "")
+
+
+def counter_means(path):
+    """{kernel name: (dispatches, mean counter value)} of a rocprofv3 counter_collection.csv."""
+    agg = collections.defaultdict(lambda: [0, 0.0])
+    with open(path) as fh:
+        for r in csv.DictReader(fh):
+            a = agg[r["Kernel_Name"]]
+            a[0] += 1
+            a[1] += float(r["Counter_Value"])
+    return {k: (n, s / n) for k, (n, s) in agg.items()}
+
+
+def copy_scale(path, copy_bytes):
+    """Known bytes / counter bytes of the calibration copy (largest copy kernel of the run)."""
+    means = counter_means(path)
+    cands = [v for k, v in means.items() if COPY_KERNEL in k]
+    if not cands:
+        raise SystemExit(f"no {COPY_KERNEL} dispatch in {path}")
+    kib = max(v[1] for v in cands)
+    return copy_bytes / (kib * 1024.0)
+
+
+def traffic_rows(fetch, write, copy_fetch, copy_write, copy_bytes, match="pof::"):
+    sf = copy_scale(copy_fetch, copy_bytes)
+    sw = copy_scale(copy_write, copy_bytes)
+    f = counter_means(fetch)
+    w = counter_means(write)
+    rows = []
+    for k in sorted(f, key=lambda k: -f[k][0] * f[k][1]):
+        if match not in k or k not in w:
+            continue
+        n, fk = f[k]
+        _, wk = w[k]
+        fb = fk * 1024.0 * sf
+        wb = wk * 1024.0 * sw
+        rows.append((short_name(k), n, fb, wb, fb + wb))
+    return sf, sw, rows
+
+
+def traffic_markdown(fetch, write, copy_fetch, copy_write, copy_bytes, command=""):
+    sf, sw, rows = traffic_rows(fetch, write, copy_fetch, copy_write, copy_bytes)
+    out = ["# HBM traffic per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE)", ""]
+    if command:
+        out += ["Commands:", "", "```", command, "```", ""]
+    out += [
+        f"Calibration: for a device copy that reads and writes {int(copy_bytes)} bytes per launch, "
+        f"true bytes = FETCH_SIZE x {sf:.4f} and WRITE_SIZE x {sw:.4f} (counters in KiB); every row "
+        f"below is scaled by these factors and averaged over the launches of the profiled run.",
+        "",
+        f"fetch_scale={sf:.6f} write_scale={sw:.6f}",
+        "",
+        "| kernel | launches | fetch MB/launch | write MB/launch | bytes_per_launch |",
+        "|---|---|---|---|---|",
+    ]
+    for name, n, fb, wb, tb in rows:
+        out.append(f"| `{name}` | {n} | {fb / 1e6:.3f} | {wb / 1e6:.3f} | {tb:.0f} |")
+    return "\n".join(out) + "\n"
+
+
+def stats_markdown(path, command=""):
+    out = ["# Kernel time (rocprofv3 --kernel-trace --stats)", ""]
+    if command:
+        out += ["Command:", "", "```", command, "```", ""]
+    out += ["| kernel | calls | total ms | average us | min us | max us | % |", "|---|---|---|---|---|---|---|"]
+    with open(path) as fh:
+        for r in csv.DictReader(fh):
+            out.append(
+                f"| `{short_name(r['Name'])[:90]}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "
+                f"{float(r['AverageNs']) / 1e3:.2f} | {float(r['MinNs']) / 1e3:.2f} | {float(r['MaxNs']) / 1e3:.2f} | "
+                f"{float(r['Percentage']):.2f} |"
+            )
+    return "\n".join(out) + "\n"
+
+
+def read_traffic(md_path, kernel):
+    """Bytes per launch of the first kernel whose name contains ``kernel``, or None."""
+    try:
+        text = open(md_path).read()
+    except OSError:
+        return None
+    for line in text.splitlines():
+        m = re.match(r"\| `([^`]*)` \| (\d+) \| [\d.]+ \| [\d.]+ \| (\d+) \|", line)
+        if m and kernel in m.group(1):
+            return float(m.group(3))
+    return None
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(prog="python -m pof.profiling")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    s = sub.add_parser("stats")
+    s.add_argument("csv")
+    s.add_argument("--command", default="")
+    t = sub.add_parser("traffic")
+    t.add_argument("--fetch", required=True)
+    t.add_argument("--write", required=True)
+    t.add_argument("--copy-fetch", required=True)
+    t.add_argument("--copy-write", required=True)
+    t.add_argument("--copy-bytes", type=float, default=float(8 << 25))
+    t.add_argument("--command", default="")
+    a = ap.parse_args(argv)
+    if a.cmd == "stats":
+        sys.stdout.write(stats_markdown(a.csv, a.command))
+    else:
+        sys.stdout.write(traffic_markdown(a.fetch, a.write, a.copy_fetch, a.copy_write, a.copy_bytes, a.command))
+
+
+if __name__ == "__main__":
+    main()
