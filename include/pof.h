@@ -29,6 +29,7 @@ extern "C" {
 #define POF_E_UNSUPPORTED -2 /* no kernel instantiation for (D, d) */
 #define POF_E_HIP -3        /* HIP runtime error */
 #define POF_E_INIT -4       /* unknown initial-trajectory method */
+#define POF_E_FULL -5       /* batch has no free IVP slot or entry range right now */
 
 /* ODE vector fields available on the device (pof/ivp.py of the reference,
  * plus fhn_classic, lorenz63, lorenz96). */
@@ -58,6 +59,7 @@ extern "C" {
 #define POF_STATUS_FIXED 3     /* ran exactly fixed_iters iterations */
 
 typedef struct pof_ctx pof_ctx;
+typedef struct pof_batch pof_batch;
 
 /* One IVP y' = f(y) on the grid ts (pof/solver.py:11-13 arguments f, y0, ts, order, init). */
 typedef struct {
@@ -155,6 +157,40 @@ int pof_linear_filtsmooth(pof_ctx* ctx, int D, int d, int64_t Ns, const double* 
                           const double* H_dev, const double* c_dev, double* smooth_mean_dev,
                           double* smooth_chol_dev, double* filt_mean_dev, double* filt_chol_dev, double* nll,
                           double* obj, double* ssq, void* stream);
+
+/* ------------------------------------------------------------------ batches
+ * Many IVPs of one state dimension D = d (q + 1) solved together: every scan
+ * level of every IVP in the group runs in one launch, each IVP keeps its own
+ * iterate state in HBM and its own convergence test, and IVPs can be admitted
+ * and retired between iterations (continuous batching).  No counterpart in
+ * the reference, which solves one IVP per call; each IVP's arithmetic is the
+ * same as pof_solve's (results are identical alone and in a batch). */
+
+/* Entries of batch workspace one IVP with n_times time points occupies (2 n_times). */
+int64_t pof_batch_entries_for(int64_t n_times);
+/* Device bytes of a batch with `entries` entries and `slots` IVP slots. */
+int64_t pof_batch_bytes(int D, int64_t entries, int32_t slots);
+/* Create a batch (allocates all of its device memory and its own stream). */
+int pof_batch_create(pof_ctx* ctx, int D, int d, int64_t entries, int32_t slots, pof_batch** out);
+void pof_batch_destroy(pof_batch* batch);
+/* Admit one IVP (solve options as in pof_solve; `out` must stay valid until
+ * the IVP finishes).  POF_E_FULL when no slot or entry range is free now. */
+int pof_batch_admit(pof_batch* batch, const pof_problem* prob, const pof_opts* opts, pof_out* out,
+                    int64_t* ticket);
+/* Enqueue one IEKS iteration for every admitted, unfinished IVP (asynchronous). */
+int pof_batch_launch(pof_batch* batch);
+/* Wait for the launched iteration, apply each IVP's convergence test
+ * (pof/solver.py:36-57) and retire finished IVPs: factor pass, output
+ * projection into their pof_out buffers, info fields.  Writes up to max_done
+ * tickets of finished IVPs to done; *n_done = number finished. */
+int pof_batch_finish(pof_batch* batch, int32_t max_done, int64_t* done, int32_t* n_done);
+/* Admitted, unfinished IVPs. */
+int pof_batch_active(const pof_batch* batch);
+/* Largest free entry range (an IVP with n_times points fits if 2 n_times <= this). */
+int64_t pof_batch_free_entries(const pof_batch* batch);
+/* Solve n independent IVPs (any mix of (D, d)) to completion: one batch per
+ * (D, d) group, groups iterated concurrently on their own streams. */
+int pof_solve_batch(pof_ctx* ctx, const pof_problem* probs, const pof_opts* opts, pof_out* outs, int32_t n);
 
 #ifdef __cplusplus
 }

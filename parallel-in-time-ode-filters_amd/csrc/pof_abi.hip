@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -332,18 +333,79 @@ static int initial_means(const pof_problem* pr, const Setup& s, std::vector<doub
 }  // namespace pof
 
 // ---------------------------------------------------------------- context
+struct pof_batch;
+
 struct pof_ctx {
   int device = 0;
   std::string err;
-  pof::Workspace ws;
-  size_t ws_cap[32] = {};  // capacities (bytes) of ws allocations, by slot
-  void* ws_ptr[32] = {};
   // per-kernel-class timing (pof_profile_*)
   bool profiling = false;
   pof::KernelTimer timer;
   double prof_ms[pof::kNumKernelClasses] = {};
   long long prof_launches[pof::kNumKernelClasses] = {};
   long long prof_work[pof::kNumKernelClasses] = {};
+  // batches kept for reuse by pof_solve / pof_ieks_step / pof_solve_batch, one per (D, d)
+  std::vector<pof_batch*> cache;
+};
+
+// A group of IVPs with one (D, d) sharing one device workspace.  IVP slots are
+// admitted and retired between iterations; each owns a range of
+// 2 (Ns + 1) entries of every array (first fit over the free ranges).
+struct pof_batch {
+  struct Slot {
+    bool used = false;
+    int k = 0;  // iterations done
+    double nll = 0.0, obj = 0.0, ssq = 0.0;
+    bool means_close = false, isnan_ = false;
+    int status = POF_STATUS_CONVERGED;
+    pof_opts opts{};
+    pof_out* out = nullptr;
+    long long ticket = 0;
+    long long start = 0, len = 0;
+    int last_wchol = 0;
+    bool raw_scale = false;  // output factors unscaled (linear filter/smoother API)
+  };
+
+  pof_ctx* ctx = nullptr;
+  int D = 0, d = 0, q = 0;
+  bool affine = false;
+  bool cached = false;
+  const pof::EngineFns* e = nullptr;
+  long long entries = 0;
+  int slots = 0;
+  hipStream_t st = nullptr;      // stream the batch works on
+  hipStream_t own_st = nullptr;  // stream created with the batch
+  hipEvent_t ev = nullptr;
+  std::vector<void*> allocs;
+  pof::GroupArgs g{};
+  // writable device tables (g holds the same pointers)
+  double *F = nullptr, *LQ = nullptr, *N1z = nullptr, *Pd = nullptr, *x0m = nullptr, *x0L = nullptr;
+  double *params = nullptr, *scale = nullptr, *scal_dev = nullptr, *scal_host = nullptr;
+  int *kind = nullptr, *Ns = nullptr, *cur = nullptr, *wchol = nullptr;
+  long long* base = nullptr;
+  double** outp = nullptr;  // [4, slots]: means, chols, full means, full chols
+  int* maps_dev = nullptr;
+  size_t maps_cap = 0;
+  // host mirrors
+  std::vector<double> hPd, hx0m, hx0L, hparams, hscale;
+  std::vector<int> hkind, hNs, hcur, hwchol;  // hcur: 0, uploaded at admission (the device flips it)
+  std::vector<long long> hbase;
+  std::vector<double*> hout;
+  std::vector<int> hmaps;
+  std::vector<Slot> sl;
+  std::map<long long, long long> free_;  // free entry ranges: start -> length
+  std::vector<int> dirty_slots;          // per-slot tables to upload before the next launch
+  std::vector<std::pair<long long, std::vector<double>>> init_host;  // initial means to upload
+  std::vector<std::pair<long long, const double*>> init_dev;         // ... or to copy on the device
+  std::vector<long long> init_zero;                                  // ... or to clear (entries, count D each)
+  std::vector<long long> init_zero_n;
+  std::vector<int> active;
+  pof::LaunchPlan plan;
+  bool plan_dirty = true;
+  bool launched = false;
+  bool need_sync = false;
+  bool unsynced = false;  // launches whose host-side copies may still be pending
+  long long next_ticket = 1;
 };
 
 namespace {
@@ -367,93 +429,25 @@ int set_err(pof_ctx* c, int code, const char* fmt, ...) {
       return set_err(ctx, POF_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));       \
   } while (0)
 
-// grow-only device buffer in slot `slot`
-hipError_t ensure(pof_ctx* ctx, int slot, size_t bytes, double** out) {
-  bytes = std::max<size_t>(bytes, 256);
-  if (ctx->ws_cap[slot] < bytes) {
-    if (ctx->ws_ptr[slot]) {
-      hipError_t e = hipFree(ctx->ws_ptr[slot]);
-      if (e != hipSuccess) return e;
-      ctx->ws_ptr[slot] = nullptr;
-      ctx->ws_cap[slot] = 0;
-    }
-    void* p = nullptr;
-    hipError_t e = hipMalloc(&p, bytes);
-    if (e != hipSuccess) return e;
-    ctx->ws_ptr[slot] = p;
-    ctx->ws_cap[slot] = bytes;
-  }
-  *out = static_cast<double*>(ctx->ws_ptr[slot]);
-  return hipSuccess;
-}
-
-int levels_of(size_t n0, size_t* n, size_t* off) {
+int n_levels_host(long long n0) {
   int L = 0;
-  size_t o = 0;
-  size_t k = n0;
-  while (k >= 1) {
-    n[L] = k;
-    off[L] = o;
-    o += k;
+  while (n0 >= 1) {
     ++L;
-    if (k == 1) break;
-    k /= 2;
+    if (n0 == 1) break;
+    n0 /= 2;
   }
   return L;
 }
 
-// Allocate / reuse the workspace for (D, d, Ns) and upload the model constants.
-int prepare_workspace(pof_ctx* ctx, int D, int d, int Ns, const std::vector<double>& F, const std::vector<double>& LQ,
-                      const std::vector<double>& N1z, const std::vector<double>& P, const std::vector<double>& x0m,
-                      const std::vector<double>& x0L, hipStream_t st) {
-  Workspace& ws = ctx->ws;
-  ws.D = D;
-  ws.d = d;
-  ws.Ns = Ns;
-  ws.f_levels = levels_of((size_t)Ns, ws.f_n, ws.f_off);
-  ws.s_levels = levels_of((size_t)Ns + 1, ws.s_n, ws.s_off);
-  const size_t fn = ws.f_off[ws.f_levels - 1] + ws.f_n[ws.f_levels - 1];
-  const size_t sn = ws.s_off[ws.s_levels - 1] + ws.s_n[ws.s_levels - 1];
-  const size_t DD = (size_t)D * D, B = sizeof(double);
-  int slot = 0;
-  HIP_TRY(ensure(ctx, slot++, fn * DD * B, &ws.fel.A));
-  HIP_TRY(ensure(ctx, slot++, fn * D * B, &ws.fel.b));
-  HIP_TRY(ensure(ctx, slot++, fn * DD * B, &ws.fel.U));
-  HIP_TRY(ensure(ctx, slot++, fn * D * B, &ws.fel.eta));
-  HIP_TRY(ensure(ctx, slot++, fn * DD * B, &ws.fel.Z));
-  HIP_TRY(ensure(ctx, slot++, fn * D * B, &ws.fpre.m));
-  HIP_TRY(ensure(ctx, slot++, fn * DD * B, &ws.fpre.L));
-  HIP_TRY(ensure(ctx, slot++, sn * D * B, &ws.sel.g));
-  HIP_TRY(ensure(ctx, slot++, sn * DD * B, &ws.sel.E));
-  HIP_TRY(ensure(ctx, slot++, sn * DD * B, &ws.sel.Dm));
-  HIP_TRY(ensure(ctx, slot++, sn * D * B, &ws.spre.m));
-  HIP_TRY(ensure(ctx, slot++, sn * DD * B, &ws.spre.L));
-  HIP_TRY(ensure(ctx, slot++, ((size_t)Ns + 1) * D * B, &ws.means[0]));
-  HIP_TRY(ensure(ctx, slot++, ((size_t)Ns + 1) * D * B, &ws.means[1]));
-  HIP_TRY(ensure(ctx, slot++, ((size_t)Ns + 1) * B, &ws.t_ssq));
-  HIP_TRY(ensure(ctx, slot++, ((size_t)Ns + 1) * B, &ws.t_nll));
-  HIP_TRY(ensure(ctx, slot++, ((size_t)Ns + 1) * B, &ws.t_obj));
-  HIP_TRY(ensure(ctx, slot++, ((size_t)Ns + 1) * B, &ws.t_viol));
-  HIP_TRY(ensure(ctx, slot++, 8 * B, &ws.scal));
-  HIP_TRY(ensure(ctx, slot++, 4 * 128 * B, &ws.part));
-  HIP_TRY(ensure(ctx, slot++, DD * B, &ws.F));
-  HIP_TRY(ensure(ctx, slot++, DD * B, &ws.LQ));
-  HIP_TRY(ensure(ctx, slot++, DD * B, &ws.N1z));
-  HIP_TRY(ensure(ctx, slot++, D * B, &ws.Pd));
-  HIP_TRY(ensure(ctx, slot++, D * B, &ws.x0m));
-  HIP_TRY(ensure(ctx, slot++, DD * B, &ws.x0L));
-  HIP_TRY(hipMemcpyAsync(ws.F, F.data(), DD * B, hipMemcpyHostToDevice, st));
-  HIP_TRY(hipMemcpyAsync(ws.LQ, LQ.data(), DD * B, hipMemcpyHostToDevice, st));
-  HIP_TRY(hipMemcpyAsync(ws.N1z, N1z.data(), DD * B, hipMemcpyHostToDevice, st));
-  HIP_TRY(hipMemcpyAsync(ws.Pd, P.data(), D * B, hipMemcpyHostToDevice, st));
-  HIP_TRY(hipMemcpyAsync(ws.x0m, x0m.data(), D * B, hipMemcpyHostToDevice, st));
-  HIP_TRY(hipMemcpyAsync(ws.x0L, x0L.data(), DD * B, hipMemcpyHostToDevice, st));
-  ws.Hx = nullptr;
-  ws.cx = nullptr;
-  return POF_OK;
+long long entries_for(long long n_times) { return 2 * n_times; }
+
+// After an iteration: every active IVP's result buffer becomes its next linearisation point.
+__global__ void k_flip_cur(int* cur, const int* ivp, int m) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a < m) cur[ivp[a]] ^= 1;
 }
 
-// after a timed iteration: accumulate event intervals per kernel class
+// after an instrumented run: accumulate event intervals per kernel class
 hipError_t collect_timing(pof_ctx* ctx, hipStream_t st) {
   if (!ctx->profiling) return hipSuccess;
   hipError_t e = hipStreamSynchronize(st);
@@ -471,17 +465,6 @@ hipError_t collect_timing(pof_ctx* ctx, hipStream_t st) {
   return hipSuccess;
 }
 
-IterOpts iter_opts(const pof_problem* pr, int cur) {
-  IterOpts o;
-  o.cur = cur;
-  o.affine = 0;
-  o.kind = pr ? pr->kind : 0;
-  o.q = pr ? pr->q : 0;
-  if (pr)
-    for (int i = 0; i < kMaxParams; ++i) o.params[i] = pr->params[i];
-  return o;
-}
-
 const EngineFns* engine_or_err(pof_ctx* ctx, int D, int d) {
   const EngineFns* e = find_engine(D, d);
   if (!e) set_err(ctx, POF_E_UNSUPPORTED, "no kernel instantiation for D=%d, d=%d", D, d);
@@ -492,6 +475,560 @@ bool isclose(double a, double b, double rtol, double atol) {
   if (std::isnan(a) || std::isnan(b)) return false;
   if (std::isinf(a) || std::isinf(b)) return a == b;
   return std::fabs(a - b) <= atol + rtol * std::fabs(b);
+}
+
+// ------------------------------------------------------------ batch memory
+void batch_free(pof_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->ctx->device);
+  if (b->st) (void)hipStreamSynchronize(b->st);
+  for (void* p : b->allocs) (void)hipFree(p);
+  if (b->scal_host) (void)hipHostFree(b->scal_host);
+  if (b->maps_dev) (void)hipFree(b->maps_dev);
+  if (b->ev) (void)hipEventDestroy(b->ev);
+  if (b->own_st) (void)hipStreamDestroy(b->own_st);
+  delete b;
+}
+
+template <class T>
+hipError_t dalloc(pof_batch* b, size_t n, T** out) {
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, std::max<size_t>(n * sizeof(T), 256));
+  if (e != hipSuccess) return e;
+  b->allocs.push_back(p);
+  *out = static_cast<T*>(p);
+  return hipSuccess;
+}
+
+// Device bytes of a batch of `entries` entries and `slots` IVPs with state dimension D.
+size_t batch_bytes(int D, long long entries, int slots) {
+  const size_t DD = (size_t)D * D;
+  const size_t per_entry = 8 * (7 * DD + 7 * (size_t)D + 4);
+  const size_t per_slot = 8 * (2 * D + DD + kMaxParams + 1 + 8 + 4) + 4 * 4 + 8;
+  return per_entry * (size_t)entries + per_slot * (size_t)slots + 8 * 4 * ((size_t)entries / kChunk + slots + 1);
+}
+
+int batch_new(pof_ctx* ctx, int D, int d, long long entries, int slots, bool affine, pof_batch** out) {
+  *out = nullptr;
+  const EngineFns* e = engine_or_err(ctx, D, d);
+  if (!e) return POF_E_UNSUPPORTED;
+  if (entries < 4 || slots < 1) return set_err(ctx, POF_E_ARG, "batch needs entries >= 4 and slots >= 1");
+  HIP_TRY(hipSetDevice(ctx->device));
+  size_t free_b = 0, total_b = 0;
+  HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+  const size_t need = batch_bytes(D, entries, slots);
+  if (need > free_b)
+    return set_err(ctx, POF_E_HIP, "batch needs %zu bytes of device memory, %zu free", need, free_b);
+  pof_batch* b = new pof_batch();
+  b->ctx = ctx;
+  b->D = D;
+  b->d = d;
+  b->q = D / d - 1;
+  b->affine = affine;
+  b->e = e;
+  b->entries = entries;
+  b->slots = slots;
+  const size_t DD = (size_t)D * D, n = (size_t)entries, S = (size_t)slots;
+  GroupArgs& g = b->g;
+  hipError_t he = hipSuccess;
+#define POF_A(ptr, count)                       \
+  if (he == hipSuccess) he = dalloc(b, (count), &(ptr));
+  double *fA, *fb, *fU, *feta, *fZ, *pm, *pL, *sg, *sE, *sD, *qm, *qL, *m0, *m1, *ts, *tn, *to, *tv, *part;
+  POF_A(fA, n * DD) POF_A(fb, n * D) POF_A(fU, n * DD) POF_A(feta, n * D) POF_A(fZ, n * DD)
+  POF_A(pm, n * D) POF_A(pL, n * DD)
+  POF_A(sg, n * D) POF_A(sE, n * DD) POF_A(sD, n * DD) POF_A(qm, n * D) POF_A(qL, n * DD)
+  POF_A(m0, n * D) POF_A(m1, n * D) POF_A(ts, n) POF_A(tn, n) POF_A(to, n) POF_A(tv, n)
+  POF_A(part, 4 * (n / kChunk + S + 1))
+  POF_A(b->F, DD) POF_A(b->LQ, DD) POF_A(b->N1z, DD) POF_A(b->Pd, S * D) POF_A(b->x0m, S * D)
+  POF_A(b->x0L, S * DD) POF_A(b->params, S * kMaxParams) POF_A(b->scale, S) POF_A(b->scal_dev, S * 8)
+  POF_A(b->kind, S) POF_A(b->Ns, S) POF_A(b->cur, S) POF_A(b->wchol, S) POF_A(b->base, S) POF_A(b->outp, 4 * S)
+#undef POF_A
+  if (he == hipSuccess) he = hipHostMalloc((void**)&b->scal_host, S * 8 * sizeof(double));
+  if (he == hipSuccess) he = hipStreamCreateWithFlags(&b->own_st, hipStreamNonBlocking);
+  if (he == hipSuccess) he = hipEventCreateWithFlags(&b->ev, hipEventDisableTiming);
+  if (he != hipSuccess) {
+    batch_free(b);
+    return set_err(ctx, POF_E_HIP, "batch allocation failed: %s", hipGetErrorString(he));
+  }
+  b->st = b->own_st;
+  g.F = b->F;
+  g.LQ = b->LQ;
+  g.N1z = b->N1z;
+  g.Pd = b->Pd;
+  g.x0m = b->x0m;
+  g.x0L = b->x0L;
+  g.params = b->params;
+  g.kind = b->kind;
+  g.Ns = b->Ns;
+  g.base = b->base;
+  g.cur = b->cur;
+  g.wchol = b->wchol;
+  g.means0 = m0;
+  g.means1 = m1;
+  g.fel = FiltArrays{fA, fb, fU, feta, fZ};
+  g.fpre = StateArrays{pm, pL};
+  g.sel = SmoArrays{sg, sE, sD};
+  g.spre = StateArrays{qm, qL};
+  g.t_ssq = ts;
+  g.t_nll = tn;
+  g.t_obj = to;
+  g.t_viol = tv;
+  g.part = part;
+  g.out_means = b->outp;
+  g.out_chols = b->outp + S;
+  g.out_full_means = b->outp + 2 * S;
+  g.out_full_chols = b->outp + 3 * S;
+  g.out_scale = b->scale;
+  g.Hx = nullptr;
+  g.cx = nullptr;
+  g.q = b->q;
+  g.affine = affine ? 1 : 0;
+  b->hPd.assign(S * D, 0.0);
+  b->hx0m.assign(S * D, 0.0);
+  b->hx0L.assign(S * DD, 0.0);
+  b->hparams.assign(S * kMaxParams, 0.0);
+  b->hscale.assign(S, 1.0);
+  b->hkind.assign(S, 0);
+  b->hNs.assign(S, 1);
+  b->hcur.assign(S, 0);
+  b->hwchol.assign(S, 0);
+  b->hbase.assign(S, 0);
+  b->hout.assign(4 * S, nullptr);
+  b->sl.assign(S, pof_batch::Slot{});
+  b->free_[0] = entries;
+  // the transition model of the group (pof/transitions.py:37-50) and N1 of a zero prior
+  if (!affine) {
+    std::vector<double> A1, L1;
+    iwp_1d(b->q, A1, L1);
+    const std::vector<double> Fm = kron_eye(d, A1, b->q + 1), LQm = kron_eye(d, L1, b->q + 1);
+    std::vector<double> M(DD * 2, 0.0), N1;
+    for (int r = 0; r < D; ++r)
+      for (int c = 0; c < D; ++c) M[(size_t)r * 2 * D + D + c] = LQm[(size_t)r * D + c];
+    tria_host(M, D, 2 * D, N1);
+    he = hipMemcpy(b->F, Fm.data(), DD * 8, hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy(b->LQ, LQm.data(), DD * 8, hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy(b->N1z, N1.data(), DD * 8, hipMemcpyHostToDevice);
+    if (he != hipSuccess) {
+      batch_free(b);
+      return set_err(ctx, POF_E_HIP, "batch set-up copy failed: %s", hipGetErrorString(he));
+    }
+  }
+  *out = b;
+  return POF_OK;
+}
+
+// first-fit range allocator over the batch's entries
+bool range_alloc(pof_batch* b, long long len, long long* start) {
+  for (auto it = b->free_.begin(); it != b->free_.end(); ++it) {
+    if (it->second >= len) {
+      *start = it->first;
+      const long long rest = it->second - len, s2 = it->first + len;
+      b->free_.erase(it);
+      if (rest > 0) b->free_[s2] = rest;
+      return true;
+    }
+  }
+  return false;
+}
+
+void range_free(pof_batch* b, long long start, long long len) {
+  auto it = b->free_.emplace(start, len).first;
+  auto nx = std::next(it);
+  if (nx != b->free_.end() && it->first + it->second == nx->first) {
+    it->second += nx->second;
+    b->free_.erase(nx);
+  }
+  if (it != b->free_.begin()) {
+    auto pv = std::prev(it);
+    if (pv->first + pv->second == it->first) {
+      pv->second += it->second;
+      b->free_.erase(it);
+    }
+  }
+}
+
+// ---------------------------------------------------------- launch plans
+// Append the launch tables of IVP slots `ids` to h; offsets in p are relative
+// to `at` (the tables' position in the device buffer is set after upload).
+int plan_tables(pof_batch* b, const std::vector<int>& ids, std::vector<int>& h, LaunchPlan& p) {
+  const int at = (int)h.size();
+  const int m = (int)ids.size();
+  h.insert(h.end(), ids.begin(), ids.end());
+  long long maxNs = 1;
+  for (int id : ids) maxNs = std::max<long long>(maxNs, b->hNs[id]);
+  p = LaunchPlan{};
+  p.m = m;
+  p.fl = n_levels_host(maxNs);
+  p.sl = n_levels_host(maxNs + 1);
+  if (p.sl > kMaxLevels) return POF_E_ARG;
+  bool overflow = false;
+  auto table = [&](auto count, int& off, int& total, int* nivps) {
+    off = (int)h.size() - at;
+    long long acc = 0;
+    int ni = 0;
+    h.push_back(0);
+    for (int id : ids) {
+      const long long c = count((long long)b->hNs[id]);
+      acc += c;
+      if (c > 0) ++ni;
+      h.push_back((int)acc);
+    }
+    if (acc > 0x7fffffffLL) overflow = true;
+    total = (int)acc;
+    if (nivps) *nivps = ni;
+  };
+  table([](long long N) { return N; }, p.leaf_off, p.leaf_total, nullptr);
+  table([](long long N) { return N + 1; }, p.pts_off, p.pts_total, nullptr);
+  table([](long long N) { return (N + 1 + kChunk - 1) / kChunk; }, p.chunk_off, p.chunk_total, nullptr);
+  for (int l = 0; l + 1 < p.fl; ++l)
+    table([l](long long N) { return N >> (l + 1); }, p.fup_off[l], p.fup_total[l], nullptr);
+  for (int l = 0; l < p.fl; ++l)
+    table([l](long long N) { return ((N >> l) + 1) / 2; }, p.fdn_off[l], p.fdn_total[l], &p.fdn_ivps[l]);
+  for (int l = 0; l + 1 < p.sl; ++l)
+    table([l](long long N) { return (N + 1) >> (l + 1); }, p.sup_off[l], p.sup_total[l], nullptr);
+  for (int l = 0; l < p.sl; ++l)
+    table([l](long long N) { return (((N + 1) >> l) + 1) / 2; }, p.sdn_off[l], p.sdn_total[l], &p.sdn_ivps[l]);
+  return overflow ? POF_E_ARG : POF_OK;
+}
+
+// Admit one IVP with its set-up done on the host.  `m_host` (or `m_dev`, or
+// neither: zeros) holds the initial means [Ns + 1, D].
+int settle(pof_batch* b) {
+  pof_ctx* ctx = b->ctx;
+  if (b->unsynced) HIP_TRY(hipStreamSynchronize(b->st));
+  b->unsynced = false;
+  return POF_OK;
+}
+
+int admit_raw(pof_batch* b, int Nsteps, const double* Pd, const double* x0m, const double* x0L, int kind,
+              const double* params, std::vector<double>* m_host, const double* m_dev, const pof_opts* opts,
+              pof_out* out, bool raw_scale, long long* ticket) {
+  pof_ctx* ctx = b->ctx;
+  if (b->launched) return set_err(ctx, POF_E_ARG, "admit between pof_batch_launch and pof_batch_finish");
+  // the host tables change below: no earlier copy from them may still be pending
+  if (settle(b) != POF_OK) return POF_E_HIP;
+  int j = -1;
+  for (int i = 0; i < b->slots; ++i)
+    if (!b->sl[i].used) {
+      j = i;
+      break;
+    }
+  const long long len = entries_for((long long)Nsteps + 1);
+  long long start = 0;
+  if (j < 0) return set_err(ctx, POF_E_FULL, "no free IVP slot");
+  if (!range_alloc(b, len, &start)) return set_err(ctx, POF_E_FULL, "no free range of %lld entries", len);
+  const int D = b->D;
+  const size_t DD = (size_t)D * D, S = (size_t)b->slots;
+  pof_batch::Slot& s = b->sl[j];
+  s = pof_batch::Slot{};
+  s.used = true;
+  s.opts = *opts;
+  s.out = out;
+  s.ticket = b->next_ticket++;
+  s.start = start;
+  s.len = len;
+  s.raw_scale = raw_scale;
+  std::copy(Pd, Pd + D, &b->hPd[(size_t)j * D]);
+  std::copy(x0m, x0m + D, &b->hx0m[(size_t)j * D]);
+  std::copy(x0L, x0L + DD, &b->hx0L[(size_t)j * DD]);
+  for (int i = 0; i < kMaxParams; ++i) b->hparams[(size_t)j * kMaxParams + i] = params ? params[i] : 0.0;
+  b->hkind[j] = kind;
+  b->hNs[j] = Nsteps;
+  b->hbase[j] = start;
+  b->hcur[j] = 0;
+  b->hwchol[j] = 0;
+  b->hout[0 * S + j] = out->means_dev;
+  b->hout[1 * S + j] = out->chols_dev;
+  b->hout[2 * S + j] = out->full_means_dev;
+  b->hout[3 * S + j] = out->full_chols_dev;
+  b->dirty_slots.push_back(j);
+  if (m_host) {
+    b->init_host.emplace_back(start, std::move(*m_host));
+  } else if (m_dev) {
+    b->init_dev.emplace_back(start, m_dev);
+  } else {
+    b->init_zero.push_back(start);
+    b->init_zero_n.push_back((long long)Nsteps + 1);
+  }
+  b->plan_dirty = true;
+  if (ticket) *ticket = s.ticket;
+  return POF_OK;
+}
+
+int batch_launch(pof_batch* b) {
+  pof_ctx* ctx = b->ctx;
+  if (b->launched) return set_err(ctx, POF_E_ARG, "pof_batch_launch called twice without pof_batch_finish");
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (b->plan_dirty) {
+    if (settle(b) != POF_OK) return POF_E_HIP;
+    b->active.clear();
+    for (int i = 0; i < b->slots; ++i)
+      if (b->sl[i].used) b->active.push_back(i);
+    if (b->active.empty()) {
+      b->plan_dirty = false;
+      b->plan.m = 0;
+      return POF_OK;
+    }
+    b->hmaps.clear();
+    int rc = plan_tables(b, b->active, b->hmaps, b->plan);
+    if (rc != POF_OK) return set_err(ctx, rc, "launch tables overflow");
+    if (b->hmaps.size() > b->maps_cap) {
+      HIP_TRY(hipStreamSynchronize(b->st));
+      if (b->maps_dev) HIP_TRY(hipFree(b->maps_dev));
+      b->maps_dev = nullptr;
+      b->maps_cap = 0;
+      const size_t cap = std::max<size_t>(b->hmaps.size() * 2, 4096);
+      HIP_TRY(hipMalloc((void**)&b->maps_dev, cap * sizeof(int)));
+      b->maps_cap = cap;
+    }
+    HIP_TRY(hipMemcpyAsync(b->maps_dev, b->hmaps.data(), b->hmaps.size() * sizeof(int), hipMemcpyHostToDevice,
+                           b->st));
+    b->plan.dev = b->maps_dev;
+    b->plan_dirty = false;
+  }
+  if (b->plan.m == 0) return POF_OK;
+  // newly admitted IVPs: per-slot tables and initial means
+  if (!b->dirty_slots.empty() || !b->init_host.empty() || !b->init_dev.empty() || !b->init_zero.empty()) {
+    const int D = b->D;
+    const size_t DD = (size_t)D * D, S = (size_t)b->slots;
+    for (int j : b->dirty_slots) {
+      HIP_TRY(hipMemcpyAsync(b->Pd + (size_t)j * D, &b->hPd[(size_t)j * D], D * 8, hipMemcpyHostToDevice, b->st));
+      HIP_TRY(hipMemcpyAsync(b->x0m + (size_t)j * D, &b->hx0m[(size_t)j * D], D * 8, hipMemcpyHostToDevice, b->st));
+      HIP_TRY(hipMemcpyAsync(b->x0L + (size_t)j * DD, &b->hx0L[(size_t)j * DD], DD * 8, hipMemcpyHostToDevice,
+                             b->st));
+      HIP_TRY(hipMemcpyAsync(b->params + (size_t)j * kMaxParams, &b->hparams[(size_t)j * kMaxParams],
+                             kMaxParams * 8, hipMemcpyHostToDevice, b->st));
+      for (int t = 0; t < 4; ++t)
+        HIP_TRY(hipMemcpyAsync(b->outp + t * S + j, &b->hout[t * S + j], sizeof(double*), hipMemcpyHostToDevice,
+                               b->st));
+      HIP_TRY(hipMemcpyAsync(b->cur + j, &b->hcur[j], sizeof(int), hipMemcpyHostToDevice, b->st));
+      HIP_TRY(hipMemcpyAsync(b->wchol + j, &b->hwchol[j], sizeof(int), hipMemcpyHostToDevice, b->st));
+    }
+    b->dirty_slots.clear();
+    HIP_TRY(hipMemcpyAsync(b->kind, b->hkind.data(), S * sizeof(int), hipMemcpyHostToDevice, b->st));
+    HIP_TRY(hipMemcpyAsync(b->Ns, b->hNs.data(), S * sizeof(int), hipMemcpyHostToDevice, b->st));
+    HIP_TRY(hipMemcpyAsync(b->base, b->hbase.data(), S * sizeof(long long), hipMemcpyHostToDevice, b->st));
+    for (auto& pr : b->init_host)
+      HIP_TRY(hipMemcpyAsync(b->g.means0 + (size_t)pr.first * D, pr.second.data(), pr.second.size() * 8,
+                             hipMemcpyHostToDevice, b->st));
+    for (auto& pr : b->init_dev) {
+      long long n = 0;
+      for (int j : b->active)
+        if (b->hbase[j] == pr.first) n = (long long)b->hNs[j] + 1;
+      HIP_TRY(hipMemcpyAsync(b->g.means0 + (size_t)pr.first * D, pr.second, (size_t)n * D * 8,
+                             hipMemcpyDeviceToDevice, b->st));
+    }
+    for (size_t i = 0; i < b->init_zero.size(); ++i)
+      HIP_TRY(hipMemsetAsync(b->g.means0 + (size_t)b->init_zero[i] * D, 0, (size_t)b->init_zero_n[i] * D * 8,
+                             b->st));
+    // host sources must outlive the copies: wait here (admission only)
+    HIP_TRY(hipStreamSynchronize(b->st));
+    b->init_host.clear();
+    b->init_dev.clear();
+    b->init_zero.clear();
+    b->init_zero_n.clear();
+  }
+  // factor flag of this iteration: the smoothed factors are needed only from
+  // an IVP's last iteration, known in advance only with fixed_iters
+  b->need_sync = false;
+  bool wchange = false;
+  for (int j : b->active) {
+    const pof_batch::Slot& s = b->sl[j];
+    const bool fixed = s.opts.fixed_iters > 0;
+    const int w = (fixed && s.k + 1 == s.opts.fixed_iters) ? 1 : 0;
+    wchange = wchange || (w != b->hwchol[j]);
+    if (!fixed || s.k + 1 >= s.opts.fixed_iters) b->need_sync = true;
+  }
+  if (wchange) {
+    if (settle(b) != POF_OK) return POF_E_HIP;
+    for (int j : b->active) {
+      const pof_batch::Slot& s = b->sl[j];
+      b->hwchol[j] = (s.opts.fixed_iters > 0 && s.k + 1 == s.opts.fixed_iters) ? 1 : 0;
+    }
+    HIP_TRY(hipMemcpyAsync(b->wchol, b->hwchol.data(), (size_t)b->slots * sizeof(int), hipMemcpyHostToDevice, b->st));
+  }
+  KernelTimer* tm = ctx->profiling ? &ctx->timer : nullptr;
+  HIP_TRY(b->e->iteration(b->g, b->plan, b->scal_dev, b->st, tm));
+  hipLaunchKernelGGL(k_flip_cur, dim3((b->plan.m + 255) / 256), dim3(256), 0, b->st, b->cur, b->plan.dev, b->plan.m);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(collect_timing(ctx, b->st));
+  if (b->need_sync) {
+    HIP_TRY(hipMemcpyAsync(b->scal_host, b->scal_dev, (size_t)b->plan.m * 8 * sizeof(double),
+                           hipMemcpyDeviceToHost, b->st));
+    HIP_TRY(hipEventRecord(b->ev, b->st));
+  }
+  b->launched = true;
+  b->unsynced = true;
+  return POF_OK;
+}
+
+// Wait for the launched iteration, apply the convergence test of
+// pof/solver.py:36-57 per IVP, and retire finished IVPs (factor pass, output
+// projection, info).  Writes up to max_done finished tickets; *n_done = count.
+int batch_finish(pof_batch* b, int max_done, long long* done, int* n_done) {
+  pof_ctx* ctx = b->ctx;
+  if (n_done) *n_done = 0;
+  if (!b->launched) return POF_OK;
+  b->launched = false;
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (!b->need_sync) {
+    for (int j : b->active) b->sl[j].k += 1;  // no IVP can finish: no host round trip
+    return POF_OK;
+  }
+  HIP_TRY(hipEventSynchronize(b->ev));
+  b->unsynced = false;
+  std::vector<int> fin;
+  for (size_t a = 0; a < b->active.size(); ++a) {
+    const int j = b->active[a];
+    pof_batch::Slot& s = b->sl[j];
+    const double* sc = b->scal_host + a * 8;
+    s.k += 1;
+    s.last_wchol = b->hwchol[j];
+    const double obj_old = s.obj;
+    s.nll = sc[0];
+    s.obj = sc[1];
+    s.ssq = sc[2];
+    s.means_close = sc[3] != 0.0;
+    s.isnan_ = std::isnan(s.obj) || std::isnan(s.nll);
+    bool finished = false;
+    if (s.opts.fixed_iters > 0) {
+      if (s.k >= s.opts.fixed_iters) {
+        s.status = POF_STATUS_FIXED;
+        finished = true;
+      }
+    } else {
+      const bool obj_conv = isclose(obj_old, s.obj, 1e-6, 1e-9);
+      const bool conv = s.isnan_ || obj_conv || s.means_close;
+      if (!(s.k <= s.opts.maxiters)) {
+        s.status = POF_STATUS_MAXITER;
+        finished = true;
+      } else if (conv) {
+        s.status = s.isnan_ ? POF_STATUS_NAN : POF_STATUS_CONVERGED;
+        finished = true;
+      }
+    }
+    if (finished) fin.push_back(j);
+  }
+  if (fin.empty()) return POF_OK;
+  // factor pass for the IVPs whose last iteration carried means only, then the output
+  std::vector<int> need;
+  for (int j : fin)
+    if (!b->sl[j].last_wchol) need.push_back(j);
+  const size_t S = (size_t)b->slots;
+  for (int j : fin) {
+    const pof_batch::Slot& s = b->sl[j];
+    b->hwchol[j] = 1;
+    // double calibration (step.py:42-44 inside the loop + solver.py:66-68)
+    b->hscale[j] = s.raw_scale ? 1.0 : (s.opts.calibrate ? s.ssq : std::sqrt(s.ssq));
+  }
+  b->hmaps.clear();
+  LaunchPlan pc, po;
+  int rc = POF_OK;
+  const int start_c = 0;
+  if (!need.empty()) rc = plan_tables(b, need, b->hmaps, pc);
+  const int start_o = (int)b->hmaps.size();
+  if (rc == POF_OK) rc = plan_tables(b, fin, b->hmaps, po);
+  if (rc != POF_OK) return set_err(ctx, rc, "launch tables overflow");
+  if (b->hmaps.size() > b->maps_cap) {
+    HIP_TRY(hipStreamSynchronize(b->st));
+    if (b->maps_dev) HIP_TRY(hipFree(b->maps_dev));
+    b->maps_dev = nullptr;
+    b->maps_cap = 0;
+    const size_t cap = std::max<size_t>(b->hmaps.size() * 2, 4096);
+    HIP_TRY(hipMalloc((void**)&b->maps_dev, cap * sizeof(int)));
+    b->maps_cap = cap;
+  }
+  HIP_TRY(hipMemcpyAsync(b->maps_dev, b->hmaps.data(), b->hmaps.size() * sizeof(int), hipMemcpyHostToDevice,
+                         b->st));
+  pc.dev = b->maps_dev + start_c;
+  po.dev = b->maps_dev + start_o;
+  HIP_TRY(hipMemcpyAsync(b->wchol, b->hwchol.data(), S * sizeof(int), hipMemcpyHostToDevice, b->st));
+  HIP_TRY(hipMemcpyAsync(b->scale, b->hscale.data(), S * sizeof(double), hipMemcpyHostToDevice, b->st));
+  KernelTimer* tm = ctx->profiling ? &ctx->timer : nullptr;
+  if (!need.empty()) {
+    HIP_TRY(b->e->smooth_chol(b->g, pc, b->st, tm));
+    HIP_TRY(collect_timing(ctx, b->st));
+  }
+  HIP_TRY(b->e->output(b->g, po, b->st));
+  HIP_TRY(hipStreamSynchronize(b->st));
+  int nd = 0;
+  for (int j : fin) {
+    pof_batch::Slot& s = b->sl[j];
+    if (s.out) {
+      s.out->iterations = (double)s.k;
+      s.out->nll = s.nll;
+      s.out->obj = s.obj;
+      s.out->sigma_squared = s.ssq;
+      s.out->calibrated = s.opts.calibrate ? 1 : 0;
+      s.out->status = s.status;
+    }
+    if (done && nd < max_done) done[nd] = s.ticket;
+    ++nd;
+    range_free(b, s.start, s.len);
+    s.used = false;
+    b->hwchol[j] = 0;
+  }
+  b->plan_dirty = true;
+  if (n_done) *n_done = nd;
+  return POF_OK;
+}
+
+// The context's reusable batch for (D, d) with room for `entries` entries and `slots` IVPs.
+int cached_batch(pof_ctx* ctx, int D, int d, long long entries, int slots, pof_batch** out) {
+  for (size_t i = 0; i < ctx->cache.size(); ++i) {
+    pof_batch* b = ctx->cache[i];
+    if (b->D != D || b->d != d) continue;
+    if (b->entries >= entries && b->slots >= slots) {
+      *out = b;
+      return POF_OK;
+    }
+    entries = std::max(entries, b->entries);
+    slots = std::max(slots, b->slots);
+    batch_free(b);
+    ctx->cache.erase(ctx->cache.begin() + i);
+    break;
+  }
+  int rc = batch_new(ctx, D, d, entries, slots, false, out);
+  if (rc != POF_OK) return rc;
+  (*out)->cached = true;
+  ctx->cache.push_back(*out);
+  return POF_OK;
+}
+
+// Run a batch until every admitted IVP has finished.
+int run_to_completion(pof_batch* b) {
+  for (;;) {
+    bool any = false;
+    for (int i = 0; i < b->slots; ++i) any = any || b->sl[i].used;
+    if (!any) return POF_OK;
+    int rc = batch_launch(b);
+    if (rc != POF_OK) return rc;
+    rc = batch_finish(b, 0, nullptr, nullptr);
+    if (rc != POF_OK) return rc;
+  }
+}
+
+// make `b` work on the caller's stream for one call (ordering with the caller's work)
+struct StreamScope {
+  pof_batch* b;
+  hipStream_t saved;
+  StreamScope(pof_batch* b_, void* stream) : b(b_), saved(b_->st) {
+    if (stream) b->st = (hipStream_t)stream;
+  }
+  ~StreamScope() { b->st = saved; }
+};
+
+int admit_problem(pof_batch* b, const pof_problem* prob, const pof_opts* opts, pof_out* out, long long* ticket) {
+  pof_ctx* ctx = b->ctx;
+  Setup s;
+  int rc = build_setup(prob, s, ctx->err);
+  if (rc != POF_OK) return rc;
+  if (s.D != b->D || s.d != b->d) return set_err(ctx, POF_E_ARG, "problem (D=%d, d=%d) does not match the batch", s.D, s.d);
+  std::vector<double> m0;
+  rc = initial_means(prob, s, m0, ctx->err);
+  if (rc != POF_OK) return rc;
+  return admit_raw(b, s.Ns, s.P.data(), s.x0m.data(), s.x0L.data(), prob->kind, prob->params, &m0, nullptr, opts, out,
+                   false, ticket);
 }
 
 }  // namespace
@@ -514,8 +1051,8 @@ int pof_ctx_create(int device, pof_ctx** out) {
 void pof_ctx_destroy(pof_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
-  for (int i = 0; i < 32; ++i)
-    if (ctx->ws_ptr[i]) (void)hipFree(ctx->ws_ptr[i]);
+  for (pof_batch* b : ctx->cache) batch_free(b);
+  ctx->cache.clear();
   if (ctx->timer.ready)
     for (int i = 0; i <= pof::kMaxTimed; ++i) (void)hipEventDestroy(ctx->timer.ev[i]);
   delete ctx;
@@ -566,14 +1103,12 @@ int pof_setup(const pof_problem* prob, double* F_host, double* QL_host, double* 
   std::string err;
   int rc = pof::build_setup(prob, s, err);
   if (rc != POF_OK) return rc;
-  const int D = s.D;
   if (F_host) std::copy(s.F.begin(), s.F.end(), F_host);
   if (QL_host) std::copy(s.LQ.begin(), s.LQ.end(), QL_host);
   if (P_host) std::copy(s.P.begin(), s.P.end(), P_host);
   if (PI_host) std::copy(s.PI.begin(), s.PI.end(), PI_host);
   if (x0_mean_host) std::copy(s.x0m.begin(), s.x0m.end(), x0_mean_host);
   if (x0_chol_host) std::copy(s.x0L.begin(), s.x0L.end(), x0_chol_host);
-  (void)D;
   return POF_OK;
 }
 
@@ -608,16 +1143,26 @@ int pof_linearize(pof_ctx* ctx, const pof_problem* prob, const double* m_dev, in
   pof::Setup s;
   int rc = pof::build_setup(prob, s, ctx->err);
   if (rc != POF_OK) return rc;
-  if (n < 0 || (n > 0 && (!m_dev || !H_dev || !c_dev))) return set_err(ctx, POF_E_ARG, "bad linearize args");
+  if (n < 0 || n > 0x3fffffff || (n > 0 && (!m_dev || !H_dev || !c_dev)))
+    return set_err(ctx, POF_E_ARG, "bad linearize args");
   const pof::EngineFns* e = engine_or_err(ctx, s.D, s.d);
   if (!e) return POF_E_UNSUPPORTED;
+  if (n == 0) return POF_OK;
   hipStream_t st = (hipStream_t)stream;
   HIP_TRY(hipSetDevice(ctx->device));
-  rc = prepare_workspace(ctx, s.D, s.d, std::max(s.Ns, 1), s.F, s.LQ, s.N1z, s.P, s.x0m, s.x0L, st);
-  if (rc != POF_OK) return rc;
-  if (n == 0) return POF_OK;
-  HIP_TRY(e->linearize(ctx->ws, iter_opts(prob, 0), m_dev, (int)n, H_dev, c_dev, st));
-  HIP_TRY(hipStreamSynchronize(st));
+  double* Pd = nullptr;
+  HIP_TRY(hipMalloc((void**)&Pd, s.D * sizeof(double)));
+  hipError_t he = hipMemcpyAsync(Pd, s.P.data(), s.D * sizeof(double), hipMemcpyHostToDevice, st);
+  pof::ModelArgs a{};
+  a.Pd = Pd;
+  a.kind = prob->kind;
+  a.q = s.q;
+  a.Ns = (int)n;
+  for (int i = 0; i < pof::kMaxParams; ++i) a.params[i] = prob->params[i];
+  if (he == hipSuccess) he = e->linearize(a, m_dev, (int)n, H_dev, c_dev, st);
+  if (he == hipSuccess) he = hipStreamSynchronize(st);
+  (void)hipFree(Pd);
+  if (he != hipSuccess) return set_err(ctx, POF_E_HIP, "linearize failed: %s", hipGetErrorString(he));
   return POF_OK;
 }
 
@@ -626,76 +1171,14 @@ int pof_solve(pof_ctx* ctx, const pof_problem* prob, const pof_opts* opts, pof_o
   pof::Setup s;
   int rc = pof::build_setup(prob, s, ctx->err);
   if (rc != POF_OK) return rc;
-  const pof::EngineFns* e = engine_or_err(ctx, s.D, s.d);
-  if (!e) return POF_E_UNSUPPORTED;
-  std::vector<double> m0;
-  rc = pof::initial_means(prob, s, m0, ctx->err);
+  if (!engine_or_err(ctx, s.D, s.d)) return POF_E_UNSUPPORTED;
+  pof_batch* b = nullptr;
+  rc = cached_batch(ctx, s.D, s.d, entries_for(prob->n_times), 1, &b);
   if (rc != POF_OK) return rc;
-  hipStream_t st = (hipStream_t)stream;
-  HIP_TRY(hipSetDevice(ctx->device));
-  rc = prepare_workspace(ctx, s.D, s.d, s.Ns, s.F, s.LQ, s.N1z, s.P, s.x0m, s.x0L, st);
+  StreamScope scope(b, stream);
+  rc = admit_problem(b, prob, opts, out, nullptr);
   if (rc != POF_OK) return rc;
-  pof::Workspace& ws = ctx->ws;
-  int cur = 0;
-  HIP_TRY(hipMemcpyAsync(ws.means[cur], m0.data(), m0.size() * sizeof(double), hipMemcpyHostToDevice, st));
-
-  // while_loop of pof/solver.py:36-57
-  double nll = 0.0, obj = 0.0, ssq = 0.0, nll_old = 0.0, obj_old = 0.0;
-  bool means_close = false, isnan_ = false;
-  int k = 0;
-  int status = POF_STATUS_CONVERGED;
-  int last_with_chol = 0;
-  double sc[8];
-  for (;;) {
-    if (opts->fixed_iters > 0) {
-      if (k >= opts->fixed_iters) { status = POF_STATUS_FIXED; break; }
-    } else if (k >= 1) {
-      const bool obj_conv = isclose(obj_old, obj, 1e-6, 1e-9);
-      const bool conv = isnan_ || obj_conv || means_close;
-      if (!(k <= opts->maxiters)) { status = POF_STATUS_MAXITER; break; }
-      if (conv) { status = isnan_ ? POF_STATUS_NAN : POF_STATUS_CONVERGED; break; }
-    }
-    IterOpts io = iter_opts(prob, cur);
-    io.timer = ctx->profiling ? &ctx->timer : nullptr;
-    // the smoothed factors are needed only from the last iteration: means-only
-    // smoother scans until then (known in advance only with fixed_iters)
-    io.with_chol = (opts->fixed_iters > 0 && k + 1 == opts->fixed_iters) ? 1 : 0;
-    last_with_chol = io.with_chol;
-    HIP_TRY(e->iteration(ws, io, st));
-    HIP_TRY(collect_timing(ctx, st));
-    cur = 1 - cur;
-    ++k;
-    if (opts->fixed_iters > 0 && k < opts->fixed_iters) continue;  // no host round trip
-    HIP_TRY(hipMemcpyAsync(sc, ws.scal, 8 * sizeof(double), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
-    nll_old = nll;
-    obj_old = obj;
-    nll = sc[0];
-    obj = sc[1];
-    ssq = sc[2];
-    means_close = sc[3] != 0.0;
-    isnan_ = std::isnan(obj) || std::isnan(nll);
-  }
-  (void)nll_old;
-  if (!last_with_chol) {
-    // smoothed factors of the last iteration's smoother elements
-    IterOpts io = iter_opts(prob, cur);
-    io.timer = ctx->profiling ? &ctx->timer : nullptr;
-    HIP_TRY(e->smooth_chol(ws, io, st));
-    HIP_TRY(collect_timing(ctx, st));
-  }
-  // double calibration (step.py:42-44 inside the loop + solver.py:66-68)
-  const double scale = opts->calibrate ? ssq : std::sqrt(ssq);
-  HIP_TRY(e->output(ws, cur, scale, out->means_dev, out->chols_dev, out->full_means_dev, out->full_chols_dev,
-                    nullptr, nullptr, st));
-  HIP_TRY(hipStreamSynchronize(st));
-  out->iterations = (double)k;
-  out->nll = nll;
-  out->obj = obj;
-  out->sigma_squared = ssq;
-  out->calibrated = opts->calibrate ? 1 : 0;
-  out->status = status;
-  return POF_OK;
+  return run_to_completion(b);
 }
 
 int pof_ieks_step(pof_ctx* ctx, const pof_problem* prob, const double* states_mean_dev, double* out_mean_dev,
@@ -704,24 +1187,26 @@ int pof_ieks_step(pof_ctx* ctx, const pof_problem* prob, const double* states_me
   pof::Setup s;
   int rc = pof::build_setup(prob, s, ctx->err);
   if (rc != POF_OK) return rc;
-  const pof::EngineFns* e = engine_or_err(ctx, s.D, s.d);
-  if (!e) return POF_E_UNSUPPORTED;
-  hipStream_t st = (hipStream_t)stream;
-  HIP_TRY(hipSetDevice(ctx->device));
-  rc = prepare_workspace(ctx, s.D, s.d, s.Ns, s.F, s.LQ, s.N1z, s.P, s.x0m, s.x0L, st);
+  if (!engine_or_err(ctx, s.D, s.d)) return POF_E_UNSUPPORTED;
+  pof_batch* b = nullptr;
+  rc = cached_batch(ctx, s.D, s.d, entries_for(prob->n_times), 1, &b);
   if (rc != POF_OK) return rc;
-  pof::Workspace& ws = ctx->ws;
-  const size_t bytes = ((size_t)s.Ns + 1) * s.D * sizeof(double);
-  HIP_TRY(hipMemcpyAsync(ws.means[0], states_mean_dev, bytes, hipMemcpyDeviceToDevice, st));
-  HIP_TRY(e->iteration(ws, iter_opts(prob, 0), st));
-  double sc[8];
-  HIP_TRY(hipMemcpyAsync(sc, ws.scal, 8 * sizeof(double), hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipStreamSynchronize(st));
-  HIP_TRY(e->output(ws, 1, std::sqrt(sc[2]), nullptr, nullptr, out_mean_dev, out_chol_dev, nullptr, nullptr, st));
-  HIP_TRY(hipStreamSynchronize(st));
-  if (nll) *nll = sc[0];
-  if (obj) *obj = sc[1];
-  if (ssq) *ssq = sc[2];
+  StreamScope scope(b, stream);
+  pof_opts o{};
+  o.maxiters = 1;
+  o.fixed_iters = 1;
+  o.calibrate = 0;  // factors scaled by sqrt(ssq) once (step.py:42-44)
+  pof_out po{};
+  po.full_means_dev = out_mean_dev;
+  po.full_chols_dev = out_chol_dev;
+  rc = admit_raw(b, s.Ns, s.P.data(), s.x0m.data(), s.x0L.data(), prob->kind, prob->params, nullptr,
+                 states_mean_dev, &o, &po, false, nullptr);
+  if (rc != POF_OK) return rc;
+  rc = run_to_completion(b);
+  if (rc != POF_OK) return rc;
+  if (nll) *nll = po.nll;
+  if (obj) *obj = po.obj;
+  if (ssq) *ssq = po.sigma_squared;
   return POF_OK;
 }
 
@@ -734,39 +1219,148 @@ int pof_linear_filtsmooth(pof_ctx* ctx, int D, int d, int64_t Ns, const double* 
   if (D < 1 || d < 1 || Ns < 1 || Ns > 0x3fffffff || !x0_mean_host || !x0_chol_host || !F_host || !QL_host ||
       !H_dev || !c_dev)
     return set_err(ctx, POF_E_ARG, "invalid linear_filtsmooth arguments");
-  const pof::EngineFns* e = engine_or_err(ctx, D, d);
-  if (!e) return POF_E_UNSUPPORTED;
-  std::vector<double> F(F_host, F_host + (size_t)D * D), LQ(QL_host, QL_host + (size_t)D * D);
-  std::vector<double> x0m(x0_mean_host, x0_mean_host + D), x0L(x0_chol_host, x0_chol_host + (size_t)D * D);
-  std::vector<double> M((size_t)D * 2 * D, 0.0), N1z;
-  for (int r = 0; r < D; ++r)
-    for (int c = 0; c < D; ++c) M[(size_t)r * 2 * D + D + c] = LQ[(size_t)r * D + c];
-  pof::tria_host(M, D, 2 * D, N1z);
-  std::vector<double> P(D, 1.0);
-  hipStream_t st = (hipStream_t)stream;
-  HIP_TRY(hipSetDevice(ctx->device));
-  int rc = prepare_workspace(ctx, D, d, (int)Ns, F, LQ, N1z, P, x0m, x0L, st);
+  if (!engine_or_err(ctx, D, d)) return POF_E_UNSUPPORTED;
+  pof_batch* b = nullptr;
+  int rc = batch_new(ctx, D, d, entries_for(Ns + 1), 1, true, &b);
   if (rc != POF_OK) return rc;
-  pof::Workspace& ws = ctx->ws;
-  ws.Hx = const_cast<double*>(H_dev);
-  ws.cx = const_cast<double*>(c_dev);
-  pof::IterOpts o;
-  o.cur = 0;
-  o.affine = 1;
-  // the means buffer only feeds the convergence partials here; keep it defined
-  HIP_TRY(hipMemsetAsync(ws.means[0], 0, ((size_t)Ns + 1) * D * sizeof(double), st));
-  HIP_TRY(e->iteration(ws, o, st));
-  double sc[8];
-  HIP_TRY(hipMemcpyAsync(sc, ws.scal, 8 * sizeof(double), hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipStreamSynchronize(st));
-  HIP_TRY(e->output(ws, 1, 1.0, nullptr, nullptr, smooth_mean_dev, smooth_chol_dev, filt_mean_dev, filt_chol_dev,
-                    st));
-  HIP_TRY(hipStreamSynchronize(st));
-  ws.Hx = nullptr;
-  ws.cx = nullptr;
-  if (nll) *nll = sc[0];
-  if (obj) *obj = sc[1];
-  if (ssq) *ssq = sc[2];
+  struct Guard {
+    pof_batch* b;
+    ~Guard() { batch_free(b); }
+  } guard{b};
+  if (stream) b->st = (hipStream_t)stream;
+  // the caller's model: F, QL, N1 = tria([0 | QL]), x0; no preconditioning (P = 1)
+  const size_t DD = (size_t)D * D;
+  std::vector<double> M(DD * 2, 0.0), N1z;
+  for (int r = 0; r < D; ++r)
+    for (int c = 0; c < D; ++c) M[(size_t)r * 2 * D + D + c] = QL_host[(size_t)r * D + c];
+  pof::tria_host(M, D, 2 * D, N1z);
+  HIP_TRY(hipMemcpy(b->F, F_host, DD * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(b->LQ, QL_host, DD * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(b->N1z, N1z.data(), DD * 8, hipMemcpyHostToDevice));
+  b->g.Hx = H_dev;
+  b->g.cx = c_dev;
+  std::vector<double> P(D, 1.0);
+  pof_opts o{};
+  o.maxiters = 1;
+  o.fixed_iters = 1;
+  o.calibrate = 0;
+  pof_out po{};
+  po.full_means_dev = smooth_mean_dev;
+  po.full_chols_dev = smooth_chol_dev;
+  // the means buffer only feeds the convergence partials here; it starts at zero
+  rc = admit_raw(b, (int)Ns, P.data(), x0_mean_host, x0_chol_host, 0, nullptr, nullptr, nullptr, &o, &po, true,
+                 nullptr);
+  if (rc != POF_OK) return rc;
+  rc = batch_launch(b);
+  if (rc != POF_OK) return rc;
+  // filtered states are read before the finish pass reuses nothing of the filter
+  HIP_TRY(hipEventSynchronize(b->ev));
+  if (filt_mean_dev || filt_chol_dev) HIP_TRY(b->e->filtered(b->g, b->plan, filt_mean_dev, filt_chol_dev, b->st));
+  rc = batch_finish(b, 0, nullptr, nullptr);
+  if (rc != POF_OK) return rc;
+  if (nll) *nll = po.nll;
+  if (obj) *obj = po.obj;
+  if (ssq) *ssq = po.sigma_squared;
+  return POF_OK;
+}
+
+// ------------------------------------------------------------------ batches
+int64_t pof_batch_entries_for(int64_t n_times) { return entries_for(n_times); }
+
+int64_t pof_batch_bytes(int D, int64_t entries, int32_t slots) { return (int64_t)batch_bytes(D, entries, slots); }
+
+int pof_batch_create(pof_ctx* ctx, int D, int d, int64_t entries, int32_t slots, pof_batch** out) {
+  if (!ctx || !out) return POF_E_ARG;
+  return batch_new(ctx, D, d, entries, slots, false, out);
+}
+
+void pof_batch_destroy(pof_batch* b) { batch_free(b); }
+
+int pof_batch_admit(pof_batch* b, const pof_problem* prob, const pof_opts* opts, pof_out* out, int64_t* ticket) {
+  if (!b || !prob || !opts || !out) return POF_E_ARG;
+  long long t = 0;
+  int rc = admit_problem(b, prob, opts, out, &t);
+  if (rc == POF_OK && ticket) *ticket = t;
+  return rc;
+}
+
+int pof_batch_launch(pof_batch* b) { return b ? batch_launch(b) : POF_E_ARG; }
+
+int pof_batch_finish(pof_batch* b, int32_t max_done, int64_t* done, int32_t* n_done) {
+  if (!b) return POF_E_ARG;
+  std::vector<long long> tmp(std::max(max_done, 0));
+  int nd = 0;
+  int rc = batch_finish(b, max_done, tmp.data(), &nd);
+  for (int i = 0; i < nd && i < max_done; ++i)
+    if (done) done[i] = tmp[i];
+  if (n_done) *n_done = nd;
+  return rc;
+}
+
+int pof_batch_active(const pof_batch* b) {
+  if (!b) return 0;
+  int n = 0;
+  for (const auto& s : b->sl) n += s.used ? 1 : 0;
+  return n;
+}
+
+int64_t pof_batch_free_entries(const pof_batch* b) {
+  if (!b) return 0;
+  long long best = 0;
+  for (const auto& kv : b->free_) best = std::max(best, kv.second);
+  return best;
+}
+
+int pof_solve_batch(pof_ctx* ctx, const pof_problem* probs, const pof_opts* opts, pof_out* outs, int32_t n) {
+  if (!ctx || (n > 0 && (!probs || !opts || !outs)) || n < 0) return set_err(ctx, POF_E_ARG, "null argument");
+  // group by (D, d): one batch (and stream) per group
+  struct Grp {
+    int D, d;
+    long long entries = 0;
+    int count = 0;
+    pof_batch* b = nullptr;
+  };
+  std::vector<Grp> groups;
+  std::vector<int> gid(n);
+  for (int i = 0; i < n; ++i) {
+    const pof_problem& p = probs[i];
+    if (p.d < 1 || p.q < 1 || p.n_times < 2) return set_err(ctx, POF_E_ARG, "problem %d: invalid sizes", i);
+    const int D = p.d * (p.q + 1);
+    if (!engine_or_err(ctx, D, p.d)) return POF_E_UNSUPPORTED;
+    int k = -1;
+    for (size_t g = 0; g < groups.size(); ++g)
+      if (groups[g].D == D && groups[g].d == p.d) k = (int)g;
+    if (k < 0) {
+      groups.push_back(Grp{D, p.d});
+      k = (int)groups.size() - 1;
+    }
+    groups[k].entries += entries_for(p.n_times);
+    groups[k].count += 1;
+    gid[i] = k;
+  }
+  for (auto& g : groups) {
+    int rc = cached_batch(ctx, g.D, g.d, g.entries, g.count, &g.b);
+    if (rc != POF_OK) return rc;
+  }
+  for (int i = 0; i < n; ++i) {
+    int rc = admit_problem(groups[gid[i]].b, &probs[i], &opts[i], &outs[i], nullptr);
+    if (rc != POF_OK) return rc;
+  }
+  // iterate all groups concurrently (one stream each) until every IVP has finished
+  for (;;) {
+    bool any = false;
+    for (auto& g : groups) {
+      if (pof_batch_active(g.b) == 0) continue;
+      any = true;
+      int rc = batch_launch(g.b);
+      if (rc != POF_OK) return rc;
+    }
+    if (!any) break;
+    for (auto& g : groups) {
+      int rc = batch_finish(g.b, 0, nullptr, nullptr);
+      if (rc != POF_OK) return rc;
+    }
+  }
   return POF_OK;
 }
 

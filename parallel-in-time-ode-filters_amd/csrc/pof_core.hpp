@@ -85,14 +85,14 @@ struct Ctx {
 // thread's worker slice.  Must be called by every thread of the block before
 // any early exit.
 template <int D, int K>
-__device__ __forceinline__ Ctx<D> make_ctx(const ModelArgs& a, double* smem) {
+__device__ __forceinline__ Ctx<D> make_ctx(const double* F, const double* LQ, const double* N1z, double* smem) {
   using G = Geo<D, K>;
   if constexpr (G::CONSTS > 0) {
     for (int t = threadIdx.x; t < D * D; t += G::BLOCK) {
       const int r = t / D, c = t % D;
-      smem[r * G::LD + c] = a.F[t];
-      smem[G::TILE + r * G::LD + c] = a.LQ[t];
-      smem[2 * G::TILE + r * G::LD + c] = a.N1z[t];
+      smem[r * G::LD + c] = F[t];
+      smem[G::TILE + r * G::LD + c] = LQ[t];
+      smem[2 * G::TILE + r * G::LD + c] = N1z[t];
     }
     __syncthreads();
   }

@@ -11,44 +11,7 @@
 namespace pof {
 
 constexpr int kMaxLevels = 40;
-
-// Device workspace of one IEKS solve (one IVP) with state dimension D and
-// Ns steps.  Scan levels are stored back to back: level l holds
-// n_l = floor(n_0 / 2^l) entries starting at entry offset off[l].
-struct Workspace {
-  int D = 0, d = 0, Ns = 0;
-  // filter scan (n_0 = Ns)
-  int f_levels = 0;
-  size_t f_n[kMaxLevels] = {};
-  size_t f_off[kMaxLevels] = {};
-  FiltArrays fel{};      // elements and up-sweep aggregates, all levels
-  StateArrays fpre{};    // prefix states, all levels
-  // smoother scan (n_0 = Ns + 1, stored in reversed time order)
-  int s_levels = 0;
-  size_t s_n[kMaxLevels] = {};
-  size_t s_off[kMaxLevels] = {};
-  SmoArrays sel{};
-  StateArrays spre{};
-  // trajectories
-  double* means[2] = {nullptr, nullptr};  // [Ns+1, D] linearisation points / smoothed means
-  // per-index partial terms, reduced in a fixed order
-  double* t_ssq = nullptr;   // [Ns]
-  double* t_nll = nullptr;   // [Ns]
-  double* t_obj = nullptr;   // [Ns]
-  double* t_viol = nullptr;  // [Ns+1]
-  double* scal = nullptr;    // [8] device scalars: nll, obj, ssq, means_converged
-  double* part = nullptr;    // [4 * 128] partial sums of the reduction
-  // model constants
-  double* F = nullptr;
-  double* LQ = nullptr;
-  double* N1z = nullptr;
-  double* Pd = nullptr;
-  double* x0m = nullptr;
-  double* x0L = nullptr;
-  double* Hx = nullptr;      // affine mode only
-  double* cx = nullptr;
-  std::vector<void*> allocs;
-};
+constexpr int kChunk = 2048;  // time points per block of the stage-1 reduction
 
 // Optional per-kernel timing with HIP events (instrumented runs only).
 constexpr int kMaxTimed = 256;
@@ -72,33 +35,38 @@ struct KernelTimer {
   bool ready = false;
 };
 
-struct IterOpts {
-  int cur = 0;           // means[cur] are the linearisation points; means[1-cur] receive the result
-  int affine = 0;        // 1: observations from Hx/cx instead of EK1
-  int kind = 0;
-  int q = 0;
-  double params[kMaxParams] = {};
-  KernelTimer* timer = nullptr;  // non-null: record an event after every launch
-  int with_chol = 1;             // smoother scan also propagates the factors D / L
+// Launch maps of one active set of a group (host view of a device int buffer):
+// the active IVP ids, then one prefix table per launch kind and level.
+struct LaunchPlan {
+  const int* dev = nullptr;  // device buffer
+  int m = 0;                 // active IVPs
+  int fl = 0, sl = 0;        // filter / smoother levels (maximum over the active IVPs)
+  int leaf_off = 0, leaf_total = 0;    // per IVP: Ns work items
+  int pts_off = 0, pts_total = 0;      // Ns + 1
+  int chunk_off = 0, chunk_total = 0;  // ceil((Ns + 1) / kChunk)
+  int fup_off[kMaxLevels] = {}, fup_total[kMaxLevels] = {};  // Ns >> (l + 1)
+  int fdn_off[kMaxLevels] = {}, fdn_total[kMaxLevels] = {};  // ((Ns >> l) + 1) / 2
+  int sup_off[kMaxLevels] = {}, sup_total[kMaxLevels] = {};  // (Ns + 1) >> (l + 1)
+  int sdn_off[kMaxLevels] = {}, sdn_total[kMaxLevels] = {};  // (((Ns + 1) >> l) + 1) / 2
+  int fdn_ivps[kMaxLevels] = {}, sdn_ivps[kMaxLevels] = {};  // active IVPs that have level l
+  LaunchMap map(int off, int total) const { return LaunchMap{dev, dev + off, m, total}; }
 };
 
-// Kernel entry points per (D, d) instantiation.
+// Kernel entry points per (D, d) instantiation, all over one group's active set.
 struct EngineFns {
   int D, d;
-  // one IEKS iteration: filter scan, smoother elements + S statistics, smoother scan,
-  // smoothed means, objective, convergence partials and the reduction into ws.scal
-  hipError_t (*iteration)(const Workspace& ws, const IterOpts& o, hipStream_t s);
+  // one IEKS iteration for every active IVP: filter scan, smoother elements and
+  // S statistics, smoother scan, smoothed means, objective, convergence
+  // partials, and the per-IVP reduction into scal[a * 8 + 0..4] (a = active index)
+  hipError_t (*iteration)(const GroupArgs& g, const LaunchPlan& p, double* scal, hipStream_t s, KernelTimer* t);
   // smoother scan with factors over the smoother elements of the last iteration
-  // (after means-only iterations): fills the smoothed factors read by output()
-  hipError_t (*smooth_chol)(const Workspace& ws, const IterOpts& o, hipStream_t s);
-  // project smoothed states: means_out[t] = E0 m_t, chol_out[t] = scale * E0 L_t;
-  // full_means / full_chols (optional) receive the preconditioned states.
-  hipError_t (*output)(const Workspace& ws, int cur, double scale, double* means_out, double* chol_out,
-                       double* full_means, double* full_chols, double* filt_means, double* filt_chols,
-                       hipStream_t s);
+  hipError_t (*smooth_chol)(const GroupArgs& g, const LaunchPlan& p, hipStream_t s, KernelTimer* t);
+  // project smoothed states into the caller's outputs (g.out_* tables, g.out_scale)
+  hipError_t (*output)(const GroupArgs& g, const LaunchPlan& p, hipStream_t s);
+  // filtered states of IVP ivp[0] (x0, then the level-0 filter prefixes)
+  hipError_t (*filtered)(const GroupArgs& g, const LaunchPlan& p, double* fm, double* fL, hipStream_t s);
   // EK1 linearisation at n points: H [n, d, D], c [n, d]
-  hipError_t (*linearize)(const Workspace& ws, const IterOpts& o, const double* m, int n, double* H, double* c,
-                          hipStream_t s);
+  hipError_t (*linearize)(const ModelArgs& a, const double* m, int n, double* H, double* c, hipStream_t s);
 };
 
 const EngineFns* find_engine(int D, int d);

@@ -1,6 +1,11 @@
 // HIP kernels of one parallel IEKS iteration for gfx950 (templates; the
 // instantiations per ODE dimension live in pof_inst_d*.hip).
 //
+// Every launch serves all active IVPs of one group (same D, d, q): work item w
+// of a launch belongs to the active IVP a with pre[a] <= w < pre[a + 1]
+// (LaunchMap), and every IVP reads and writes only its own entry range, so a
+// result does not depend on which other IVPs share the launch.
+//
 // Filter: JAX's odd/even associative scan (pof/parallel_filtsmooth/filter.py:35)
 // reproduced level by level -- up-sweep level l pairs (2i, 2i+1), down-sweep
 // fills the even positions -- so every factor is built by the same sequence
@@ -20,46 +25,83 @@ namespace pof {
 
 namespace {
 
-inline ModelArgs model_args(const Workspace& ws, const IterOpts& o) {
-  ModelArgs a{};
-  a.F = ws.F;
-  a.LQ = ws.LQ;
-  a.N1z = ws.N1z;
-  a.Pd = ws.Pd;
-  a.x0m = ws.x0m;
-  a.x0L = ws.x0L;
-  a.lin = o.affine ? nullptr : ws.means[o.cur];
-  a.Hx = ws.Hx;
-  a.cx = ws.cx;
-  a.kind = o.kind;
-  a.q = o.q;
-  a.Ns = ws.Ns;
-  for (int i = 0; i < kMaxParams; ++i) a.params[i] = o.params[i];
-  return a;
-}
-
-inline FiltArrays at(const FiltArrays& x, size_t off, int D) {
+__host__ __device__ inline FiltArrays at(const FiltArrays& x, size_t off, int D) {
   const size_t DD = (size_t)D * D;
   return FiltArrays{x.A + off * DD, x.b + off * D, x.U + off * DD, x.eta + off * D, x.Z + off * DD};
 }
-inline SmoArrays at(const SmoArrays& x, size_t off, int D) {
+__host__ __device__ inline SmoArrays at(const SmoArrays& x, size_t off, int D) {
   const size_t DD = (size_t)D * D;
   return SmoArrays{x.g + off * D, x.E + off * DD, x.Dm + off * DD};
 }
-inline StateArrays at(const StateArrays& x, size_t off, int D) {
+__host__ __device__ inline StateArrays at(const StateArrays& x, size_t off, int D) {
   return StateArrays{x.m + off * D, x.L + off * (size_t)D * D};
 }
 
 }  // namespace
 
+// --------------------------------------------------------------- work lists
+// Active IVP index of work item w: the largest a with pre[a] <= w.
+__device__ __forceinline__ int find_seg(const int* pre, int m, int w) {
+  int lo = 0, hi = m - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (pre[mid] <= w) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// IVP id j and local index i of work item w (false past the end).
+__device__ __forceinline__ bool locate(const LaunchMap& lm, int w, int& j, int& i) {
+  if (w >= lm.total) return false;
+  const int a = find_seg(lm.pre, lm.m, w);
+  j = lm.ivp[a];
+  i = w - lm.pre[a];
+  return true;
+}
+
+// Entry offset of scan level l inside an IVP's range (level t holds n0 >> t entries).
+__device__ __forceinline__ long long lev_off(long long n0, int l) {
+  long long o = 0;
+  for (int t = 0; t < l; ++t) o += n0 >> t;
+  return o;
+}
+// Number of scan levels of n0 >= 1 elements (n0, n0 / 2, ..., 1).
+__device__ __forceinline__ int n_levels(long long n0) { return 64 - __clzll(n0); }
+
+// Model view of IVP j of the group.
+template <int D>
+__device__ __forceinline__ ModelArgs view(const GroupArgs& g, int j) {
+  ModelArgs a;
+  const long long base = g.base[j];
+  a.F = g.F;
+  a.LQ = g.LQ;
+  a.N1z = g.N1z;
+  a.Pd = g.Pd + (size_t)j * D;
+  a.x0m = g.x0m + (size_t)j * D;
+  a.x0L = g.x0L + (size_t)j * D * D;
+  a.lin = g.affine ? nullptr : ((g.cur[j] ? g.means1 : g.means0) + (size_t)base * D);
+  a.Hx = g.Hx;
+  a.cx = g.cx;
+  a.kind = g.kind[j];
+  a.q = g.q;
+  a.Ns = g.Ns[j];
+  a.pad = 0;
+#pragma unroll
+  for (int i = 0; i < kMaxParams; ++i) a.params[i] = g.params[(size_t)j * kMaxParams + i];
+  return a;
+}
+
 // ------------------------------------------------------------------ kernels
 // Leaf elements: worker k builds filter element k (k = 0 carries x0).
 template <int D, int d>
-__global__ __launch_bounds__((Geo<D, kLdsLeaf>::BLOCK)) void k_filter_leaf(ModelArgs a, FiltArrays out, int n) {
+__global__ __launch_bounds__((Geo<D, kLdsLeaf>::BLOCK)) void k_filter_leaf(GroupArgs g, LaunchMap lm) {
   extern __shared__ double smem[];
-  Ctx<D> cx = make_ctx<D, kLdsLeaf>(a, smem);
-  const int k = cx.wid;
-  if (k >= n) return;
+  Ctx<D> cx = make_ctx<D, kLdsLeaf>(g.F, g.LQ, g.N1z, smem);
+  int j, k;
+  if (!locate(lm, cx.wid, j, k)) return;
+  const ModelArgs a = view<D>(g, j);
+  const FiltArrays out = at(g.fel, g.base[j], D);
   double A[D], U[D], Z[D];
   double b, eta;
   filter_element<D, d>(cx, a, k, k == 0, A, b, U, eta, Z);
@@ -76,11 +118,14 @@ __global__ __launch_bounds__((Geo<D, kLdsLeaf>::BLOCK)) void k_filter_leaf(Model
 
 // Up-sweep: out[i] = in[2i] o in[2i+1]
 template <int D>
-__global__ __launch_bounds__((Geo<D, kLdsFull>::BLOCK)) void k_filter_up(ModelArgs a, FiltArrays in, FiltArrays out, int n_out) {
+__global__ __launch_bounds__((Geo<D, kLdsFull>::BLOCK)) void k_filter_up(GroupArgs g, LaunchMap lm, int l) {
   extern __shared__ double smem[];
-  Ctx<D> cx = make_ctx<D, kLdsFull>(a, smem);
-  const int i = cx.wid;
-  if (i >= n_out) return;
+  Ctx<D> cx = make_ctx<D, kLdsFull>(g.F, g.LQ, g.N1z, smem);
+  int j, i;
+  if (!locate(lm, cx.wid, j, i)) return;
+  const long long N = g.Ns[j], base = g.base[j];
+  const FiltArrays in = at(g.fel, base + lev_off(N, l), D);
+  const FiltArrays out = at(g.fel, base + lev_off(N, l + 1), D);
   StateArrays none{};
   filter_combine<D, false>(cx, in, none, 2 * (size_t)i, in, 2 * (size_t)i + 1, out, none, (size_t)i);
 }
@@ -89,13 +134,17 @@ __global__ __launch_bounds__((Geo<D, kLdsFull>::BLOCK)) void k_filter_up(ModelAr
 //   p[0] = state(e[0]); p[2i+1] = up[i]; p[2i] = up[i-1] o e[2i]  (i >= 1)
 // With top != 0 (a level of one element) only p[0] = state(e[0]) is written.
 template <int D>
-__global__ __launch_bounds__((Geo<D, kLdsState>::BLOCK)) void k_filter_down(ModelArgs a, FiltArrays el, StateArrays up,
-                                                               StateArrays p, int n) {
+__global__ __launch_bounds__((Geo<D, kLdsState>::BLOCK)) void k_filter_down(GroupArgs g, LaunchMap lm, int l) {
   extern __shared__ double smem[];
-  Ctx<D> cx = make_ctx<D, kLdsState>(a, smem);
-  const int i = cx.wid;
+  Ctx<D> cx = make_ctx<D, kLdsState>(g.F, g.LQ, g.N1z, smem);
+  int j, i;
+  if (!locate(lm, cx.wid, j, i)) return;
+  const long long N = g.Ns[j], base = g.base[j];
+  const int n = (int)(N >> l);
+  const FiltArrays el = at(g.fel, base + lev_off(N, l), D);
+  const StateArrays p = at(g.fpre, base + lev_off(N, l), D);
+  const StateArrays up = (l + 1 < n_levels(N)) ? at(g.fpre, base + lev_off(N, l + 1), D) : StateArrays{};
   const int j0 = 2 * i;
-  if (j0 >= n) return;
   const int r = cx.lane;
   if (j0 == 0) {
     if (r < D) {
@@ -121,14 +170,19 @@ __global__ __launch_bounds__((Geo<D, kLdsState>::BLOCK)) void k_filter_down(Mode
 //   S = tria([H_s tria([F L_s | LQ]) | 0]), yhat = H_s F m_s + c_s,
 //   ssq_s = |S^-T yhat|^2 (utils.py:110-112), nll_s = -loglik(yhat, S) (utils.py:22-30).
 template <int D, int d>
-__global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK)) void k_filter_post(ModelArgs a, StateArrays pre0, SmoArrays sel,
-                                                               double* t_ssq, double* t_nll) {
+__global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK)) void k_filter_post(GroupArgs g, LaunchMap lm) {
   extern __shared__ double smem[];
-  Ctx<D> cx = make_ctx<D, kLdsPost>(a, smem);
+  Ctx<D> cx = make_ctx<D, kLdsPost>(g.F, g.LQ, g.N1z, smem);
   constexpr int LD = D + 1;
-  const int s = cx.wid;
+  int j, s;
+  if (!locate(lm, cx.wid, j, s)) return;
+  const ModelArgs a = view<D>(g, j);
+  const long long base = g.base[j];
+  const StateArrays pre0 = at(g.fpre, base, D);
+  const SmoArrays sel = at(g.sel, base, D);
+  double* t_ssq = g.t_ssq + base;
+  double* t_nll = g.t_nll + base;
   const int Ns = a.Ns;
-  if (s > Ns) return;
   const int r = cx.lane;
   const bool act = r < D;
   const double* mp = (s == 0) ? a.x0m : pre0.m + (size_t)(s - 1) * D;
@@ -275,25 +329,33 @@ __global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK)) void k_filter_post(Model
 
 // Smoother up-sweep
 template <int D>
-__global__ __launch_bounds__((Geo<D, kLdsSmooth>::BLOCK)) void k_smooth_up(ModelArgs a, SmoArrays in, SmoArrays out, int n_out,
-                                                             int with_chol) {
+__global__ __launch_bounds__((Geo<D, kLdsSmooth>::BLOCK)) void k_smooth_up(GroupArgs g, LaunchMap lm, int l) {
   extern __shared__ double smem[];
-  Ctx<D> cx = make_ctx<D, kLdsSmooth>(a, smem);
-  const int i = cx.wid;
-  if (i >= n_out) return;
+  Ctx<D> cx = make_ctx<D, kLdsSmooth>(g.F, g.LQ, g.N1z, smem);
+  int j, i;
+  if (!locate(lm, cx.wid, j, i)) return;
+  const long long n0 = (long long)g.Ns[j] + 1, base = g.base[j];
+  const SmoArrays in = at(g.sel, base + lev_off(n0, l), D);
+  const SmoArrays out = at(g.sel, base + lev_off(n0, l + 1), D);
+  const int with_chol = g.wchol[j];
   StateArrays none{};
   smoother_combine<D, false>(cx, in, none, 2 * (size_t)i, in, 2 * (size_t)i + 1, out, none, (size_t)i,
                              with_chol != 0);
 }
 
 template <int D>
-__global__ __launch_bounds__((Geo<D, kLdsSmooth>::BLOCK)) void k_smooth_down(ModelArgs a, SmoArrays el, StateArrays up,
-                                                               StateArrays p, int n, int with_chol) {
+__global__ __launch_bounds__((Geo<D, kLdsSmooth>::BLOCK)) void k_smooth_down(GroupArgs g, LaunchMap lm, int l) {
   extern __shared__ double smem[];
-  Ctx<D> cx = make_ctx<D, kLdsSmooth>(a, smem);
-  const int i = cx.wid;
+  Ctx<D> cx = make_ctx<D, kLdsSmooth>(g.F, g.LQ, g.N1z, smem);
+  int j, i;
+  if (!locate(lm, cx.wid, j, i)) return;
+  const long long n0 = (long long)g.Ns[j] + 1, base = g.base[j];
+  const int n = (int)(n0 >> l);
+  const SmoArrays el = at(g.sel, base + lev_off(n0, l), D);
+  const StateArrays p = at(g.spre, base + lev_off(n0, l), D);
+  const StateArrays up = (l + 1 < n_levels(n0)) ? at(g.spre, base + lev_off(n0, l + 1), D) : StateArrays{};
+  const int with_chol = g.wchol[j];
   const int j0 = 2 * i;
-  if (j0 >= n) return;
   const int r = cx.lane;
   if (j0 == 0) {
     if (r < D) {
@@ -322,10 +384,18 @@ __global__ __launch_bounds__((Geo<D, kLdsSmooth>::BLOCK)) void k_smooth_down(Mod
 // of smoother.py:20); viol_t = 1 if any |m_old - m_new| > 1e-8 + 1e-13 |m_new|
 // (convergence_criteria.py:10, jnp.isclose).
 template <int D>
-__global__ void k_smooth_post(const double* F, const double* LQ, StateArrays spre0, const double* mold, double* mnew,
-                              double* t_obj, double* t_viol, int Ns) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t > Ns) return;
+__global__ void k_smooth_post(GroupArgs g, LaunchMap lm) {
+  int j, t;
+  if (!locate(lm, blockIdx.x * blockDim.x + threadIdx.x, j, t)) return;
+  const int Ns = g.Ns[j];
+  const long long base = g.base[j];
+  const StateArrays spre0 = at(g.spre, base, D);
+  const double* mold = (g.cur[j] ? g.means1 : g.means0) + (size_t)base * D;
+  double* mnew = (g.cur[j] ? g.means0 : g.means1) + (size_t)base * D;
+  double* t_obj = g.t_obj + base;
+  double* t_viol = g.t_viol + base;
+  const double* F = g.F;
+  const double* LQ = g.LQ;
   const double* m = spre0.m + (size_t)(Ns - t) * D;
   double mt[D];
   load_row(mt, m);
@@ -362,25 +432,26 @@ __global__ void k_smooth_post(const double* F, const double* LQ, StateArrays spr
   }
 }
 
-// Fixed-order reduction of the per-index terms, stage 1: block b sums the
-// contiguous segment [b * seg, (b + 1) * seg) into part[b * 4 + 0..3]
-// (ssq, nll, obj: sums; violations: max).
-constexpr int kRedBlocks = 128;
-static __global__ void k_reduce_part(const double* t_ssq, const double* t_nll, const double* t_obj,
-                                     const double* t_viol, int Ns, double* part) {
+// Fixed-order reduction of the per-step terms of each IVP, stage 1: block b
+// (chunk c of IVP j) sums the time points [c kChunk, (c + 1) kChunk) into
+// part[b * 4 + 0..3] (ssq, nll, obj: sums over t < Ns; violations: max over t <= Ns).
+static __global__ void k_reduce_part(GroupArgs g, LaunchMap lm) {
   __shared__ double sh[4][256];
   const int tid = threadIdx.x;
-  const int seg = (Ns + 1 + kRedBlocks - 1) / kRedBlocks;
-  const int lo = blockIdx.x * seg;
-  const int hi = min(lo + seg, Ns + 1);
+  int j, c;
+  if (!locate(lm, blockIdx.x, j, c)) return;
+  const int Ns = g.Ns[j];
+  const long long base = g.base[j];
+  const int lo = c * kChunk;
+  const int hi = min(lo + kChunk, Ns + 1);
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
   for (int i = lo + tid; i < hi; i += 256) {
     if (i < Ns) {
-      s0 += t_ssq[i];
-      s1 += t_nll[i];
-      s2 += t_obj[i];
+      s0 += g.t_ssq[base + i];
+      s1 += g.t_nll[base + i];
+      s2 += g.t_obj[base + i];
     }
-    s3 = fmax(s3, t_viol[i]);
+    s3 = fmax(s3, g.t_viol[base + i]);
   }
   sh[0][tid] = s0;
   sh[1][tid] = s1;
@@ -396,43 +467,45 @@ static __global__ void k_reduce_part(const double* t_ssq, const double* t_nll, c
     }
     __syncthreads();
   }
-  if (tid < 4) part[blockIdx.x * 4 + tid] = sh[tid][0];
+  if (tid < 4) g.part[(size_t)blockIdx.x * 4 + tid] = sh[tid][0];
 }
 
-// Stage 2 (one block of kRedBlocks threads): scal[0] = nll, scal[1] = obj,
-// scal[2] = ssq, scal[3] = 1 if all means are close, scal[4] = NaN flag.
-static __global__ void k_reduce_final(const double* part, int Ns, int d, double* scal) {
-  __shared__ double sh[4][kRedBlocks];
-  const int tid = threadIdx.x;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) sh[k][tid] = part[tid * 4 + k];
-  __syncthreads();
-  for (int w = kRedBlocks / 2; w > 0; w >>= 1) {
-    if (tid < w) {
-      sh[0][tid] += sh[0][tid + w];
-      sh[1][tid] += sh[1][tid + w];
-      sh[2][tid] += sh[2][tid + w];
-      sh[3][tid] = fmax(sh[3][tid], sh[3][tid + w]);
-    }
-    __syncthreads();
+// Stage 2, block a per active IVP: its chunks in order.  scal[a * 8 + k]:
+// 0 nll, 1 obj, 2 ssq (sum / (Ns d)), 3 = 1 if all means are close, 4 = NaN flag.
+static __global__ void k_reduce_final(GroupArgs g, LaunchMap lm, int d, double* scal) {
+  const int a = blockIdx.x;
+  if (a >= lm.m || threadIdx.x != 0) return;
+  const int Ns = g.Ns[lm.ivp[a]];
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  for (int b = lm.pre[a]; b < lm.pre[a + 1]; ++b) {
+    s0 += g.part[(size_t)b * 4 + 0];
+    s1 += g.part[(size_t)b * 4 + 1];
+    s2 += g.part[(size_t)b * 4 + 2];
+    s3 = fmax(s3, g.part[(size_t)b * 4 + 3]);
   }
-  if (tid == 0) {
-    scal[0] = sh[1][0];
-    scal[1] = sh[2][0];
-    scal[2] = sh[0][0] / Ns / d;
-    scal[3] = (sh[3][0] == 0.0) ? 1.0 : 0.0;
-    scal[4] = (isnan(scal[0]) || isnan(scal[1])) ? 1.0 : 0.0;
-  }
+  double* o = scal + (size_t)a * 8;
+  o[0] = s1;
+  o[1] = s2;
+  o[2] = s0 / Ns / d;
+  o[3] = (s3 == 0.0) ? 1.0 : 0.0;
+  o[4] = (isnan(s1) || isnan(s2)) ? 1.0 : 0.0;
 }
 
 // Output projection: means_out[t][i] = P_ii0 m_t[i0], chol_out[t][i][:] = scale * P_ii0 L_t[i0][:]
 // with i0 = i (q+1) (E0 = E0_raw P, pof/convenience.py:26-27; solver.py:71).
 template <int D, int d>
-__global__ void k_output(const double* Pd, StateArrays spre0, int Ns, double scale, double* means_out,
-                         double* chol_out, double* full_means, double* full_chols) {
+__global__ void k_output(GroupArgs g, LaunchMap lm) {
   constexpr int q1 = D / d;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t > Ns) return;
+  int j, t;
+  if (!locate(lm, blockIdx.x * blockDim.x + threadIdx.x, j, t)) return;
+  const int Ns = g.Ns[j];
+  const StateArrays spre0 = at(g.spre, g.base[j], D);
+  const double* Pd = g.Pd + (size_t)j * D;
+  const double scale = g.out_scale[j];
+  double* means_out = g.out_means[j];
+  double* chol_out = g.out_chols[j];
+  double* full_means = g.out_full_means[j];
+  double* full_chols = g.out_full_chols[j];
   const double* m = spre0.m + (size_t)(Ns - t) * D;
   const double* L = spre0.L + (size_t)(Ns - t) * D * D;
 #pragma unroll
@@ -456,10 +529,12 @@ __global__ void k_output(const double* Pd, StateArrays spre0, int Ns, double sca
 
 // Filtered states (state s = x0 for s = 0, else level-0 filter prefix s-1).
 template <int D>
-__global__ void k_filtered_out(const double* x0m, const double* x0L, StateArrays fpre0, int Ns, double* fm,
-                               double* fL) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s > Ns) return;
+__global__ void k_filtered_out(GroupArgs g, LaunchMap lm, double* fm, double* fL) {
+  int j, s;
+  if (!locate(lm, blockIdx.x * blockDim.x + threadIdx.x, j, s)) return;
+  const double* x0m = g.x0m + (size_t)j * D;
+  const double* x0L = g.x0L + (size_t)j * D * D;
+  const StateArrays fpre0 = at(g.fpre, g.base[j], D);
   const double* m = (s == 0) ? x0m : fpre0.m + (size_t)(s - 1) * D;
   const double* L = (s == 0) ? x0L : fpre0.L + (size_t)(s - 1) * D * D;
   if (fm) {
@@ -522,96 +597,77 @@ struct Rec {
 }  // namespace
 
 template <int D>
-void smooth_scan(const Workspace& ws, const ModelArgs& a, int wc, hipStream_t st, Rec& rec) {
-  for (int l = 0; l + 1 < ws.s_levels; ++l) {
-    const int nout = (int)ws.s_n[l + 1];
-    POF_LAUNCH((k_smooth_up<D>), kLdsSmooth, nout, st, a,
-                       at(ws.sel, ws.s_off[l], D), at(ws.sel, ws.s_off[l + 1], D), nout, wc);
-    rec(kKSmoothUp, nout);
+void smooth_scan(const GroupArgs& g, const LaunchPlan& p, hipStream_t st, Rec& rec) {
+  for (int l = 0; l + 1 < p.sl; ++l) {
+    const int nw = p.sup_total[l];
+    if (nw == 0) continue;
+    POF_LAUNCH((k_smooth_up<D>), kLdsSmooth, nw, st, g, p.map(p.sup_off[l], nw), l);
+    rec(kKSmoothUp, nw);
   }
-  for (int l = ws.s_levels - 1; l >= 0; --l) {
-    const int n = (int)ws.s_n[l];
-    StateArrays up = (l + 1 < ws.s_levels) ? at(ws.spre, ws.s_off[l + 1], D) : StateArrays{};
-    POF_LAUNCH((k_smooth_down<D>), kLdsSmooth, (n + 1) / 2, st, a,
-                       at(ws.sel, ws.s_off[l], D), up, at(ws.spre, ws.s_off[l], D), n, wc);
-    rec(kKSmoothDown, (n + 1) / 2 - 1);
+  for (int l = p.sl - 1; l >= 0; --l) {
+    const int nw = p.sdn_total[l];
+    if (nw == 0) continue;
+    POF_LAUNCH((k_smooth_down<D>), kLdsSmooth, nw, st, g, p.map(p.sdn_off[l], nw), l);
+    rec(kKSmoothDown, nw - p.sdn_ivps[l]);
   }
 }
 
 template <int D, int d>
-hipError_t run_smooth_chol(const Workspace& ws, const IterOpts& o, hipStream_t st) {
-  const ModelArgs a = model_args(ws, o);
-  Rec rec(o.timer, st);
-  smooth_scan<D>(ws, a, 1, st, rec);
+hipError_t run_smooth_chol(const GroupArgs& g, const LaunchPlan& p, hipStream_t st, KernelTimer* t) {
+  Rec rec(t, st);
+  smooth_scan<D>(g, p, st, rec);
   return hipGetLastError();
 }
 
 template <int D, int d>
-hipError_t run_iteration(const Workspace& ws, const IterOpts& o, hipStream_t st) {
-  const ModelArgs a = model_args(ws, o);
-  const int Ns = ws.Ns;
-  Rec rec(o.timer, st);
+hipError_t run_iteration(const GroupArgs& g, const LaunchPlan& p, double* scal, hipStream_t st, KernelTimer* t) {
+  Rec rec(t, st);
   // ---- filter: leaf elements, up-sweep, down-sweep
-  {
-    FiltArrays l0 = at(ws.fel, ws.f_off[0], D);
-    POF_LAUNCH((k_filter_leaf<D, d>), kLdsLeaf, Ns, st, a, l0, Ns);
-    rec(kKFilterLeaf, Ns);
-    for (int l = 0; l + 1 < ws.f_levels; ++l) {
-      const int nout = (int)ws.f_n[l + 1];
-      POF_LAUNCH((k_filter_up<D>), kLdsFull, nout, st, a,
-                         at(ws.fel, ws.f_off[l], D), at(ws.fel, ws.f_off[l + 1], D), nout);
-      rec(kKFilterUp, nout);
-    }
-    for (int l = ws.f_levels - 1; l >= 0; --l) {
-      const int n = (int)ws.f_n[l];
-      StateArrays up = (l + 1 < ws.f_levels) ? at(ws.fpre, ws.f_off[l + 1], D) : StateArrays{};
-      POF_LAUNCH((k_filter_down<D>), kLdsState, (n + 1) / 2, st, a,
-                         at(ws.fel, ws.f_off[l], D), up, at(ws.fpre, ws.f_off[l], D), n);
-      rec(kKFilterDown, (n + 1) / 2 - 1);
-    }
+  POF_LAUNCH((k_filter_leaf<D, d>), kLdsLeaf, p.leaf_total, st, g, p.map(p.leaf_off, p.leaf_total));
+  rec(kKFilterLeaf, p.leaf_total);
+  for (int l = 0; l + 1 < p.fl; ++l) {
+    const int nw = p.fup_total[l];
+    if (nw == 0) continue;
+    POF_LAUNCH((k_filter_up<D>), kLdsFull, nw, st, g, p.map(p.fup_off[l], nw), l);
+    rec(kKFilterUp, nw);
+  }
+  for (int l = p.fl - 1; l >= 0; --l) {
+    const int nw = p.fdn_total[l];
+    if (nw == 0) continue;
+    POF_LAUNCH((k_filter_down<D>), kLdsState, nw, st, g, p.map(p.fdn_off[l], nw), l);
+    rec(kKFilterDown, nw - p.fdn_ivps[l]);
   }
   // ---- smoother elements + observation statistics
-  {
-    SmoArrays s0 = at(ws.sel, ws.s_off[0], D);
-    POF_LAUNCH((k_filter_post<D, d>), kLdsPost, Ns + 1, st, a,
-                       at(ws.fpre, ws.f_off[0], D), s0, ws.t_ssq, ws.t_nll);
-    rec(kKFilterPost, Ns);
-  }
+  POF_LAUNCH((k_filter_post<D, d>), kLdsPost, p.pts_total, st, g, p.map(p.pts_off, p.pts_total));
+  rec(kKFilterPost, p.leaf_total);
   // ---- smoother scan (reversed order)
-  smooth_scan<D>(ws, a, o.with_chol, st, rec);
-  // ---- smoothed means, objective, convergence partials, reduction
-  {
-    const int nt = Ns + 1;
-    hipLaunchKernelGGL((k_smooth_post<D>), dim3((nt + 127) / 128), dim3(128), 0, st, ws.F, ws.LQ,
-                       at(ws.spre, ws.s_off[0], D), ws.means[o.cur], ws.means[1 - o.cur], ws.t_obj, ws.t_viol, Ns);
-    rec(kKSmoothPost, Ns + 1);
-    hipLaunchKernelGGL(k_reduce_part, dim3(kRedBlocks), dim3(256), 0, st, ws.t_ssq, ws.t_nll, ws.t_obj,
-                       ws.t_viol, Ns, ws.part);
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedBlocks), 0, st, ws.part, Ns, d, ws.scal);
-    rec(kKReduce, Ns);
-  }
+  smooth_scan<D>(g, p, st, rec);
+  // ---- smoothed means, objective, convergence partials, per-IVP reduction
+  hipLaunchKernelGGL((k_smooth_post<D>), dim3((p.pts_total + 127) / 128), dim3(128), 0, st, g,
+                     p.map(p.pts_off, p.pts_total));
+  rec(kKSmoothPost, p.pts_total);
+  hipLaunchKernelGGL(k_reduce_part, dim3(p.chunk_total), dim3(256), 0, st, g, p.map(p.chunk_off, p.chunk_total));
+  hipLaunchKernelGGL(k_reduce_final, dim3(p.m), dim3(64), 0, st, g, p.map(p.chunk_off, p.chunk_total), d, scal);
+  rec(kKReduce, p.leaf_total);
   return hipGetLastError();
 }
 
 template <int D, int d>
-hipError_t run_output(const Workspace& ws, int cur, double scale, double* means_out, double* chol_out,
-                      double* full_means, double* full_chols, double* filt_means, double* filt_chols,
-                      hipStream_t st) {
-  (void)cur;
-  const int nt = ws.Ns + 1;
-  hipLaunchKernelGGL((k_output<D, d>), dim3((nt + 127) / 128), dim3(128), 0, st, ws.Pd,
-                     at(ws.spre, ws.s_off[0], D), ws.Ns, scale, means_out, chol_out, full_means, full_chols);
-  if (filt_means || filt_chols) {
-    hipLaunchKernelGGL((k_filtered_out<D>), dim3((nt + 127) / 128), dim3(128), 0, st, ws.x0m, ws.x0L,
-                       at(ws.fpre, ws.f_off[0], D), ws.Ns, filt_means, filt_chols);
-  }
+hipError_t run_output(const GroupArgs& g, const LaunchPlan& p, hipStream_t st) {
+  hipLaunchKernelGGL((k_output<D, d>), dim3((p.pts_total + 127) / 128), dim3(128), 0, st, g,
+                     p.map(p.pts_off, p.pts_total));
   return hipGetLastError();
 }
 
 template <int D, int d>
-hipError_t run_linearize(const Workspace& ws, const IterOpts& o, const double* m, int n, double* H, double* c,
-                         hipStream_t st) {
-  const ModelArgs a = model_args(ws, o);
+hipError_t run_filtered(const GroupArgs& g, const LaunchPlan& p, double* fm, double* fL, hipStream_t st) {
+  hipLaunchKernelGGL((k_filtered_out<D>), dim3((p.pts_total + 127) / 128), dim3(128), 0, st, g,
+                     p.map(p.pts_off, p.pts_total), fm, fL);
+  return hipGetLastError();
+}
+
+template <int D, int d>
+hipError_t run_linearize(const ModelArgs& a, const double* m, int n, double* H, double* c, hipStream_t st) {
   hipLaunchKernelGGL((k_linearize<D, d>), dim3((n + 127) / 128), dim3(128), 0, st, a, m, n, H, c);
   return hipGetLastError();
 }

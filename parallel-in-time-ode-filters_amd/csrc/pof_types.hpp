@@ -43,4 +43,53 @@ struct StateArrays {
   double* L;    // [n, D, D]
 };
 
+// --------------------------------------------------------------- batch group
+// A group of IVPs that share (D, d, q) and therefore F, LQ and N1z.  IVP j
+// owns the entry range [base[j], base[j] + 2 (Ns[j] + 1)) of every array:
+// its filter levels (entries base + sum_{t<l} Ns >> t), its smoother levels
+// (n_0 = Ns + 1, same rule) and its time points (Ns + 1 entries).
+struct GroupArgs {
+  const double* F;            // [D*D]
+  const double* LQ;           // [D*D]
+  const double* N1z;          // [D*D]
+  const double* Pd;           // [slots, D]
+  const double* x0m;          // [slots, D]
+  const double* x0L;          // [slots, D*D]
+  const double* params;       // [slots, kMaxParams]
+  const int* kind;            // [slots]
+  const int* Ns;              // [slots]
+  const long long* base;      // [slots] first entry of the IVP's range
+  const int* cur;             // [slots] means buffer holding the linearisation points
+  const int* wchol;           // [slots] 1: smoother scans also carry the factors
+  double* means0;             // [entries, D]
+  double* means1;             // [entries, D]
+  FiltArrays fel;             // filter elements and up-sweep aggregates
+  StateArrays fpre;           // filter prefix states
+  SmoArrays sel;              // smoother elements and aggregates
+  StateArrays spre;           // smoother prefix states
+  double* t_ssq;              // [entries] per-step partial terms
+  double* t_nll;
+  double* t_obj;
+  double* t_viol;
+  double* part;               // [chunks, 4] reduction partials
+  double* const* out_means;   // [slots] caller-owned outputs (may hold null)
+  double* const* out_chols;
+  double* const* out_full_means;
+  double* const* out_full_chols;
+  const double* out_scale;    // [slots] factor scale of the output
+  const double* Hx;           // affine mode (one IVP at entry 0): [Ns, d, D]
+  const double* cx;           // [Ns, d]
+  int q;
+  int affine;
+};
+
+// Work list of one launch: active IVP a (id ivp[a]) owns the work items
+// [pre[a], pre[a + 1]); total = pre[m].
+struct LaunchMap {
+  const int* ivp;
+  const int* pre;
+  int m;
+  int total;
+};
+
 }  // namespace pof

@@ -16,6 +16,7 @@ POF_E_ARG = -1
 POF_E_UNSUPPORTED = -2
 POF_E_HIP = -3
 POF_E_INIT = -4
+POF_E_FULL = -5
 
 KINDS = {
     "logistic": 1,
@@ -101,6 +102,16 @@ SIGNATURES = {
         [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int64, _dp, _dp, _dp, _dp, _vp, _vp, _vp, _vp, _vp, _vp,
          _dp, _dp, _dp, _vp],
     ),
+    "pof_batch_entries_for": (ctypes.c_int64, [ctypes.c_int64]),
+    "pof_batch_bytes": (ctypes.c_int64, [ctypes.c_int, ctypes.c_int64, ctypes.c_int32]),
+    "pof_batch_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int32, _P(_vp)]),
+    "pof_batch_destroy": (None, [_vp]),
+    "pof_batch_admit": (ctypes.c_int, [_vp, _P(Problem), _P(Opts), _P(Out), _P(ctypes.c_int64)]),
+    "pof_batch_launch": (ctypes.c_int, [_vp]),
+    "pof_batch_finish": (ctypes.c_int, [_vp, ctypes.c_int32, _P(ctypes.c_int64), _P(ctypes.c_int32)]),
+    "pof_batch_active": (ctypes.c_int, [_vp]),
+    "pof_batch_free_entries": (ctypes.c_int64, [_vp]),
+    "pof_solve_batch": (ctypes.c_int, [_vp, _P(Problem), _P(Opts), _P(Out), ctypes.c_int32]),
 }
 
 _lock = threading.Lock()

@@ -1,0 +1,149 @@
+"""Batched and continuously batched solves on the GPU.
+
+Every IVP in a batch runs the same kernels on its own workspace range, with
+its own reductions and convergence test, so its result must be bitwise
+identical to the single-IVP solve (``torch.equal``), whatever else shares the
+launches.  The single solves themselves are checked against the oracle in
+test_gpu_parity.py; a few batch members are also checked against the oracle
+here (means to 1e-9, 1e-8 for q = 4; iterations exact).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _mix(seed, n, nmax=300):
+    """A seeded mix of IVPs over several (d, order) groups with uneven lengths."""
+    import pof.ivp
+
+    rng = np.random.default_rng(seed)
+    makers = [
+        lambda r: pof.ivp.logistic(r=float(r.uniform(0.5, 3.0))),
+        lambda r: pof.ivp.fhn_classic(a=float(r.uniform(0.1, 0.3)), b=float(r.uniform(0.1, 0.3)),
+                                      c=float(r.uniform(2, 4))),
+        lambda r: pof.ivp.lorenz63(rho=float(r.uniform(20, 30)), tmax=2.0),
+        lambda r: pof.ivp.fitzhughnagumo(tmax=20.0),
+    ]
+    probs = []
+    for i in range(n):
+        ivp = makers[i % len(makers)](rng)
+        y0 = ivp.y0 + 0.1 * rng.standard_normal(ivp.y0.shape)
+        N = int(rng.integers(16, nmax))
+        q = int(rng.integers(1, 5))
+        if ivp.f.name == "lorenz63" and q == 4:
+            q = 3  # (D, d) = (15, 3) is the largest d = 3 table entry used here
+        ts = np.linspace(ivp.t0, ivp.tmax, N)
+        probs.append((ivp.f, y0, ts, q, "constant"))
+    return probs
+
+
+def _equal(a, b):
+    import torch
+
+    return torch.equal(a, b)
+
+
+def test_batch_equals_single_solves(gpu):
+    import pof.batch
+    import pof.solver
+
+    probs = _mix(4, 24)
+    print(f"batch of {len(probs)} mixed IVPs", flush=True)
+    res = pof.batch.solve_batch(probs, maxiters=60)
+    for (f, y0, ts, q, init), (ys, info) in zip(probs, res):
+        ys1, info1 = pof.solver.solve(f=f, y0=y0, ts=ts, order=q, init=init, maxiters=60)
+        assert info["iterations"] == info1["iterations"], (f.name, q, len(ts))
+        assert info["status"] == info1["status"]
+        assert _equal(ys.mean, ys1.mean), (f.name, q, len(ts))
+        assert _equal(ys.chol, ys1.chol)
+        assert info["sigma_squared"] == info1["sigma_squared"]
+        assert info["nll"] == info1["nll"] and info["obj"] == info1["obj"]
+
+
+def test_batch_members_match_oracle(gpu):
+    import pof.batch
+    from oracle import problems as op
+    from oracle import solver as osolver
+
+    # non-chaotic members only: converged iterates, so rounding stays at the 1e-12 level
+    probs = [p for p in _mix(7, 12, nmax=120) if p[0].name != "lorenz63"]
+    res = pof.batch.solve_batch(probs, maxiters=60)
+    for (f, y0, ts, q, init), (ys, info) in zip(probs, res):
+        oprob = op.Problem(f.name, f.params, y0, ts[0], ts[-1])
+        (om, _), oinfo = osolver.solve(oprob, ts, q, init=init, maxiters=60)
+        m = ys.mean.cpu().numpy()
+        assert info["iterations"] == oinfo["iterations"], (f.name, q, len(ts))
+        err = np.abs(m - om).max() / max(np.abs(om).max(), 1e-300)
+        # q = 4 priors (cond(Q) = 4.8e5) amplify rounding to ~5e-9; the acceptance bar is 1e-6
+        assert err < (1e-8 if q == 4 else 1e-9), (f.name, q, len(ts), err)
+
+
+def test_fixed_iterations_batch(gpu):
+    import pof.batch
+    import pof.solver
+
+    probs = _mix(11, 9)
+    res = pof.batch.solve_batch(probs, fixed_iters=3)
+    for (f, y0, ts, q, init), (ys, info) in zip(probs, res):
+        ys1, info1 = pof.solver.solve(f=f, y0=y0, ts=ts, order=q, init=init, fixed_iters=3)
+        assert info["iterations"] == 3.0 and info["status"] == "fixed"
+        assert _equal(ys.mean, ys1.mean) and _equal(ys[1], ys1[1])
+
+
+def test_continuous_admission_same_as_alone(gpu):
+    """IVPs admitted while others are mid-solve: each result equals its solve alone."""
+    import pof.batch
+    import pof.ivp
+    import pof.solver
+
+    rng = np.random.default_rng(5)
+    b = pof.batch.Batch(d=2, order=3, entries=20000, slots=8)
+    ivps = []
+    for i in range(7):
+        ivp = pof.ivp.fhn_classic(a=float(rng.uniform(0.1, 0.3)), b=float(rng.uniform(0.1, 0.3)),
+                                  c=float(rng.uniform(2, 4)))
+        ts = np.linspace(0.0, 20.0, int(rng.integers(50, 900)))
+        ivps.append((ivp.f, ivp.y0 + 0.1 * rng.standard_normal(2), ts))
+    results = {}
+    tickets = {}
+    order = list(range(len(ivps)))
+    # admit three, iterate twice, admit the rest one per step
+    for i in order[:3]:
+        tickets[b.admit(ivps[i][0], ivps[i][1], ivps[i][2], init="constant", maxiters=80)] = i
+    steps = 0
+    pending = order[3:]
+    while b.active or pending:
+        if steps >= 2 and pending:
+            i = pending.pop(0)
+            tickets[b.admit(ivps[i][0], ivps[i][1], ivps[i][2], init="constant", maxiters=80)] = i
+        for t, r in b.step():
+            results[tickets[t]] = r
+        steps += 1
+        assert steps < 500
+    assert len(results) == len(ivps)
+    for i, (f, y0, ts) in enumerate(ivps):
+        ys1, info1 = pof.solver.solve(f=f, y0=y0, ts=ts, order=3, init="constant", maxiters=80)
+        ys, info = results[i]
+        assert info["iterations"] == info1["iterations"], i
+        assert _equal(ys.mean, ys1.mean), i
+        assert _equal(ys[1], ys1[1]), i
+    b.close()
+
+
+def test_batch_full_is_reported(gpu):
+    import pof.batch
+    import pof.ivp
+
+    ivp = pof.ivp.logistic()
+    b = pof.batch.Batch(d=1, order=2, entries=pof.batch.entries_for(100), slots=4)
+    b.admit(ivp.f, ivp.y0, np.linspace(0, 10, 100), init="constant")
+    with pytest.raises(pof.batch.BatchFull):
+        b.admit(ivp.f, ivp.y0, np.linspace(0, 10, 50), init="constant")
+    while b.active:
+        b.step()
+    # the range is free again
+    b.admit(ivp.f, ivp.y0, np.linspace(0, 10, 100), init="constant")
+    while b.active:
+        b.step()
+    b.close()
