@@ -6,6 +6,11 @@ IWP order q = 3 (state dimension D = 12), float64 throughout, constant
 initial trajectory, exactly 10 IEKS iterations per solve.  Inputs are tiny
 (y0, ts); everything else lives in HBM.
 
+Also measured (same line, objects "batched" and "served"): a batch of 2048
+mixed IVPs solved together vs the plain one-IVP-per-launch scan, and a
+Poisson request stream served with continuous batching vs a FIFO plain
+per-request scan (p50/p99 latency, solves/s).
+
 Prints one JSON line: solves/s over K timed solves, the roofline of the
 dominant kernel class (algorithmic flops per launch over its mean launch
 time, measured with HIP events on the launch stream in an instrumented
@@ -159,6 +164,114 @@ def cpu_baseline():
     }
 
 
+def batched_run(n_ivps, maxiters, plain_sample, oracle_sample, seed=4):
+    """Mixed-IVP batch (one launch per scan level per (d, q) group) vs the plain per-IVP scan.
+
+    The batch is solved once untimed (allocation, warm-up) and once timed.  The
+    plain one-IVP-per-launch time is the sum over a random sample of the IVPs
+    solved one at a time (each checked to equal its batch result bitwise),
+    scaled to the whole batch.
+    """
+    import numpy as np
+    import torch
+
+    import pof.batch
+    import pof.solver
+    from pof import workloads
+
+    probs = workloads.mixed_ivps(n_ivps, seed=seed)
+    pof.batch.solve_batch(probs, maxiters=maxiters)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = pof.batch.solve_batch(probs, maxiters=maxiters)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    rng = np.random.default_rng(0)
+    idx = rng.choice(n_ivps, size=min(plain_sample, n_ivps), replace=False)
+    t_plain = 0.0
+    same = True
+    for i in idx:
+        f, y0, ts, q, init = probs[i]
+        t0 = time.perf_counter()
+        ys, _ = pof.solver.solve(f=f, y0=y0, ts=ts, order=q, init=init, maxiters=maxiters)
+        torch.cuda.synchronize()
+        t_plain += time.perf_counter() - t0
+        same = same and bool(torch.equal(ys.mean, res[i][0].mean)) and bool(torch.equal(ys.chol, res[i][0].chol))
+    plain_total = t_plain / len(idx) * n_ivps
+    # accuracy against the FP64 oracle on the cheapest converged members
+    from oracle import problems as op
+    from oracle import solver as osol
+
+    conv = [i for i in range(n_ivps) if res[i][1]["status"] == "converged"]
+    conv.sort(key=lambda i: len(probs[i][2]) * (probs[i][0].d * (probs[i][3] + 1)) ** 3)
+    errs = []
+    for i in conv[:oracle_sample]:
+        f, y0, ts, q, init = probs[i]
+        (om, _), _ = osol.solve(op.Problem(f.name, f.params, y0, ts[0], ts[-1]), ts, q, init=init,
+                                maxiters=maxiters)
+        errs.append(workloads.max_rel_err(res[i][0].mean.cpu().numpy(), om))
+    status = [r[1]["status"] for r in res]
+    its = np.array([r[1]["iterations"] for r in res])
+    return {
+        "workload": f"{n_ivps} IVPs: logistic / FitzHugh-Nagumo classic / Lorenz-63 with random parameters and y0, "
+                    f"Ns log-uniform [256, 16384], q uniform 1..4, f64, maxiters {maxiters}, constant init, seed {seed}",
+        "solves_per_s": n_ivps / t_batch,
+        "seconds": t_batch,
+        "plain_solves_per_s": n_ivps / plain_total,
+        "plain_seconds_estimated": plain_total,
+        "plain_sample": int(len(idx)),
+        "speedup_vs_plain": plain_total / t_batch,
+        "same_as_alone": same,
+        "iterations_mean": float(its.mean()),
+        "status_counts": {k: status.count(k) for k in sorted(set(status))},
+        "max_rel_err_vs_oracle": max(e[0] for e in errs) if errs else None,
+        "normwise_err_vs_oracle": max(e[1] for e in errs) if errs else None,
+        "oracle_sample": len(errs),
+    }
+
+
+def served_run(n_req, rate, maxiters, seed=5):
+    """Poisson request stream served with continuous batching vs a FIFO plain per-request scan."""
+    import numpy as np
+    import torch
+
+    import pof.serve
+    import pof.solver
+    from pof import workloads
+
+    reqs = workloads.mixed_ivps(n_req, seed=seed, ns_min=256, ns_max=65536)
+    arr = workloads.poisson_arrivals(n_req, rate, seed=seed)
+    srv = pof.serve.Server(budget_bytes=96 << 30, maxiters=maxiters)
+    srv.run(reqs[: min(24, n_req)], np.zeros(min(24, n_req)))  # warm-up: allocate the groups
+    t0 = time.perf_counter()
+    recs = srv.run(reqs, arr)
+    wall = time.perf_counter() - t0
+    st = pof.serve.latency_stats(recs)
+
+    def solve_one(r):
+        f, y0, ts, q, init = r
+        ys, _ = pof.solver.solve(f=f, y0=y0, ts=ts, order=q, init=init, maxiters=maxiters)
+        return ys.mean.cpu().numpy()
+
+    t0 = time.perf_counter()
+    precs = pof.serve.fifo_plain(reqs, arr, solve_one)
+    pwall = time.perf_counter() - t0
+    pst = pof.serve.latency_stats(precs)
+    same = all(np.array_equal(a.means, b.means) for a, b in zip(recs, precs))
+    srv.close()
+    return {
+        "workload": f"{n_req} requests, Poisson arrivals at {rate:.1f}/s (seed {seed}), same IVP mix with "
+                    f"Ns log-uniform [256, 65536], maxiters {maxiters}",
+        "solves_per_s": n_req / wall,
+        "p50_ms": st["p50_ms"],
+        "p99_ms": st["p99_ms"],
+        "plain_fifo_solves_per_s": n_req / pwall,
+        "plain_fifo_p50_ms": pst["p50_ms"],
+        "plain_fifo_p99_ms": pst["p99_ms"],
+        "same_as_alone": bool(same),
+    }
+
+
 def copy_probe():
     """A plain device copy of known size (2^25 float64 = 256 MiB read + 256 MiB written per launch).
 
@@ -184,6 +297,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--copy-probe", action="store_true", help="only run the HBM counter calibration copy")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched and served measurements")
+    ap.add_argument("--batch-ivps", type=int, default=2048)
+    ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--serve-rate", type=float, default=None,
+                    help="requests/s (default: 70%% of the measured batched capacity, scaled to the longer Ns)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part of this build",
@@ -195,6 +313,16 @@ def main():
     g = gpu_run(args.steps, args.warmup)
     if args.verbose:
         print(json.dumps(g["per_class"], indent=1), file=sys.stderr)
+    batched = served = None
+    if not args.no_batch:
+        batched = batched_run(args.batch_ivps, maxiters=100, plain_sample=32, oracle_sample=4)
+        if args.verbose:
+            print(json.dumps(batched), file=sys.stderr, flush=True)
+        # mean Ns of log-uniform [256, 65536] over [256, 16384]: 3.04x more work per request
+        rate = args.serve_rate or 0.7 * batched["solves_per_s"] / 3.04
+        served = served_run(args.serve_requests, rate, maxiters=100)
+        if args.verbose:
+            print(json.dumps(served), file=sys.stderr, flush=True)
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     ms_step = 1e3 * g["seconds"] / args.steps
     out = {
@@ -223,6 +351,8 @@ def main():
         "cpu_baseline": cpu,
         "kernel_classes_ms_per_solve": {k: round(v["ms"], 4) for k, v in g["per_class"].items()},
         "finite": g["finite"],
+        "batched": batched,
+        "served": served,
     }
     print(json.dumps(out), flush=True)
 
