@@ -1,12 +1,19 @@
 // Device building blocks of the square-root parallel IEKS on CDNA (gfx950).
 //
-// Execution model: one "worker" of W lanes (W = 32 for D <= 16, 64 for
-// D <= 32; two or one workers per 64-wide wavefront) owns one scan element.
-// Lane r of a worker holds row r of every matrix of the element in registers
-// (row-per-lane layout), so the row-local parts of products, triangular
-// solves and Householder updates need no communication.  Rows that other
-// lanes need are staged through the worker's slice of LDS; a worker never
+// Execution model: one "worker" of W lanes (32 or 64) owns one scan element.
+// Row r of every matrix of the element is spread over S adjacent lanes
+// (lanes r S .. r S + S - 1; S = 4 for D <= 8, 2 for D <= 16, else 1, in the
+// scan kernels; S = 1 in the per-element kernels): the lane
+// with sub-index s holds the columns c = t S + s, t = 0, 1, ... of the row
+// in registers ("split rows").  Reductions along a row and broadcasts from
+// one column owner to the others are DPP quad permutations; rows that other
+// lanes need are staged through the worker's slice of LDS.  A worker never
 // synchronises with other workers, only with its own wavefront.
+//
+// Rows of D (or d) columns are padded with zeros to a multiple of S.  A
+// matrix built from two column blocks (e.g. [M | U2]) keeps each block
+// padded; the Householder LQ maps pivot i to its padded column, so the
+// padding changes no reflector.
 //
 // Arithmetic follows the reference (pof/parallel_filtsmooth/filter.py:50-142,
 // smoother.py:37-63, utils.py:33-41) in float64.  tria() is a Householder LQ
@@ -31,6 +38,29 @@ __device__ __forceinline__ void wsync() {
 }
 
 // ------------------------------------------------------------------ layout
+template <int n, int S>
+constexpr int padded() {
+  return (n + S - 1) / S * S;
+}
+
+// Widest row split for state dimension D: 2D rows of S lanes fit in 64 lanes.
+template <int D>
+constexpr int split_of() {
+  return (8 * D <= 64) ? 4 : (4 * D <= 64) ? 2 : 1;
+}
+
+// Lanes per worker and the padded row length for state dimension D with
+// rows split over S lanes.
+template <int D_, int S_>
+struct Lay {
+  static constexpr int D = D_;
+  static constexpr int S = S_;
+  static constexpr int W = (2 * D_ * S <= 32) ? 32 : 64;
+  static constexpr int DP = padded<D_, S>();
+  static constexpr int LD = DP + 1;  // LDS row stride (odd: no bank conflicts down a column)
+  static constexpr int NL = DP / S;  // registers of a D-column row
+};
+
 // LDS layout kinds: how many tiles / broadcast vectors a worker of each
 // kernel needs and whether the block stages the model constants F, LQ, N1z.
 enum LdsKind : int { kLdsFull = 0, kLdsState, kLdsSmooth, kLdsLeaf, kLdsPost };
@@ -42,35 +72,44 @@ struct LdsNeeds {
   static constexpr bool CONSTS = (K == kLdsLeaf || K == kLdsPost);
 };
 
+// Row split per kernel kind: the scan combines use the widest split (their
+// work is per column); the per-element kernels, dominated by work every lane
+// of a row repeats, keep one lane per row and two workers per wavefront.
+template <int D, int K>
+constexpr int kind_split() {
+  return (K == kLdsLeaf || K == kLdsPost) ? 1 : split_of<D>();
+}
+
 template <int D_, int K = kLdsFull>
-struct Geo {
-  static constexpr int D = D_;
-  static constexpr int W = (2 * D_ <= 32) ? 32 : 64;      // lanes per worker
-  static constexpr int LD = D_ + 1;                      // padded LDS row stride
-  static constexpr int TILE = D_ * LD;
-  static constexpr int NT = LdsNeeds<K>::NT;             // tiles per worker
-  static constexpr int NV = LdsNeeds<K>::NV;             // broadcast vectors per worker
-  static constexpr int BC = 4 * D_ + 2;                  // Householder broadcast rows (two matrices)
+struct Geo : Lay<D_, kind_split<D_, K>()> {
+  using L = Lay<D_, kind_split<D_, K>()>;
+  static constexpr int TILE = D_ * L::LD;
+  static constexpr int NT = LdsNeeds<K>::NT;        // tiles per worker
+  static constexpr int NV = LdsNeeds<K>::NV;        // broadcast vectors per worker (2D entries each)
+  static constexpr int BC = 4 * L::DP + 2;          // pivot-row broadcast of two matrices of 2 DP columns
   static constexpr int WORKER = BC + NV * 2 * D_ + NT * TILE;
   static constexpr int CONSTS = LdsNeeds<K>::CONSTS ? 3 * TILE : 0;  // F, LQ, N1z
   static constexpr size_t bytes_for(int wpb) { return sizeof(double) * (CONSTS + wpb * WORKER); }
   // workers per block: as many as fit in 64 KiB of LDS, at most 256 threads
-  static constexpr int WPB = (W == 32 && bytes_for(8) <= 65536)   ? 8
-                             : (W == 32 && bytes_for(4) <= 65536) ? 4
-                             : (bytes_for(2) <= 65536)            ? 2
-                                                                  : 1;
-  static constexpr int BLOCK = W * WPB;
+  static constexpr int WPB = (L::W == 32 && bytes_for(8) <= 65536)   ? 8
+                             : (bytes_for(256 / L::W) <= 65536)      ? 256 / L::W
+                             : (bytes_for(2) <= 65536)               ? 2
+                                                                     : 1;
+  static constexpr int BLOCK = L::W * WPB;
   static constexpr size_t lds_bytes() { return bytes_for(WPB); }
   static_assert(bytes_for(WPB) <= 65536, "state dimension too large for one worker per block");
 };
 
-template <int D>
+template <int D, int S_>
 struct Ctx {
-  static constexpr int LD = D + 1;
+  static constexpr int S = S_;
+  static constexpr int LD = Lay<D, S_>::LD;
   static constexpr int TILE = D * LD;
   int lane;           // lane inside the worker
+  int row;            // matrix row of this lane
+  int sub;            // column class of this lane inside its row
   int wid;            // global worker index
-  double* w;          // worker LDS slice (starts with the Householder broadcast row)
+  double* w;          // worker LDS slice (starts with the Householder broadcast rows)
   double* vbase;      // broadcast vectors
   double* tbase;      // tiles
   const double* sF;   // block LDS constants, row stride LD (kernels that stage them)
@@ -81,23 +120,36 @@ struct Ctx {
   __device__ double* bc() const { return w; }
 };
 
-// Stage F, LQ, N1z into block LDS (kinds that need them) and carve out this
-// thread's worker slice.  Must be called by every thread of the block before
-// any early exit.
+// Opaque copy of a value: keeps the optimiser from correlating the many lane
+// tests of one function (which otherwise duplicates whole regions per branch
+// and multiplies register pressure).
+__device__ __forceinline__ int opaque(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
+// Stage F, LQ, N1z into block LDS (kinds that need them; padding columns are
+// zero) and carve out this thread's worker slice.  Must be called by every
+// thread of the block before any early exit.
 template <int D, int K>
-__device__ __forceinline__ Ctx<D> make_ctx(const double* F, const double* LQ, const double* N1z, double* smem) {
+__device__ __forceinline__ Ctx<D, Geo<D, K>::S> make_ctx(const double* F, const double* LQ, const double* N1z,
+                                                          double* smem) {
   using G = Geo<D, K>;
   if constexpr (G::CONSTS > 0) {
-    for (int t = threadIdx.x; t < D * D; t += G::BLOCK) {
-      const int r = t / D, c = t % D;
-      smem[r * G::LD + c] = F[t];
-      smem[G::TILE + r * G::LD + c] = LQ[t];
-      smem[2 * G::TILE + r * G::LD + c] = N1z[t];
+    for (int t = threadIdx.x; t < D * G::DP; t += G::BLOCK) {
+      const int r = t / G::DP, c = t % G::DP;
+      const bool in = c < D;
+      const int cc = in ? c : 0;
+      smem[r * G::LD + c] = in ? F[r * D + cc] : 0.0;
+      smem[G::TILE + r * G::LD + c] = in ? LQ[r * D + cc] : 0.0;
+      smem[2 * G::TILE + r * G::LD + c] = in ? N1z[r * D + cc] : 0.0;
     }
     __syncthreads();
   }
-  Ctx<D> cx;
+  Ctx<D, G::S> cx;
   cx.lane = threadIdx.x % G::W;
+  cx.row = opaque(cx.lane / G::S);
+  cx.sub = opaque(cx.lane % G::S);
   const int wl = threadIdx.x / G::W;
   cx.wid = blockIdx.x * G::WPB + wl;
   cx.w = smem + G::CONSTS + wl * G::WORKER;
@@ -110,14 +162,6 @@ __device__ __forceinline__ Ctx<D> make_ctx(const double* F, const double* LQ, co
 }
 
 // ----------------------------------------------------------------- helpers
-// Opaque copy of a value: keeps the optimiser from correlating the many lane
-// tests of one function (which otherwise duplicates whole regions per branch
-// and multiplies register pressure).
-__device__ __forceinline__ int opaque(int v) {
-  asm volatile("" : "+v"(v));
-  return v;
-}
-
 // Pin accumulators at a point of the instruction stream.  Within one basic
 // block the instruction selector is free to emit every load of an unrolled
 // product first and the arithmetic last, which keeps each loaded value in a
@@ -149,33 +193,6 @@ __device__ __forceinline__ void store_row(double* p, const double (&x)[N]) {
   for (int i = 0; i < N; ++i) p[i] = x[i];
 }
 
-// out[c] = sum_k x[k] * T[k][c]   (row of X @ T), T with row stride LD
-template <int K, int NC, int LD, int NX>
-__device__ __forceinline__ void row_times_tile(const double (&x)[NX], const double* T, double (&out)[NC]) {
-#pragma unroll
-  for (int c = 0; c < NC; ++c) out[c] = 0.0;
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-#pragma unroll
-    for (int c = 0; c < NC; ++c) out[c] = fma(x[k], T[k * LD + c], out[c]);
-    pin(out);
-  }
-}
-
-// out[c] = sum_k T1[k][r] * T2[k][c]   (row r of T1^T @ T2)
-template <int K, int NC, int LD>
-__device__ __forceinline__ void tile_t_times_tile(int r, const double* T1, const double* T2, double (&out)[NC]) {
-#pragma unroll
-  for (int c = 0; c < NC; ++c) out[c] = 0.0;
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    const double t = T1[k * LD + r];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) out[c] = fma(t, T2[k * LD + c], out[c]);
-    pin(out);
-  }
-}
-
 // 1/x to about one ulp: the hardware estimate refined by two Newton steps
 // (x is a nonzero norm-sized value here; no denormal or overflow handling).
 __device__ __forceinline__ double rcp(double x) {
@@ -186,76 +203,322 @@ __device__ __forceinline__ double rcp(double x) {
   return fma(r, e, r);
 }
 
-// Householder LQ of NG independent NR x NC matrices; row r of matrix g sits
-// in lane g * NR + r.  On return that lane holds row r of the lower-
-// triangular factor L of its matrix (columns > r are zero), L L^T = M M^T:
-// the reflectors of LAPACK dgeqrf on M^T (dlarfg: beta = -sign(alpha) *
-// norm, tau = (beta - alpha) / beta, v = [1, x / (alpha - beta)], tau = 0
-// when the tail x is zero, no reflector for a length-1 tail).
-//
-// Per pivot step the pivot lane broadcasts its raw row through LDS and every
-// lane derives (beta, tau, scale) itself, so there is no divergent pivot
-// section; the dot products with the unscaled tail overlap the square root
-// and reciprocals.  Finished rows are not cleared per step; the strict upper
-// part is zeroed once at the end.
-template <int NR, int NC, int NG = 1>
-__device__ __forceinline__ void lq(double (&a)[NC], int lane_in, double* bc) {
-  static_assert(NG == 1 || NG == 2, "one or two matrices");
-  constexpr int K = (NR < NC) ? NR : NC;
-  // opaque copy of the lane index: stops the optimiser from threading the
-  // per-pivot lane tests into specialised copies of the surrounding code
-  int lane = lane_in;
-  asm volatile("" : "+v"(lane));
-  const int g = (NG == 2 && lane >= NR) ? 1 : 0;
-  const int li = lane - g * NR;
-  const bool valid = lane < NG * NR;
-  double* row = bc + g * NC;  // broadcast row of this lane's matrix
+// sqrt(x) and 1/sqrt(x) for a normal x > 0, to about one ulp: the hardware
+// reciprocal square root estimate refined by Goldschmidt's iteration
+// (g -> sqrt(x), h -> 1 / (2 sqrt(x))) and a final Newton correction of g.
+__device__ __forceinline__ void sqrt_rsqrt(double x, double& sq, double& rsq) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  double r = fma(-g, h, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  r = fma(-g, h, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  const double d = fma(-g, g, x);
+  sq = fma(d, h, g);
+  rsq = 2.0 * h;
+}
+
+// ------------------------------------------------------- row communication
+// Quad permutation of a double (two 32-bit DPP moves).
+template <int CTRL>
+__device__ __forceinline__ double dppd(double v) {
+  const long long x = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)x, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(x >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Sum over the S lanes of a row; every lane of the row gets the same bits.
+template <int S>
+__device__ __forceinline__ double rowsum(double v) {
+  if constexpr (S >= 2) v = v + dppd<0xB1>(v);  // quad_perm [1,0,3,2]
+  if constexpr (S >= 4) v = v + dppd<0x4E>(v);  // quad_perm [2,3,0,1]
+  return v;
+}
+
+// The value held by sub-lane K of this lane's row.
+template <int S, int K>
+__device__ __forceinline__ double from_sub(double v) {
+  if constexpr (S == 1) {
+    return v;
+  } else if constexpr (S == 2) {
+    return dppd<(K == 0) ? 0xA0 : 0xF5>(v);  // quad_perm [K,K,2+K,2+K]
+  } else {
+    return dppd<(K & 3) * 0x55>(v);  // quad_perm [K,K,K,K]
+  }
+}
+// k is a compile-time constant after unrolling
+template <int S>
+__device__ __forceinline__ double from_sub_k(double v, int k) {
+  if (S == 1 || k == 0) return from_sub<S, 0>(v);
+  if (k == 1) return from_sub<S, 1>(v);
+  if (k == 2) return from_sub<S, 2>(v);
+  return from_sub<S, 3>(v);
+}
+
+// x[t] = p[t S + sub] for columns < n, 0 for the padding columns.
+template <int S, int n, int NX>
+__device__ __forceinline__ void load_dist(double (&x)[NX], const double* p, int sub) {
 #pragma unroll
-  for (int i = 0; i < K; ++i) {
+  for (int t = 0; t < NX; ++t) {
+    const int c = t * S + sub;
+    if ((t + 1) * S <= n) {
+      x[t] = p[c];
+    } else if (t * S < n) {
+      const bool in = c < n;
+      const double v = p[in ? c : 0];
+      x[t] = in ? v : 0.0;
+    } else {
+      x[t] = 0.0;
+    }
+  }
+}
+
+// p[t S + sub] = x[t] for columns < n.
+template <int S, int n, int NX>
+__device__ __forceinline__ void store_dist(double* p, const double (&x)[NX], int sub) {
+#pragma unroll
+  for (int t = 0; t < NX; ++t) {
+    const int c = t * S + sub;
+    if ((t + 1) * S <= n) {
+      p[c] = x[t];
+    } else if (t * S < n) {
+      if (c < n) p[c] = x[t];
+    }
+  }
+}
+
+// Replicated copy f[0..n) of a row held in split form.
+template <int S, int n, int NX>
+__device__ __forceinline__ void gather(const double (&x)[NX], double (&f)[n]) {
+#pragma unroll
+  for (int c = 0; c < n; ++c) f[c] = from_sub_k<S>(x[c / S], c % S);
+}
+
+// The lane's columns t S + sub of a replicated row f[0..n) (0 past n).
+template <int S, int n, int NX>
+__device__ __forceinline__ void own(const double (&f)[n], int sub, double (&x)[NX]) {
+#pragma unroll
+  for (int t = 0; t < NX; ++t) {
+    double v = 0.0;
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+      const int c = t * S + k;
+      if (c < n) v = (sub == k) ? f[c] : v;
+    }
+    x[t] = v;
+  }
+}
+
+// sum_c x[c] v[c] over the row's n columns (x split, v replicated in LDS);
+// every lane of the row gets the sum.
+template <int S, int n, int NX>
+__device__ __forceinline__ double row_dot(const double (&x)[NX], const double* v, int sub) {
+  double acc = 0.0;
+#pragma unroll
+  for (int t = 0; t < NX; ++t) {
+    const int c = t * S + sub;
+    if ((t + 1) * S <= n) {
+      acc = fma(x[t], v[c], acc);
+    } else if (t * S < n) {
+      const double vv = v[c < n ? c : 0];
+      acc = fma(x[t], (c < n) ? vv : 0.0, acc);
+    }
+  }
+  return rowsum<S>(acc);
+}
+
+// ----------------------------------------------------------------- products
+// out[t] = sum_{k < K} x[k] T[k][t S + sub] over columns < n (T: stride LD)
+template <int S, int K, int n, int LD, int NO, int NXF>
+__device__ __forceinline__ void row_times_tile(const double (&x)[NXF], const double* T, int sub, double (&out)[NO]) {
+  zero(out);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+#pragma unroll
+    for (int t = 0; t < NO; ++t) {
+      const int c = t * S + sub;
+      if ((t + 1) * S <= n) {
+        out[t] = fma(x[k], T[k * LD + c], out[t]);
+      } else if (t * S < n) {
+        const double v = T[k * LD + (c < n ? c : 0)];
+        out[t] = fma(x[k], (c < n) ? v : 0.0, out[t]);
+      }
+    }
+    pin(out);
+  }
+}
+
+// out[t] = sum_{k < K} T1[k][r] T2[k][t S + sub] over columns < n
+template <int S, int K, int n, int LD, int NO>
+__device__ __forceinline__ void tile_t_times_tile(int r, const double* T1, const double* T2, int sub,
+                                                  double (&out)[NO]) {
+  zero(out);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const double a = T1[k * LD + r];
+#pragma unroll
+    for (int t = 0; t < NO; ++t) {
+      const int c = t * S + sub;
+      if ((t + 1) * S <= n) {
+        out[t] = fma(a, T2[k * LD + c], out[t]);
+      } else if (t * S < n) {
+        const double v = T2[k * LD + (c < n ? c : 0)];
+        out[t] = fma(a, (c < n) ? v : 0.0, out[t]);
+      }
+    }
+    pin(out);
+  }
+}
+
+// ---------------------------------------------------------- Householder LQ
+// LQ of NG independent NR x (B1 + B2) matrices whose columns come in two
+// blocks (B1, B2), each padded to a multiple of S.  Row r of matrix g sits in
+// lane row g NR + r, split over its S lanes.  On return that row holds row r
+// of the lower-triangular factor L (L L^T = M M^T, compact columns > r zero,
+// padding columns zero): the reflectors of LAPACK dgeqrf on M^T (dlarfg:
+// beta = -sign(alpha) norm, tau = (beta - alpha) / beta, v = [1, x / (alpha - beta)],
+// tau = 0 for a zero tail, no reflector for a length-1 tail).
+//
+// Per pivot step the pivot row broadcasts its raw tail through LDS; every
+// lane forms partial sums over its own columns, reduces them along its row
+// with DPP and derives (beta, tau, scale) itself, so no lane sits in a
+// divergent pivot section.
+//
+// The pivots [I0, I1) are applied; FINISH clears the strict upper part at the
+// end (so one factorisation can be run in pieces, with other work between).
+template <int S, int NR, int B1, int B2, int NG = 1, int I0 = 0, int I1 = 1 << 20, bool FINISH = true>
+__device__ __forceinline__ void lq(double (&a)[(padded<B1, S>() + padded<B2, S>()) / S], int row, int sub,
+                                   double* bc) {
+  static_assert(NG == 1 || NG == 2, "one or two matrices");
+  constexpr int P1 = padded<B1, S>();
+  constexpr int NP = P1 + padded<B2, S>();
+  constexpr int NX = NP / S;
+  constexpr int NC = B1 + B2;
+  constexpr int K = (NR < NC) ? NR : NC;
+  const int g = (NG == 2 && row >= NR) ? 1 : 0;
+  const int li = row - g * NR;
+  const bool valid = row < NG * NR;
+  double* brow = bc + g * NP;  // broadcast row of this lane's matrix
+#pragma unroll
+  for (int i = I0; i < ((I1 < K) ? I1 : K); ++i) {
     if (i == NC - 1) break;  // a length-1 tail: H = I
+    const int pc = (i < B1) ? i : P1 + (i - B1);  // padded column of pivot i
+    const int t0 = pc / S, k0 = pc % S;
     if (valid && li == i) {
 #pragma unroll
-      for (int j = i; j < NC; ++j) row[j] = a[j];
+      for (int t = t0; t < NX; ++t) brow[t * S + sub] = a[t];
     }
     wsync();
-    double p[NC];
+    const double alpha = brow[pc];
+    double p[NX];
 #pragma unroll
-    for (int j = i; j < NC; ++j) p[j] = row[j];
-    const double alpha = p[i];
-    // sum of squares of the pivot tail and this row's dot with it, in four
-    // independent chains each
+    for (int t = t0; t < NX; ++t) p[t] = brow[t * S + sub];
+    // the pivot's own group of S columns: only columns after pc are tail
+    const bool tl = sub > k0;
+    const double p0 = tl ? p[t0] : 0.0;
     double sq[4] = {0.0, 0.0, 0.0, 0.0};
     double dt[4] = {0.0, 0.0, 0.0, 0.0};
+    sq[0] = p0 * p0;
+    dt[0] = a[t0] * p0;
 #pragma unroll
-    for (int j = i + 1; j < NC; ++j) {
-      sq[(j - i - 1) & 3] = fma(p[j], p[j], sq[(j - i - 1) & 3]);
-      dt[(j - i - 1) & 3] = fma(a[j], p[j], dt[(j - i - 1) & 3]);
+    for (int t = t0 + 1; t < NX; ++t) {
+      sq[(t - t0) & 3] = fma(p[t], p[t], sq[(t - t0) & 3]);
+      dt[(t - t0) & 3] = fma(a[t], p[t], dt[(t - t0) & 3]);
     }
-    const double sig = (sq[0] + sq[1]) + (sq[2] + sq[3]);
-    const double dot = (dt[0] + dt[1]) + (dt[2] + dt[3]);
-    const double nrm = sqrt(fma(alpha, alpha, sig));
+    const double ai = from_sub_k<S>(a[t0], k0);  // this row's entry in the pivot column
+    const double sig = rowsum<S>((sq[0] + sq[1]) + (sq[2] + sq[3]));
+    const double dot = rowsum<S>((dt[0] + dt[1]) + (dt[2] + dt[3]));
+    double nrm, rnrm;
+    sqrt_rsqrt(fma(alpha, alpha, sig), nrm, rnrm);
     const double bc_ = -copysign(nrm, alpha);  // Fortran SIGN: honours the sign of -0.0
     const double den = alpha - bc_;
     const double sc_ = rcp(den);
-    const double tc_ = -den * rcp(bc_);
+    const double tc_ = den * copysign(rnrm, alpha);  // (beta - alpha) / beta
     const bool nz = sig != 0.0;
     const double beta = nz ? bc_ : alpha;
     const double scal = nz ? sc_ : 0.0;
     const double tau = nz ? tc_ : 0.0;
-    if (valid && li == i) a[i] = beta;
+    const bool owner = sub == k0;
+    if (valid && li == i && owner) a[t0] = beta;
     if (valid && li > i) {
-      const double tw = tau * fma(scal, dot, a[i]);
-      a[i] -= tw;
+      const double tw = tau * fma(scal, dot, ai);
       const double tws = tw * scal;
+      a[t0] = owner ? a[t0] - tw : fma(-tws, p0, a[t0]);
 #pragma unroll
-      for (int j = i + 1; j < NC; ++j) a[j] = fma(-tws, p[j], a[j]);
+      for (int t = t0 + 1; t < NX; ++t) a[t] = fma(-tws, p[t], a[t]);
     }
   }
+  if constexpr (FINISH) {
+    // strict upper part: compact column index of each padded column
 #pragma unroll
-  for (int j = 1; j < NC; ++j) a[j] = (valid && j > li) ? 0.0 : a[j];
+    for (int t = 0; t < NX; ++t) {
+      const int c = t * S + sub;
+      const int j = (c < P1) ? c : B1 + (c - P1);
+      a[t] = (valid && j > li) ? 0.0 : a[t];
+    }
+  }
   wsync();
 }
 
+// --------------------------------------------------- triangular solves (split)
+// y L^T = v (forward substitution; = L y = v), L lower n x n in LDS (stride
+// LD), y split over the row's lanes, holding v on entry.
+template <int S, int n, int LD, int NY>
+__device__ __forceinline__ void rsolve_lowerT_s(double (&y)[NY], const double* L, int sub) {
+  double rd[NY];
+#pragma unroll
+  for (int t = 0; t < NY; ++t) {
+    const int c = t * S + sub;
+    const int cc = (c < n) ? c : 0;
+    rd[t] = rcp(L[cc * LD + cc]);
+  }
+#pragma unroll
+  for (int j = 0; j < n; ++j) {
+    const int t = j / S, k = j % S;
+    const double yj = from_sub_k<S>(y[t] * rd[t], k);
+    if (sub == k) y[t] = yj;
+#pragma unroll
+    for (int tt = t; tt < NY; ++tt) {
+      const int i = tt * S + sub;
+      const bool upd = (tt > t || sub > k) && i < n;
+      const double l = L[(i < n ? i : 0) * LD + j];
+      y[tt] = upd ? fma(-yj, l, y[tt]) : y[tt];
+    }
+    pin1(y[t]);
+  }
+}
+
+// y L = v (backward substitution; = L^T y = v), L lower n x n in LDS.
+template <int S, int n, int LD, int NY>
+__device__ __forceinline__ void rsolve_lower_s(double (&y)[NY], const double* L, int sub) {
+  double rd[NY];
+#pragma unroll
+  for (int t = 0; t < NY; ++t) {
+    const int c = t * S + sub;
+    const int cc = (c < n) ? c : 0;
+    rd[t] = rcp(L[cc * LD + cc]);
+  }
+#pragma unroll
+  for (int j = n - 1; j >= 0; --j) {
+    const int t = j / S, k = j % S;
+    const double yj = from_sub_k<S>(y[t] * rd[t], k);
+    if (sub == k) y[t] = yj;
+#pragma unroll
+    for (int tt = 0; tt <= t; ++tt) {
+      const int i = tt * S + sub;
+      const bool upd = tt < t || sub < k;
+      const double l = L[j * LD + i];
+      y[tt] = upd ? fma(-yj, l, y[tt]) : y[tt];
+    }
+    pin1(y[t]);
+  }
+}
+
+// ---------------------------------------------- triangular solves (replicated)
+// Small systems (n <= d) solved by every lane on a full copy.
 // y L = v with L lower triangular (n x n in LDS, stride LD); y holds v on entry.
 template <int n, int LD, int NY>
 __device__ __forceinline__ void rsolve_lower(double (&y)[NY], const double* L) {
@@ -340,26 +603,31 @@ __device__ __forceinline__ void observation(const ModelArgs& a, int k, double (&
   }
 }
 
-// select row `lane` (< d) of a register matrix without dynamic indexing
+// select row `r` (< d) of a register matrix without dynamic indexing
 template <int d, int D>
-__device__ __forceinline__ void select_row(const double (&H)[d][D], int lane, double (&h)[D]) {
+__device__ __forceinline__ void select_row(const double (&H)[d][D], int r, double (&h)[D]) {
 #pragma unroll
   for (int j = 0; j < D; ++j) h[j] = H[0][j];
 #pragma unroll
   for (int i = 1; i < d; ++i) {
 #pragma unroll
-    for (int j = 0; j < D; ++j) h[j] = (lane == i) ? H[i][j] : h[j];
+    for (int j = 0; j < D; ++j) h[j] = (r == i) ? H[i][j] : h[j];
   }
 }
 
 // ----------------------------------------------------------- filter element
 // _get_element (pof/parallel_filtsmooth/filter.py:50-81) for element k with
-// prior (x0m, x0L) when first, else the zero prior.  Lane r < D returns row r.
-template <int D, int d>
-__device__ void filter_element(const Ctx<D>& cx, const ModelArgs& a, int k, bool first, double (&A)[D],
-                               double& b, double (&U)[D], double& eta, double (&Z)[D]) {
-  constexpr int LD = D + 1;
-  const int r = opaque(cx.lane);
+// prior (x0m, x0L) when first, else the zero prior.  Row r < D of A, U, Z is
+// returned split over the row's lanes; b and eta (entry r) in every lane of row r.
+template <int D, int d, int SS>
+__device__ void filter_element(const Ctx<D, SS>& cx, const ModelArgs& a, int k, bool first,
+                               double (&A)[Lay<D, SS>::NL], double& b, double (&U)[Lay<D, SS>::NL], double& eta,
+                               double (&Z)[Lay<D, SS>::NL]) {
+  using L = Lay<D, SS>;
+  constexpr int S = L::S, LD = L::LD, NL = L::NL;
+  constexpr int NLd = padded<d, S>() / S;
+  const int r = cx.row, s = cx.sub;
+  const int rr = (r < D) ? r : 0;
   double H[d][D];
   double cv[d];
   observation<D, d>(a, k, H, cv);
@@ -373,96 +641,102 @@ __device__ void filter_element(const Ctx<D>& cx, const ModelArgs& a, int k, bool
   double m1r = 0.0;
   if (first) {
     // m1 = F m0, N1 = tria([F L0 | LQ])
-    double row[2 * D];
-    if (r < D) {
+    double row[2 * NL];
+    {
       double fr[D];
-      load_row(fr, cx.sF + r * LD);
-      double fl[D];
+      load_row(fr, cx.sF + rr * LD);
+      double fl[NL];
       zero(fl);
 #pragma unroll
       for (int k2 = 0; k2 < D; ++k2) {
 #pragma unroll
-        for (int c = 0; c < D; ++c) fl[c] = fma(fr[k2], a.x0L[k2 * D + c], fl[c]);
+        for (int t = 0; t < NL; ++t) {
+          const int c = t * S + s;
+          const double v = a.x0L[k2 * D + ((c < D) ? c : 0)];
+          fl[t] = fma(fr[k2], (c < D) ? v : 0.0, fl[t]);
+        }
         m1r = fma(fr[k2], a.x0m[k2], m1r);
         pin(fl);
       }
+      double lq_r[NL];
+      load_dist<S, D>(lq_r, cx.sLQ + rr * LD, s);
 #pragma unroll
-      for (int c = 0; c < D; ++c) {
-        row[c] = fl[c];
-        row[D + c] = cx.sLQ[r * LD + c];
+      for (int t = 0; t < NL; ++t) {
+        row[t] = (r < D) ? fl[t] : 0.0;
+        row[NL + t] = (r < D) ? lq_r[t] : 0.0;
       }
-    } else {
-      zero(row);
     }
-    lq<D, 2 * D>(row, r, cx.bc());
+    lq<S, D, D, D>(row, r, s, cx.bc());
     if (r < D) {
 #pragma unroll
-      for (int c = 0; c < D; ++c) T0[r * LD + c] = row[c];
+      for (int t = 0; t < NL; ++t) T0[r * LD + t * S + s] = row[t];
       V0[r] = m1r;
     }
     wsync();
     N1 = T0;
   }
 
-  // Psi_ = [[H N1, R], [N1, 0]]  (R = 0), rows in lanes 0..d+D-1
-  double psi[D + d];
+  // Psi_ = [[H N1, R], [N1, 0]]  (R = 0): rows 0..d-1 and d..d+D-1, column blocks (D, d)
+  double psi[NL + NLd];
   {
     double h[D];
     select_row<d, D>(H, r, h);
-    if (r < d) {
-      double hn[D];
-      row_times_tile<D, D, LD>(h, N1, hn);
+    double hn[NL];
+    row_times_tile<S, D, D, LD>(h, N1, s, hn);
+    double nr[NL];
+    const int rn = (r >= d && r < d + D) ? r - d : 0;
+    load_dist<S, D>(nr, N1 + rn * LD, s);
 #pragma unroll
-      for (int c = 0; c < D; ++c) psi[c] = hn[c];
-    } else if (r < d + D) {
+    for (int t = 0; t < NL; ++t) psi[t] = (r < d) ? hn[t] : ((r < d + D) ? nr[t] : 0.0);
 #pragma unroll
-      for (int c = 0; c < D; ++c) psi[c] = N1[(r - d) * LD + c];
-    } else {
-#pragma unroll
-      for (int c = 0; c < D; ++c) psi[c] = 0.0;
-    }
-#pragma unroll
-    for (int c = 0; c < d; ++c) psi[D + c] = 0.0;
+    for (int t = 0; t < NLd; ++t) psi[NL + t] = 0.0;
   }
-  lq<d + D, D + d>(psi, r, cx.bc());
+  lq<S, d + D, D, d>(psi, r, s, cx.bc());
 
-  // Psi11 -> T1
-  if (r < d) {
-#pragma unroll
-    for (int c = 0; c < d; ++c) T1[r * LD + c] = psi[c];
-  }
+  // Psi11 -> T1 (compact columns < d lie in the first block)
+  if (r < d) store_dist<S, d>(T1 + r * LD, psi, s);
   wsync();
-  // K row = Psi21 row Psi11^-1 (lanes d..d+D-1), then move K and U rows to lane r - d
-  if (r >= d && r < d + D) {
+  // K row = Psi21 row Psi11^-1 -> T3, U row = Psi22 -> T2 (rows d..d+D-1 move to r - d)
+  {
     double kk[d];
+    gather<S, d>(psi, kk);
+    if (r >= d && r < d + D) {
+      rsolve_lower<d, LD>(kk, T1);
+      if (s == 0) {
 #pragma unroll
-    for (int c = 0; c < d; ++c) kk[c] = psi[c];
-    rsolve_lower<d, LD>(kk, T1);
+        for (int c = 0; c < d; ++c) T3[(r - d) * LD + c] = kk[c];
+      }
+      // compact columns d .. D + d - 1 of the row are U's columns 0 .. D - 1
+      constexpr int P1 = L::DP;
 #pragma unroll
-    for (int c = 0; c < d; ++c) T3[(r - d) * LD + c] = kk[c];
-#pragma unroll
-    for (int c = 0; c < D; ++c) T2[(r - d) * LD + c] = psi[d + c];
+      for (int t = 0; t < NL + NLd; ++t) {
+        const int c = t * S + s;
+        const int j = (c < P1) ? c : D + (c - P1);
+        const bool in = (c < P1) ? (c < D) : (c - P1 < d);
+        if (in && j >= d) T2[(r - d) * LD + (j - d)] = psi[t];
+      }
+    }
   }
   wsync();
 
-  // HF = H F (every lane), hm1 = H m1 + c
-  double HF[d][D];
+  // HF = H F (split columns), hm1 = H m1 + c (every lane)
+  double HF[d][NL];
   double hm1[d];
 #pragma unroll
   for (int i = 0; i < d; ++i) {
 #pragma unroll
-    for (int c = 0; c < D; ++c) HF[i][c] = 0.0;
+    for (int t = 0; t < NL; ++t) HF[i][t] = 0.0;
     hm1[i] = cv[i];
   }
 #pragma unroll
   for (int j = 0; j < D; ++j) {
-    double fj[D];
-    load_row(fj, cx.sF + j * LD);
+    double fj[NL];
+    load_dist<S, D>(fj, cx.sF + j * LD, s);
     const double m1j = first ? V0[j] : 0.0;
 #pragma unroll
     for (int i = 0; i < d; ++i) {
 #pragma unroll
-      for (int c = 0; c < D; ++c) HF[i][c] = fma(H[i][j], fj[c], HF[i][c]);
+      for (int t = 0; t < NL; ++t) HF[i][t] = fma(H[i][j], fj[t], HF[i][t]);
       hm1[i] = fma(H[i][j], m1j, hm1[i]);
       pin(HF[i]);
     }
@@ -474,22 +748,18 @@ __device__ void filter_element(const Ctx<D>& cx, const ModelArgs& a, int k, bool
   for (int i = 0; i < d; ++i) wv[i] = cv[i];
   lsolve_lower<d, LD>(wv, T1);
 
-  const int rr = (r < D) ? r : 0;
   {
     double kr[d];
 #pragma unroll
     for (int i = 0; i < d; ++i) kr[i] = T3[rr * LD + i];
-    load_row(U, T2 + rr * LD);
+    load_dist<S, D>(U, T2 + rr * LD, s);
     // A = F - K H F ; b = m1 - K (H m1 + c)
-    double fr[D];
-    load_row(fr, cx.sF + rr * LD);
+    load_dist<S, D>(A, cx.sF + rr * LD, s);
     double bb = m1r;
-#pragma unroll
-    for (int c = 0; c < D; ++c) A[c] = fr[c];
 #pragma unroll
     for (int i = 0; i < d; ++i) {
 #pragma unroll
-      for (int c = 0; c < D; ++c) A[c] = fma(-kr[i], HF[i][c], A[c]);
+      for (int t = 0; t < NL; ++t) A[t] = fma(-kr[i], HF[i][t], A[t]);
       bb = fma(-kr[i], hm1[i], bb);
     }
     b = bb;
@@ -512,13 +782,15 @@ __device__ void filter_element(const Ctx<D>& cx, const ModelArgs& a, int k, bool
     eta = e;
   }
   if constexpr (D > d) {
-#pragma unroll
-    for (int c = 0; c < D; ++c) Z[c] = (c < d) ? z[(c < d) ? c : 0] : 0.0;
+    // Z = [Zr | 0]
+    own<S, d>(z, s, Z);
   } else {
-    // nx <= ny: Z = tria(Z) with Z of shape D x d
-    lq<D, d>(z, r, cx.bc());
+    // nx <= ny: Z = tria(Zr) with Zr of shape D x d (D == d here)
+    double zd[NLd];
+    own<S, d>(z, s, zd);
+    lq<S, D, d, 0>(zd, r, s, cx.bc());
 #pragma unroll
-    for (int c = 0; c < D; ++c) Z[c] = z[c];
+    for (int t = 0; t < NL; ++t) Z[t] = zd[t];
   }
   wsync();
 }
@@ -528,15 +800,16 @@ __device__ void filter_element(const Ctx<D>& cx, const ModelArgs& a, int k, bool
 // earlier.  Reads e1 at index i1 of `x1`, e2 at index i2 of `x2`, writes index
 // o of `y`.  If `state_only`, e1 is a prefix state (b1, U1) read from
 // (s1.m, s1.L) and only (b, U) of the result are produced, into (so.m, so.L).
-template <int D, bool state_only>
-__device__ void filter_combine(const Ctx<D>& cx, const FiltArrays& x1, const StateArrays& s1, size_t i1,
+template <int D, bool state_only, int SS>
+__device__ void filter_combine(const Ctx<D, SS>& cx, const FiltArrays& x1, const StateArrays& s1, size_t i1,
                                const FiltArrays& x2, size_t i2, const FiltArrays& y, const StateArrays& so,
                                size_t o) {
-  constexpr int LD = D + 1;
-  const int r = opaque(cx.lane);
+  using L = Lay<D, SS>;
+  constexpr int S = L::S, LD = L::LD, NL = L::NL;
+  const int r = cx.row, s = cx.sub;
   const bool act = r < D;
   const int rr = act ? r : 0;  // clamped row for branch-free loads
-  // the full combine factors U (lanes 0..D-1) and Z (lanes D..2D-1) together
+  // the full combine factors U (rows 0..D-1) and Z (rows D..2D-1) together
   const bool zl = !state_only && r >= D && r < 2 * D;
   const int rz = zl ? r - D : 0;
   double* TU1 = cx.tile(0);
@@ -558,74 +831,87 @@ __device__ void filter_combine(const Ctx<D>& cx, const FiltArrays& x1, const Sta
   const double b1r = b1p[i1 * D + rr];
   const double eta2r = x2.eta[i2 * D + rr];
   const double b2r = x2.b[i2 * D + rr];
-  double a2[D], u2[D], z1[D];
+  double a2[NL], u2[NL], z1[NL];
   double eta1r = 0.0;
   {
-    double u1[D], z2[D], a1[D];
-    load_row(u1, U1p + (i1 * D + rr) * D);
-    load_row(z2, x2.Z + (i2 * D + rr) * D);
-    load_row(a2, x2.A + (i2 * D + rr) * D);
-    load_row(u2, x2.U + (i2 * D + rr) * D);
+    double u1[NL], z2[NL], a1[NL];
+    load_dist<S, D>(u1, U1p + (i1 * D + rr) * D, s);
+    load_dist<S, D>(z2, x2.Z + (i2 * D + rr) * D, s);
     if constexpr (!state_only) {
-      load_row(a1, x1.A + (i1 * D + rr) * D);
-      load_row(z1, x1.Z + (i1 * D + rz) * D);  // Z1 rows feed the lanes D..2D-1
+      load_dist<S, D>(a1, x1.A + (i1 * D + rr) * D, s);
       eta1r = x1.eta[i1 * D + rr];
     }
     if (act) {
 #pragma unroll
-      for (int c = 0; c < D; ++c) {
-        TU1[r * LD + c] = u1[c];
-        TZ2[r * LD + c] = z2[c];
-        if constexpr (!state_only) TA1[r * LD + c] = a1[c];
+      for (int t = 0; t < NL; ++t) {
+        TU1[r * LD + t * S + s] = u1[t];
+        TZ2[r * LD + t * S + s] = z2[t];
+        if constexpr (!state_only) TA1[r * LD + t * S + s] = a1[t];
       }
       Vb1[r] = b1r;
       Veta2[r] = eta2r;
     }
   }
   wsync();
-  // Xi_ = [[U1^T Z2, I], [Z2, 0]]
-  double xr[2 * D];
-  if (r < D) {
-    double t[D];
-    tile_t_times_tile<D, D, LD>(r, TU1, TZ2, t);
+  // Xi_ = [[U1^T Z2, I], [Z2, 0]]: rows 0..2D-1, column blocks (D, D)
+  double xr[2 * NL];
+  {
+    double t1[NL];
+    tile_t_times_tile<S, D, D, LD>(rr, TU1, TZ2, s, t1);
+    double zr[NL];
+    const int r2 = (r >= D && r < 2 * D) ? r - D : 0;
+    load_dist<S, D>(zr, TZ2 + r2 * LD, s);
 #pragma unroll
-    for (int c = 0; c < D; ++c) {
-      xr[c] = t[c];
-      xr[D + c] = (c == r) ? 1.0 : 0.0;
+    for (int t = 0; t < NL; ++t) {
+      xr[t] = act ? t1[t] : ((r < 2 * D) ? zr[t] : 0.0);
+      xr[NL + t] = (act && t * S + s == r) ? 1.0 : 0.0;
     }
-  } else if (r < 2 * D) {
-#pragma unroll
-    for (int c = 0; c < D; ++c) {
-      xr[c] = TZ2[(r - D) * LD + c];
-      xr[D + c] = 0.0;
-    }
-  } else {
-    zero(xr);
   }
-  lq<2 * D, 2 * D>(xr, r, cx.bc());
-  if (r < D) {
+  // the rows of A2, U2 (and Z1) are needed only after this factorisation: their
+  // loads are issued half way through it, where fewer registers are live and
+  // the remaining pivot steps hide the memory latency
+  lq<S, 2 * D, D, D, 1, 0, D, false>(xr, r, s, cx.bc());
+  load_dist<S, D>(a2, x2.A + (i2 * D + rr) * D, s);
+  load_dist<S, D>(u2, x2.U + (i2 * D + rr) * D, s);
+  if constexpr (!state_only) {
+    load_dist<S, D>(z1, x1.Z + (i1 * D + rz) * D, s);  // Z1 rows feed the rows D..2D-1
+  } else {
+    zero(z1);
+  }
+  lq<S, 2 * D, D, D, 1, D>(xr, r, s, cx.bc());
+  if (act) {
 #pragma unroll
-    for (int c = 0; c < D; ++c) TX11[r * LD + c] = xr[c];
+    for (int t = 0; t < NL; ++t) TX11[r * LD + t * S + s] = xr[t];
   } else if (r < 2 * D) {
 #pragma unroll
-    for (int c = 0; c < D; ++c) {
-      TX21[(r - D) * LD + c] = xr[c];
-      if constexpr (!state_only) TX22[(r - D) * LD + c] = xr[D + c];  // tile 4 exists in the full layout only
+    for (int t = 0; t < NL; ++t) {
+      TX21[(r - D) * LD + t * S + s] = xr[t];
+      if constexpr (!state_only) TX22[(r - D) * LD + t * S + s] = xr[NL + t];  // full layout only
     }
   }
   wsync();
 
   // MT = A2 U1 Xi11^-T ; W = A2 - MT Xi21^T
-  double mt[D], wr[D];
-  row_times_tile<D, D, LD>(a2, TU1, mt);
-  rsolve_lowerT<D, LD>(mt, TX11);
+  double mt[NL], wr[NL];
+  {
+    double a2f[D];
+    gather<S, D>(a2, a2f);
+    row_times_tile<S, D, D, LD>(a2f, TU1, s, mt);
+  }
+  rsolve_lowerT_s<S, D, LD>(mt, TX11, s);
+  {
+    double mtf[D];
+    gather<S, D>(mt, mtf);
 #pragma unroll
-  for (int c = 0; c < D; ++c) {
-    double acc = a2[c];
+    for (int t = 0; t < NL; ++t) {
+      const int c = t * S + s;
+      const int cc = (c < D) ? c : 0;
+      double acc = a2[t];
 #pragma unroll
-    for (int j = 0; j < D; ++j) acc = fma(-mt[j], TX21[c * LD + j], acc);
-    wr[c] = acc;
-    pin1(wr[c]);
+      for (int j = 0; j < D; ++j) acc = fma(-mtf[j], TX21[cc * LD + j], acc);
+      wr[t] = (c < D) ? acc : 0.0;
+      pin1(wr[t]);
+    }
   }
 
   // b = W (b1 + U1 U1^T eta2) + b2   (all global stores are issued at the end:
@@ -642,19 +928,21 @@ __device__ void filter_combine(const Ctx<D>& cx, const FiltArrays& x1, const Sta
     for (int j = 0; j < D; ++j) sv = fma(TU1[rr * LD + j], Vu[j], sv);
     if (act) Vs[r] = sv;
     wsync();
-    bb = b2r;
-#pragma unroll
-    for (int k = 0; k < D; ++k) bb = fma(wr[k], Vs[k], bb);
+    bb = b2r + row_dot<S, D>(wr, Vs, s);
   }
   // A = W A1 and eta = A1^T (I - Xi21 Xi11^-1 U1^T)(eta2 - Z2 Z2^T b1) + eta1
-  double ar[state_only ? 1 : D];
+  double ar[state_only ? 1 : NL];
   double eo = 0.0;
   if constexpr (!state_only) {
     double* Vt = cx.vec(4);
     double* Vs2 = cx.vec(5);
     double* Vu2 = cx.vec(6);
     double* Vs3 = cx.vec(7);
-    row_times_tile<D, D, LD>(wr, TA1, ar);
+    {
+      double wf[D];
+      gather<S, D>(wr, wf);
+      row_times_tile<S, D, D, LD>(wf, TA1, s, ar);
+    }
     {
       double t = 0.0;
 #pragma unroll
@@ -674,13 +962,16 @@ __device__ void filter_combine(const Ctx<D>& cx, const FiltArrays& x1, const Sta
       if (act) Vu2[r] = uu;
     }
     wsync();
-    double w2[D];
-    load_row(w2, Vu2);
-    lsolve_lower<D, LD>(w2, TX11);
+    double w2[NL];
+    load_dist<S, D>(w2, Vu2, s);
+    rsolve_lowerT_s<S, D, LD>(w2, TX11, s);  // Xi11^-1 (U1^T s2)
     {
-      double v2 = 0.0;
+      double tx[NL];
+      load_dist<S, D>(tx, TX21 + rr * LD, s);
+      double acc = 0.0;
 #pragma unroll
-      for (int j = 0; j < D; ++j) v2 = fma(TX21[rr * LD + j], w2[j], v2);
+      for (int t = 0; t < NL; ++t) acc = fma(tx[t], w2[t], acc);
+      const double v2 = rowsum<S>(acc);
       if (act) Vs3[r] = s2 - v2;
     }
     wsync();
@@ -688,37 +979,32 @@ __device__ void filter_combine(const Ctx<D>& cx, const FiltArrays& x1, const Sta
 #pragma unroll
     for (int k = 0; k < D; ++k) eo = fma(TA1[k * LD + rr], Vs3[k], eo);
   }
-  // U = tria([MT | U2]) in lanes 0..D-1 and, for the full combine,
-  // Z = tria([A1^T Xi22 | Z1]) in lanes D..2D-1, as one paired LQ
-  double jr[2 * D];
-  if (act) {
-#pragma unroll
-    for (int c = 0; c < D; ++c) {
-      jr[c] = mt[c];
-      jr[D + c] = u2[c];
-    }
-  } else if (zl) {
-    double t[D];
-    tile_t_times_tile<D, D, LD>(rz, TA1, TX22, t);
-#pragma unroll
-    for (int c = 0; c < D; ++c) {
-      jr[c] = t[c];
-      jr[D + c] = z1[c];
-    }
-  } else {
-    zero(jr);
-  }
-  lq<D, 2 * D, state_only ? 1 : 2>(jr, r, cx.bc());
-  const double(&lo)[D] = *reinterpret_cast<const double(*)[D]>(jr);
-  if (act) {
-    (state_only ? so.m : y.b)[o * D + r] = bb;
-    store_row((state_only ? so.L : y.U) + (o * D + r) * D, lo);
+  // U = tria([MT | U2]) in rows 0..D-1 and, for the full combine,
+  // Z = tria([A1^T Xi22 | Z1]) in rows D..2D-1, as one paired LQ
+  double jr[2 * NL];
+  {
+    double tz[NL];
     if constexpr (!state_only) {
-      store_row(y.A + (o * D + r) * D, ar);
-      y.eta[o * D + r] = eo;
+      tile_t_times_tile<S, D, D, LD>(rz, TA1, TX22, s, tz);
+    } else {
+      zero(tz);
+    }
+#pragma unroll
+    for (int t = 0; t < NL; ++t) {
+      jr[t] = act ? mt[t] : (zl ? tz[t] : 0.0);
+      jr[NL + t] = act ? u2[t] : (zl ? z1[t] : 0.0);
+    }
+  }
+  lq<S, D, D, D, state_only ? 1 : 2>(jr, r, s, cx.bc());
+  if (act) {
+    if (s == 0) (state_only ? so.m : y.b)[o * D + r] = bb;
+    store_dist<S, D>((state_only ? so.L : y.U) + (o * D + r) * D, jr, s);
+    if constexpr (!state_only) {
+      store_dist<S, D>(y.A + (o * D + r) * D, ar, s);
+      if (s == 0) y.eta[o * D + r] = eo;
     }
   } else if (zl) {
-    store_row(y.Z + (o * D + rz) * D, lo);
+    store_dist<S, D>(y.Z + (o * D + rz) * D, jr, s);
   }
   wsync();
 }
@@ -728,12 +1014,13 @@ __device__ void filter_combine(const Ctx<D>& cx, const FiltArrays& x1, const Sta
 // aggregate later in time.  full: (g, E, D) = (E2 g1 + g2, E2 E1,
 // tria([E2 D1 | D2])).  state_only: (m, L) = (E2 m1 + g2, tria([E2 L1 | D2])).
 // with_chol = false skips every D/L term (means-only iterations).
-template <int D, bool state_only>
-__device__ void smoother_combine(const Ctx<D>& cx, const SmoArrays& x1, const StateArrays& s1, size_t i1,
+template <int D, bool state_only, int SS>
+__device__ void smoother_combine(const Ctx<D, SS>& cx, const SmoArrays& x1, const StateArrays& s1, size_t i1,
                                  const SmoArrays& x2, size_t i2, const SmoArrays& y, const StateArrays& so,
                                  size_t o, bool with_chol) {
-  constexpr int LD = D + 1;
-  const int r = opaque(cx.lane);
+  using L = Lay<D, SS>;
+  constexpr int S = L::S, LD = L::LD, NL = L::NL;
+  const int r = cx.row, s = cx.sub;
   const bool act = r < D;
   const int rr = act ? r : 0;
   double* TE1 = cx.tile(0);
@@ -742,56 +1029,50 @@ __device__ void smoother_combine(const Ctx<D>& cx, const SmoArrays& x1, const St
   const double* g1p = state_only ? s1.m : x1.g;
   const double* D1p = state_only ? s1.L : x1.Dm;
   // all global reads first (one memory latency)
-  double e2[D], d2[D];
+  double e2[NL], d2[NL];
   const double g2r = x2.g[i2 * D + rr];
-  load_row(e2, x2.E + (i2 * D + rr) * D);
-  if (with_chol) load_row(d2, x2.Dm + (i2 * D + rr) * D);
+  load_dist<S, D>(e2, x2.E + (i2 * D + rr) * D, s);
+  if (with_chol) load_dist<S, D>(d2, x2.Dm + (i2 * D + rr) * D, s);
   {
     const double g1 = g1p[i1 * D + rr];
-    double e1[D], d1[D];
-    if (!state_only) load_row(e1, x1.E + (i1 * D + rr) * D);
-    if (with_chol) load_row(d1, D1p + (i1 * D + rr) * D);
+    double e1[NL], d1[NL];
+    if (!state_only) load_dist<S, D>(e1, x1.E + (i1 * D + rr) * D, s);
+    if (with_chol) load_dist<S, D>(d1, D1p + (i1 * D + rr) * D, s);
     if (act) {
       Vg1[r] = g1;
       if (!state_only) {
 #pragma unroll
-        for (int c = 0; c < D; ++c) TE1[r * LD + c] = e1[c];
+        for (int t = 0; t < NL; ++t) TE1[r * LD + t * S + s] = e1[t];
       }
       if (with_chol) {
 #pragma unroll
-        for (int c = 0; c < D; ++c) TD1[r * LD + c] = d1[c];
+        for (int t = 0; t < NL; ++t) TD1[r * LD + t * S + s] = d1[t];
       }
     }
   }
   wsync();
   // results stay in registers; every global store is issued at the end so
   // that no wait on outstanding memory operations lands inside the LQ below
-  double gg = g2r;
-#pragma unroll
-  for (int k = 0; k < D; ++k) gg = fma(e2[k], Vg1[k], gg);
-  double er[state_only ? 1 : D];
-  if constexpr (!state_only) row_times_tile<D, D, LD>(e2, TE1, er);
-  double dr[2 * D];
+  const double gg = g2r + row_dot<S, D>(e2, Vg1, s);
+  double e2f[D];
+  gather<S, D>(e2, e2f);
+  double er[state_only ? 1 : NL];
+  if constexpr (!state_only) row_times_tile<S, D, D, LD>(e2f, TE1, s, er);
+  double dr[2 * NL];
   if (with_chol) {
-    {
-      double t[D];
-      row_times_tile<D, D, LD>(e2, TD1, t);
+    double t1[NL];
+    row_times_tile<S, D, D, LD>(e2f, TD1, s, t1);
 #pragma unroll
-      for (int c = 0; c < D; ++c) {
-        dr[c] = t[c];
-        dr[D + c] = d2[c];
-      }
+    for (int t = 0; t < NL; ++t) {
+      dr[t] = t1[t];
+      dr[NL + t] = d2[t];
     }
-    lq<D, 2 * D>(dr, r, cx.bc());
+    lq<S, D, D, D>(dr, r, s, cx.bc());
   }
   if (act) {
-    (state_only ? so.m : y.g)[o * D + r] = gg;
-    if constexpr (!state_only) store_row(y.E + (o * D + r) * D, er);
-    if (with_chol) {
-      double* Do = state_only ? so.L : y.Dm;
-#pragma unroll
-      for (int c = 0; c < D; ++c) Do[(o * D + r) * D + c] = dr[c];
-    }
+    if (s == 0) (state_only ? so.m : y.g)[o * D + r] = gg;
+    if constexpr (!state_only) store_dist<S, D>(y.E + (o * D + r) * D, er, s);
+    if (with_chol) store_dist<S, D>((state_only ? so.L : y.Dm) + (o * D + r) * D, dr, s);
   }
   wsync();
 }

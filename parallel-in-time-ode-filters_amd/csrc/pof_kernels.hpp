@@ -97,22 +97,25 @@ __device__ __forceinline__ ModelArgs view(const GroupArgs& g, int j) {
 template <int D, int d>
 __global__ __launch_bounds__((Geo<D, kLdsLeaf>::BLOCK)) void k_filter_leaf(GroupArgs g, LaunchMap lm) {
   extern __shared__ double smem[];
-  Ctx<D> cx = make_ctx<D, kLdsLeaf>(g.F, g.LQ, g.N1z, smem);
+  auto cx = make_ctx<D, kLdsLeaf>(g.F, g.LQ, g.N1z, smem);
   int j, k;
   if (!locate(lm, cx.wid, j, k)) return;
   const ModelArgs a = view<D>(g, j);
   const FiltArrays out = at(g.fel, g.base[j], D);
-  double A[D], U[D], Z[D];
+  constexpr int S = decltype(cx)::S, NL = Lay<D, S>::NL;
+  double A[NL], U[NL], Z[NL];
   double b, eta;
   filter_element<D, d>(cx, a, k, k == 0, A, b, U, eta, Z);
-  const int r = cx.lane;
+  const int r = cx.row;
   if (r < D) {
     const size_t o = (size_t)k * D + r;
-    store_row(out.A + o * D, A);
-    store_row(out.U + o * D, U);
-    store_row(out.Z + o * D, Z);
-    out.b[o] = b;
-    out.eta[o] = eta;
+    store_dist<S, D>(out.A + o * D, A, cx.sub);
+    store_dist<S, D>(out.U + o * D, U, cx.sub);
+    store_dist<S, D>(out.Z + o * D, Z, cx.sub);
+    if (cx.sub == 0) {
+      out.b[o] = b;
+      out.eta[o] = eta;
+    }
   }
 }
 
@@ -120,7 +123,7 @@ __global__ __launch_bounds__((Geo<D, kLdsLeaf>::BLOCK)) void k_filter_leaf(Group
 template <int D>
 __global__ __launch_bounds__((Geo<D, kLdsFull>::BLOCK)) void k_filter_up(GroupArgs g, LaunchMap lm, int l) {
   extern __shared__ double smem[];
-  Ctx<D> cx = make_ctx<D, kLdsFull>(g.F, g.LQ, g.N1z, smem);
+  auto cx = make_ctx<D, kLdsFull>(g.F, g.LQ, g.N1z, smem);
   int j, i;
   if (!locate(lm, cx.wid, j, i)) return;
   const long long N = g.Ns[j], base = g.base[j];
@@ -136,7 +139,7 @@ __global__ __launch_bounds__((Geo<D, kLdsFull>::BLOCK)) void k_filter_up(GroupAr
 template <int D>
 __global__ __launch_bounds__((Geo<D, kLdsState>::BLOCK)) void k_filter_down(GroupArgs g, LaunchMap lm, int l) {
   extern __shared__ double smem[];
-  Ctx<D> cx = make_ctx<D, kLdsState>(g.F, g.LQ, g.N1z, smem);
+  auto cx = make_ctx<D, kLdsState>(g.F, g.LQ, g.N1z, smem);
   int j, i;
   if (!locate(lm, cx.wid, j, i)) return;
   const long long N = g.Ns[j], base = g.base[j];
@@ -144,13 +147,15 @@ __global__ __launch_bounds__((Geo<D, kLdsState>::BLOCK)) void k_filter_down(Grou
   const FiltArrays el = at(g.fel, base + lev_off(N, l), D);
   const StateArrays p = at(g.fpre, base + lev_off(N, l), D);
   const StateArrays up = (l + 1 < n_levels(N)) ? at(g.fpre, base + lev_off(N, l + 1), D) : StateArrays{};
+  constexpr int S = decltype(cx)::S, NL = Lay<D, S>::NL;
   const int j0 = 2 * i;
-  const int r = cx.lane;
+  const int r = cx.row, sb = cx.sub;
   if (j0 == 0) {
     if (r < D) {
-      p.m[r] = el.b[r];
-#pragma unroll
-      for (int c = 0; c < D; ++c) p.L[r * D + c] = el.U[r * D + c];
+      double u[NL];
+      load_dist<S, D>(u, el.U + r * D, sb);
+      store_dist<S, D>(p.L + r * D, u, sb);
+      if (sb == 0) p.m[r] = el.b[r];
     }
   } else {
     FiltArrays none{};
@@ -158,9 +163,10 @@ __global__ __launch_bounds__((Geo<D, kLdsState>::BLOCK)) void k_filter_down(Grou
   }
   const int j1 = j0 + 1;
   if (j1 < n && r < D) {
-    p.m[(size_t)j1 * D + r] = up.m[(size_t)i * D + r];
-#pragma unroll
-    for (int c = 0; c < D; ++c) p.L[((size_t)j1 * D + r) * D + c] = up.L[((size_t)i * D + r) * D + c];
+    double u[NL];
+    load_dist<S, D>(u, up.L + ((size_t)i * D + r) * D, sb);
+    store_dist<S, D>(p.L + ((size_t)j1 * D + r) * D, u, sb);
+    if (sb == 0) p.m[(size_t)j1 * D + r] = up.m[(size_t)i * D + r];
   }
 }
 
@@ -172,8 +178,10 @@ __global__ __launch_bounds__((Geo<D, kLdsState>::BLOCK)) void k_filter_down(Grou
 template <int D, int d>
 __global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK)) void k_filter_post(GroupArgs g, LaunchMap lm) {
   extern __shared__ double smem[];
-  Ctx<D> cx = make_ctx<D, kLdsPost>(g.F, g.LQ, g.N1z, smem);
-  constexpr int LD = D + 1;
+  auto cx = make_ctx<D, kLdsPost>(g.F, g.LQ, g.N1z, smem);
+  using L = Lay<D, decltype(cx)::S>;
+  constexpr int S = L::S, LD = L::LD, NL = L::NL;
+  constexpr int NLd = padded<d, S>() / S;
   int j, s;
   if (!locate(lm, cx.wid, j, s)) return;
   const ModelArgs a = view<D>(g, j);
@@ -183,19 +191,20 @@ __global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK)) void k_filter_post(Group
   double* t_ssq = g.t_ssq + base;
   double* t_nll = g.t_nll + base;
   const int Ns = a.Ns;
-  const int r = cx.lane;
+  const int r = cx.row, sb = cx.sub;
   const bool act = r < D;
+  const int rr = act ? r : 0;
   const double* mp = (s == 0) ? a.x0m : pre0.m + (size_t)(s - 1) * D;
   const double* Lp = (s == 0) ? a.x0L : pre0.L + (size_t)(s - 1) * D * D;
   const size_t ro = (size_t)(Ns - s);
   if (s == Ns) {
     if (act) {
-      sel.g[ro * D + r] = mp[r];
-#pragma unroll
-      for (int c = 0; c < D; ++c) {
-        sel.E[(ro * D + r) * D + c] = 0.0;
-        sel.Dm[(ro * D + r) * D + c] = Lp[r * D + c];
-      }
+      double lr[NL], zr[NL];
+      load_dist<S, D>(lr, Lp + r * D, sb);
+      zero(zr);
+      store_dist<S, D>(sel.Dm + (ro * D + r) * D, lr, sb);
+      store_dist<S, D>(sel.E + (ro * D + r) * D, zr, sb);
+      if (sb == 0) sel.g[ro * D + r] = mp[r];
     }
     return;
   }
@@ -203,6 +212,7 @@ __global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK)) void k_filter_post(Group
   double* TP11 = cx.tile(1);
   double* TE = cx.tile(2);
   double* TD = cx.tile(3);
+  double* TS = cx.tile(4);
   double* Vm = cx.vec(0);
   double* Vfm = cx.vec(1);
   // global reads first: the filtered state and the linearisation point of obs model s
@@ -210,88 +220,81 @@ __global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK)) void k_filter_post(Group
   double cv[d];
   observation<D, d>(a, s, H, cv);
   {
-    const int rr = act ? r : 0;
-    double lr[D];
-    load_row(lr, Lp + rr * D);
+    double lr[NL];
+    load_dist<S, D>(lr, Lp + rr * D, sb);
     const double mr = mp[rr];
     if (act) {
 #pragma unroll
-      for (int c = 0; c < D; ++c) TL[r * LD + c] = lr[c];
+      for (int t = 0; t < NL; ++t) TL[r * LD + t * S + sb] = lr[t];
       Vm[r] = mr;
     }
   }
   wsync();
-  // Phi_ = [[F L, LQ], [L, 0]]
-  double ph[2 * D];
-  double fm = 0.0;
-  if (r < D) {
+  // Phi_ = [[F L, LQ], [L, 0]]: rows 0..2D-1, column blocks (D, D)
+  double ph[2 * NL];
+  {
     double fr[D];
-    load_row(fr, cx.sF + r * LD);
-    double t[D];
-    row_times_tile<D, D, LD>(fr, TL, t);
+    load_row(fr, cx.sF + rr * LD);
+    double t1[NL];
+    row_times_tile<S, D, D, LD>(fr, TL, sb, t1);
+    double fm = 0.0;
 #pragma unroll
     for (int k = 0; k < D; ++k) fm = fma(fr[k], Vm[k], fm);
+    if (act) Vfm[r] = fm;
+    double lq_r[NL], lr[NL];
+    load_dist<S, D>(lq_r, cx.sLQ + rr * LD, sb);
+    const int r2 = (r >= D && r < 2 * D) ? r - D : 0;
+    load_dist<S, D>(lr, TL + r2 * LD, sb);
 #pragma unroll
-    for (int c = 0; c < D; ++c) {
-      ph[c] = t[c];
-      ph[D + c] = cx.sLQ[r * LD + c];
+    for (int t = 0; t < NL; ++t) {
+      ph[t] = act ? t1[t] : ((r < 2 * D) ? lr[t] : 0.0);
+      ph[NL + t] = act ? lq_r[t] : 0.0;
     }
-    Vfm[r] = fm;
-  } else if (r < 2 * D) {
-#pragma unroll
-    for (int c = 0; c < D; ++c) {
-      ph[c] = TL[(r - D) * LD + c];
-      ph[D + c] = 0.0;
-    }
-  } else {
-    zero(ph);
   }
-  lq<2 * D, 2 * D>(ph, r, cx.bc());
-  if (r < D) {
+  lq<S, 2 * D, D, D>(ph, r, sb, cx.bc());
+  if (act) {
 #pragma unroll
-    for (int c = 0; c < D; ++c) TP11[r * LD + c] = ph[c];
+    for (int t = 0; t < NL; ++t) TP11[r * LD + t * S + sb] = ph[t];
   }
   wsync();
-  // E row = Phi21 row Phi11^-1 ; D = Phi22 ; moved to lane r - D
-  if (r >= D && r < 2 * D) {
-    double e[D];
+  // E row = Phi21 row Phi11^-1 ; D = Phi22 ; moved to row r - D
+  {
+    double e[NL];
 #pragma unroll
-    for (int c = 0; c < D; ++c) e[c] = ph[c];
-    rsolve_lower<D, LD>(e, TP11);
+    for (int t = 0; t < NL; ++t) e[t] = ph[t];
+    rsolve_lower_s<S, D, LD>(e, TP11, sb);
+    if (r >= D && r < 2 * D) {
 #pragma unroll
-    for (int c = 0; c < D; ++c) {
-      TE[(r - D) * LD + c] = e[c];
-      TD[(r - D) * LD + c] = ph[D + c];
+      for (int t = 0; t < NL; ++t) {
+        TE[(r - D) * LD + t * S + sb] = e[t];
+        TD[(r - D) * LD + t * S + sb] = ph[NL + t];
+      }
     }
   }
   wsync();
   // observation statistics with obs model s (the smoother element stays in
   // LDS until the end so that no store is outstanding during the LQ below)
-  double sr[D + d];
+  double sr[NL + NLd];
   {
     double h[D];
     select_row<d, D>(H, r, h);
-    double t[D];
-    row_times_tile<D, D, LD>(h, TP11, t);
+    double t1[NL];
+    row_times_tile<S, D, D, LD>(h, TP11, sb, t1);
 #pragma unroll
-    for (int c = 0; c < D; ++c) sr[c] = (r < d) ? t[c] : 0.0;
+    for (int t = 0; t < NL; ++t) sr[t] = (r < d) ? t1[t] : 0.0;
 #pragma unroll
-    for (int c = 0; c < d; ++c) sr[D + c] = 0.0;
+    for (int t = 0; t < NLd; ++t) sr[NL + t] = 0.0;
   }
-  lq<d, D + d>(sr, r, cx.bc());
-  double* TS = cx.tile(4);
-  if (r < d) {
-#pragma unroll
-    for (int c = 0; c < d; ++c) TS[r * LD + c] = sr[c];
-  }
+  lq<S, d, D, d>(sr, r, sb, cx.bc());
+  if (r < d) store_dist<S, d>(TS + r * LD, sr, sb);
   wsync();
-  if (r == 0) {
+  if (r == 0 && sb == 0) {
     double yh[d];
 #pragma unroll
     for (int i = 0; i < d; ++i) {
       double acc = cv[i];
 #pragma unroll
-      for (int j = 0; j < D; ++j) acc = fma(H[i][j], Vfm[j], acc);
+      for (int k = 0; k < D; ++k) acc = fma(H[i][k], Vfm[k], acc);
       yh[i] = acc;
     }
     double wz[d], yz[d];
@@ -314,15 +317,15 @@ __global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK)) void k_filter_post(Group
     t_nll[s] = 0.5 * nn + (ld + d * log(2.0 * M_PI) / 2.0);
   }
   if (act) {
-    double e[D];
-    load_row(e, TE + r * LD);
+    double e[NL], dd[NL];
+    load_dist<S, D>(e, TE + r * LD, sb);
+    load_dist<S, D>(dd, TD + r * LD, sb);
     double gg = Vm[r];
 #pragma unroll
-    for (int k = 0; k < D; ++k) gg = fma(-e[k], Vfm[k], gg);
-    sel.g[ro * D + r] = gg;
-    store_row(sel.E + (ro * D + r) * D, e);
-#pragma unroll
-    for (int c = 0; c < D; ++c) sel.Dm[(ro * D + r) * D + c] = TD[r * LD + c];
+    for (int k = 0; k < D; ++k) gg = fma(-TE[r * LD + k], Vfm[k], gg);
+    if (sb == 0) sel.g[ro * D + r] = gg;
+    store_dist<S, D>(sel.E + (ro * D + r) * D, e, sb);
+    store_dist<S, D>(sel.Dm + (ro * D + r) * D, dd, sb);
   }
   wsync();
 }
@@ -331,7 +334,7 @@ __global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK)) void k_filter_post(Group
 template <int D>
 __global__ __launch_bounds__((Geo<D, kLdsSmooth>::BLOCK)) void k_smooth_up(GroupArgs g, LaunchMap lm, int l) {
   extern __shared__ double smem[];
-  Ctx<D> cx = make_ctx<D, kLdsSmooth>(g.F, g.LQ, g.N1z, smem);
+  auto cx = make_ctx<D, kLdsSmooth>(g.F, g.LQ, g.N1z, smem);
   int j, i;
   if (!locate(lm, cx.wid, j, i)) return;
   const long long n0 = (long long)g.Ns[j] + 1, base = g.base[j];
@@ -346,7 +349,7 @@ __global__ __launch_bounds__((Geo<D, kLdsSmooth>::BLOCK)) void k_smooth_up(Group
 template <int D>
 __global__ __launch_bounds__((Geo<D, kLdsSmooth>::BLOCK)) void k_smooth_down(GroupArgs g, LaunchMap lm, int l) {
   extern __shared__ double smem[];
-  Ctx<D> cx = make_ctx<D, kLdsSmooth>(g.F, g.LQ, g.N1z, smem);
+  auto cx = make_ctx<D, kLdsSmooth>(g.F, g.LQ, g.N1z, smem);
   int j, i;
   if (!locate(lm, cx.wid, j, i)) return;
   const long long n0 = (long long)g.Ns[j] + 1, base = g.base[j];
@@ -355,14 +358,16 @@ __global__ __launch_bounds__((Geo<D, kLdsSmooth>::BLOCK)) void k_smooth_down(Gro
   const StateArrays p = at(g.spre, base + lev_off(n0, l), D);
   const StateArrays up = (l + 1 < n_levels(n0)) ? at(g.spre, base + lev_off(n0, l + 1), D) : StateArrays{};
   const int with_chol = g.wchol[j];
+  constexpr int S = decltype(cx)::S, NL = Lay<D, S>::NL;
   const int j0 = 2 * i;
-  const int r = cx.lane;
+  const int r = cx.row, sb = cx.sub;
   if (j0 == 0) {
     if (r < D) {
-      p.m[r] = el.g[r];
+      if (sb == 0) p.m[r] = el.g[r];
       if (with_chol) {
-#pragma unroll
-        for (int c = 0; c < D; ++c) p.L[r * D + c] = el.Dm[r * D + c];
+        double u[NL];
+        load_dist<S, D>(u, el.Dm + r * D, sb);
+        store_dist<S, D>(p.L + r * D, u, sb);
       }
     }
   } else {
@@ -371,10 +376,11 @@ __global__ __launch_bounds__((Geo<D, kLdsSmooth>::BLOCK)) void k_smooth_down(Gro
   }
   const int j1 = j0 + 1;
   if (j1 < n && r < D) {
-    p.m[(size_t)j1 * D + r] = up.m[(size_t)i * D + r];
+    if (sb == 0) p.m[(size_t)j1 * D + r] = up.m[(size_t)i * D + r];
     if (with_chol) {
-#pragma unroll
-      for (int c = 0; c < D; ++c) p.L[((size_t)j1 * D + r) * D + c] = up.L[((size_t)i * D + r) * D + c];
+      double u[NL];
+      load_dist<S, D>(u, up.L + ((size_t)i * D + r) * D, sb);
+      store_dist<S, D>(p.L + ((size_t)j1 * D + r) * D, u, sb);
     }
   }
 }
