@@ -330,6 +330,26 @@ __device__ __forceinline__ double row_dot(const double (&x)[NX], const double* v
   return rowsum<S>(acc);
 }
 
+// sum_{k < n} p[k * stride] v[k] (a row (stride 1) or a column (stride LD) of
+// an LDS tile times an LDS vector), split over the row's lanes and reduced
+// with DPP; every lane of the row gets the sum.
+template <int S, int n, int stride>
+__device__ __forceinline__ double lds_dot(const double* p, const double* v, int sub) {
+  double acc = 0.0;
+#pragma unroll
+  for (int t = 0; t < (n + S - 1) / S; ++t) {
+    const int k = t * S + sub;
+    if ((t + 1) * S <= n) {
+      acc = fma(p[k * stride], v[k], acc);
+    } else {
+      const int kk = (k < n) ? k : 0;
+      const double x = p[kk * stride] * v[kk];
+      acc += (k < n) ? x : 0.0;
+    }
+  }
+  return rowsum<S>(acc);
+}
+
 // ----------------------------------------------------------------- products
 // out[t] = sum_{k < K} x[k] T[k][t S + sub] over columns < n (T: stride LD)
 template <int S, int K, int n, int LD, int NO, int NXF>
@@ -419,18 +439,29 @@ __device__ __forceinline__ void lq(double (&a)[(padded<B1, S>() + padded<B2, S>(
     // the pivot's own group of S columns: only columns after pc are tail
     const bool tl = sub > k0;
     const double p0 = tl ? p[t0] : 0.0;
-    double sq[4] = {0.0, 0.0, 0.0, 0.0};
-    double dt[4] = {0.0, 0.0, 0.0, 0.0};
+    // (CH independent chains each: one lane sums NX - t0 columns)
+    constexpr int CH = (NX > 16) ? 4 : 2;
+    double sq[CH], dt[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) sq[c] = dt[c] = 0.0;
     sq[0] = p0 * p0;
     dt[0] = a[t0] * p0;
 #pragma unroll
     for (int t = t0 + 1; t < NX; ++t) {
-      sq[(t - t0) & 3] = fma(p[t], p[t], sq[(t - t0) & 3]);
-      dt[(t - t0) & 3] = fma(a[t], p[t], dt[(t - t0) & 3]);
+      sq[(t - t0) % CH] = fma(p[t], p[t], sq[(t - t0) % CH]);
+      dt[(t - t0) % CH] = fma(a[t], p[t], dt[(t - t0) % CH]);
+    }
+    double sqs = sq[0], dts = dt[0];
+    if constexpr (CH == 4) {
+      sqs = (sq[0] + sq[1]) + (sq[2] + sq[3]);
+      dts = (dt[0] + dt[1]) + (dt[2] + dt[3]);
+    } else {
+      sqs = sq[0] + sq[1];
+      dts = dt[0] + dt[1];
     }
     const double ai = from_sub_k<S>(a[t0], k0);  // this row's entry in the pivot column
-    const double sig = rowsum<S>((sq[0] + sq[1]) + (sq[2] + sq[3]));
-    const double dot = rowsum<S>((dt[0] + dt[1]) + (dt[2] + dt[3]));
+    const double sig = rowsum<S>(sqs);
+    const double dot = rowsum<S>(dts);
     double nrm, rnrm;
     sqrt_rsqrt(fma(alpha, alpha, sig), nrm, rnrm);
     const double bc_ = -copysign(nrm, alpha);  // Fortran SIGN: honours the sign of -0.0
@@ -918,20 +949,17 @@ __device__ void filter_combine(const Ctx<D, SS>& cx, const FiltArrays& x1, const
   // a store in flight would make the next wait on a load also wait for it)
   double bb;
   {
-    double u = 0.0;
-#pragma unroll
-    for (int k = 0; k < D; ++k) u = fma(TU1[k * LD + rr], Veta2[k], u);
+    const double u = lds_dot<S, D, LD>(TU1 + rr, Veta2, s);
     if (act) Vu[r] = u;
     wsync();
-    double sv = b1r;
-#pragma unroll
-    for (int j = 0; j < D; ++j) sv = fma(TU1[rr * LD + j], Vu[j], sv);
+    const double sv = b1r + lds_dot<S, D, 1>(TU1 + rr * LD, Vu, s);
     if (act) Vs[r] = sv;
     wsync();
     bb = b2r + row_dot<S, D>(wr, Vs, s);
   }
-  // A = W A1 and eta = A1^T (I - Xi21 Xi11^-1 U1^T)(eta2 - Z2 Z2^T b1) + eta1
-  double ar[state_only ? 1 : NL];
+  // eta = A1^T (I - Xi21 Xi11^-1 U1^T)(eta2 - Z2 Z2^T b1) + eta1 and A = W A1
+  // (A's rows go to the LDS of U1, unused from here on, so that they are not
+  // held in registers through the factorisation below)
   double eo = 0.0;
   if constexpr (!state_only) {
     double* Vt = cx.vec(4);
@@ -939,26 +967,15 @@ __device__ void filter_combine(const Ctx<D, SS>& cx, const FiltArrays& x1, const
     double* Vu2 = cx.vec(6);
     double* Vs3 = cx.vec(7);
     {
-      double wf[D];
-      gather<S, D>(wr, wf);
-      row_times_tile<S, D, D, LD>(wf, TA1, s, ar);
-    }
-    {
-      double t = 0.0;
-#pragma unroll
-      for (int k = 0; k < D; ++k) t = fma(TZ2[k * LD + rr], Vb1[k], t);
+      const double t = lds_dot<S, D, LD>(TZ2 + rr, Vb1, s);
       if (act) Vt[r] = t;
     }
     wsync();
-    double s2 = eta2r;
-#pragma unroll
-    for (int j = 0; j < D; ++j) s2 = fma(-TZ2[rr * LD + j], Vt[j], s2);
+    const double s2 = eta2r - lds_dot<S, D, 1>(TZ2 + rr * LD, Vt, s);
     if (act) Vs2[r] = s2;
     wsync();
     {
-      double uu = 0.0;
-#pragma unroll
-      for (int k = 0; k < D; ++k) uu = fma(TU1[k * LD + rr], Vs2[k], uu);
+      const double uu = lds_dot<S, D, LD>(TU1 + rr, Vs2, s);
       if (act) Vu2[r] = uu;
     }
     wsync();
@@ -975,9 +992,18 @@ __device__ void filter_combine(const Ctx<D, SS>& cx, const FiltArrays& x1, const
       if (act) Vs3[r] = s2 - v2;
     }
     wsync();
-    eo = eta1r;
+    eo = eta1r + lds_dot<S, D, LD>(TA1 + rr, Vs3, s);
+    {
+      double wf[D];
+      gather<S, D>(wr, wf);
+      double ar[NL];
+      row_times_tile<S, D, D, LD>(wf, TA1, s, ar);
+      wsync();  // every row has read U1 for the last time
+      if (act) {
 #pragma unroll
-    for (int k = 0; k < D; ++k) eo = fma(TA1[k * LD + rr], Vs3[k], eo);
+        for (int t = 0; t < NL; ++t) TU1[r * LD + t * S + s] = ar[t];
+      }
+    }
   }
   // U = tria([MT | U2]) in rows 0..D-1 and, for the full combine,
   // Z = tria([A1^T Xi22 | Z1]) in rows D..2D-1, as one paired LQ
@@ -1000,6 +1026,8 @@ __device__ void filter_combine(const Ctx<D, SS>& cx, const FiltArrays& x1, const
     if (s == 0) (state_only ? so.m : y.b)[o * D + r] = bb;
     store_dist<S, D>((state_only ? so.L : y.U) + (o * D + r) * D, jr, s);
     if constexpr (!state_only) {
+      double ar[NL];
+      load_dist<S, D>(ar, TU1 + r * LD, s);
       store_dist<S, D>(y.A + (o * D + r) * D, ar, s);
       if (s == 0) y.eta[o * D + r] = eo;
     }

@@ -74,6 +74,11 @@ def test_batch_members_match_oracle(gpu):
         (om, _), oinfo = osolver.solve(oprob, ts, q, init=init, maxiters=60)
         m = ys.mean.cpu().numpy()
         assert info["iterations"] == oinfo["iterations"], (f.name, q, len(ts))
+        if oinfo["iterations"] > 60:
+            # not converged (e.g. a logistic y0 < 0 blows up in finite time): the
+            # iterates amplify rounding, the oracle's own results move by ~1e-9
+            # under a 1e-15 change of y0; only the iteration count is compared
+            continue
         err = np.abs(m - om).max() / max(np.abs(om).max(), 1e-300)
         # q = 4 priors (cond(Q) = 4.8e5) amplify rounding to ~5e-9; the acceptance bar is 1e-6
         assert err < (1e-8 if q == 4 else 1e-9), (f.name, q, len(ts), err)
