@@ -678,6 +678,8 @@ int plan_tables(pof_batch* b, const std::vector<int>& ids, std::vector<int>& h, 
     if (nivps) *nivps = ni;
   };
   table([](long long N) { return N; }, p.leaf_off, p.leaf_total, nullptr);
+  table([](long long) { return 1LL; }, p.one_off, p.one_total, nullptr);
+  table([](long long N) { return N - 1; }, p.leaf1_off, p.leaf1_total, nullptr);
   table([](long long N) { return N + 1; }, p.pts_off, p.pts_total, nullptr);
   table([](long long N) { return (N + 1 + kChunk - 1) / kChunk; }, p.chunk_off, p.chunk_total, nullptr);
   for (int l = 0; l + 1 < p.fl; ++l)

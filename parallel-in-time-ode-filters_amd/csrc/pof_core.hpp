@@ -595,14 +595,26 @@ __device__ __forceinline__ void lsolve_lowerT(double (&x)[NY], const double* L) 
 }
 
 // ---------------------------------------------------------- observation model
-// Affine observation of element k (noiseless): y = H x + c, either from EK1 at
-// lin[k+1] (pof/step.py:12-22, pof/observations.py:35-40: H = E1 - J E0,
-// c = J E0 m - f(E0 m)) or from explicit arrays.  Every lane computes the
-// whole (small) H and c.
+// Affine observation of element k (noiseless): y = H x + c.
+//
+// EK1 (pof/step.py:12-22, pof/observations.py:35-40) linearises at lin[k+1]:
+// H = E1 - J E0, c = J E0 m - f(E0 m).  In the IWP state layout row i of H has
+// hc[i][l] = -J[i][l] Pd[l q1] at the columns l q1 (l < d) and hd[i] =
+// Pd[i q1 + 1] at column i q1 + 1, zeros elsewhere; the products below use
+// only those d + 1 entries (skipping exact zeros leaves finite results
+// unchanged).  The affine variant reads an explicit dense H.  Every lane
+// holds the whole (small) model.
+template <int D, int d, bool EK1>
+struct Obs;
+
 template <int D, int d>
-__device__ __forceinline__ void observation(const ModelArgs& a, int k, double (&H)[d][D], double (&c)[d]) {
-  if (a.lin != nullptr) {
-    constexpr int q1 = D / d;
+struct Obs<D, d, true> {
+  static constexpr int q1 = D / d;
+  double hc[d][d];
+  double hd[d];
+  double c[d];
+
+  __device__ __forceinline__ void load(const ModelArgs& a, int k) {
     const double* m = a.lin + (size_t)(k + 1) * D;
     double y[d];
 #pragma unroll
@@ -612,18 +624,105 @@ __device__ __forceinline__ void observation(const ModelArgs& a, int k, double (&
     field_and_jacobian<d>(a.kind, a.params, y, f, J);
 #pragma unroll
     for (int i = 0; i < d; ++i) {
-#pragma unroll
-      for (int j = 0; j < D; ++j) H[i][j] = 0.0;
       double acc = 0.0;
 #pragma unroll
       for (int l = 0; l < d; ++l) {
-        H[i][l * q1] = -J[i][l] * a.Pd[l * q1];
+        hc[i][l] = -J[i][l] * a.Pd[l * q1];
         acc = fma(J[i][l], y[l], acc);
       }
-      H[i][i * q1 + 1] += a.Pd[i * q1 + 1];
+      hd[i] = a.Pd[i * q1 + 1];
       c[i] = acc - f[i];
     }
-  } else {
+  }
+  // sum_j H[i][j] v[j st]
+  __device__ __forceinline__ double dot(int i, const double* v, int st) const {
+    double acc = 0.0;
+#pragma unroll
+    for (int l = 0; l < d; ++l) acc = fma(hc[i][l], v[l * q1 * st], acc);
+    return fma(hd[i], v[(i * q1 + 1) * st], acc);
+  }
+  // row r (< d, any lane value) of H times the tile T: out[t] = sum_j H[r][j] T[j][t S + sub]
+  template <int S, int n, int LD, int NO>
+  __device__ __forceinline__ void row_times_tile(int r, const double* T, int sub, double (&out)[NO]) const {
+    double hr[d + 1];
+#pragma unroll
+    for (int l = 0; l < d; ++l) hr[l] = hc[0][l];
+    hr[d] = hd[0];
+#pragma unroll
+    for (int i = 1; i < d; ++i) {
+#pragma unroll
+      for (int l = 0; l < d; ++l) hr[l] = (r == i) ? hc[i][l] : hr[l];
+      hr[d] = (r == i) ? hd[i] : hr[d];
+    }
+    const int rd = ((r < d) ? r : 0) * q1 + 1;
+    zero(out);
+#pragma unroll
+    for (int l = 0; l <= d; ++l) {
+      const double* Tr = T + ((l < d) ? l * q1 : rd) * LD;
+#pragma unroll
+      for (int t = 0; t < NO; ++t) {
+        const int cc = t * S + sub;
+        if ((t + 1) * S <= n) {
+          out[t] = fma(hr[l], Tr[cc], out[t]);
+        } else if (t * S < n) {
+          const double v = Tr[(cc < n) ? cc : 0];
+          out[t] = fma(hr[l], (cc < n) ? v : 0.0, out[t]);
+        }
+      }
+      pin(out);
+    }
+  }
+  // (K H)[j] = sum_i K[i] H[i][j] for the row vector K (length d), as the
+  // coefficients of the columns l q1 (kh[l]) and i q1 + 1 (kh[d + i])
+  __device__ __forceinline__ void left_mul(const double (&K)[d], double (&kh)[2 * d]) const {
+#pragma unroll
+    for (int l = 0; l < d; ++l) {
+      double acc = 0.0;
+#pragma unroll
+      for (int i = 0; i < d; ++i) acc = fma(K[i], hc[i][l], acc);
+      kh[l] = acc;
+    }
+#pragma unroll
+    for (int i = 0; i < d; ++i) kh[d + i] = K[i] * hd[i];
+  }
+  // out[t] -= sum_j (K H)[j] T[j][t S + sub]
+  template <int S, int n, int LD, int NO>
+  __device__ __forceinline__ void sub_kh_tile(const double (&kh)[2 * d], const double* T, int sub,
+                                              double (&out)[NO]) const {
+#pragma unroll
+    for (int m = 0; m < 2 * d; ++m) {
+      const double* Tr = T + ((m < d) ? m * q1 : (m - d) * q1 + 1) * LD;
+#pragma unroll
+      for (int t = 0; t < NO; ++t) {
+        const int cc = t * S + sub;
+        if ((t + 1) * S <= n) {
+          out[t] = fma(-kh[m], Tr[cc], out[t]);
+        } else if (t * S < n) {
+          const double v = Tr[(cc < n) ? cc : 0];
+          out[t] = fma(-kh[m], (cc < n) ? v : 0.0, out[t]);
+        }
+      }
+      pin(out);
+    }
+  }
+  __device__ __forceinline__ void dense(double (&H)[d][D]) const {
+#pragma unroll
+    for (int i = 0; i < d; ++i) {
+#pragma unroll
+      for (int j = 0; j < D; ++j) H[i][j] = 0.0;
+#pragma unroll
+      for (int l = 0; l < d; ++l) H[i][l * q1] = hc[i][l];
+      H[i][i * q1 + 1] += hd[i];
+    }
+  }
+};
+
+template <int D, int d>
+struct Obs<D, d, false> {
+  double H[d][D];
+  double c[d];
+
+  __device__ __forceinline__ void load(const ModelArgs& a, int k) {
     const double* Hk = a.Hx + (size_t)k * d * D;
 #pragma unroll
     for (int i = 0; i < d; ++i) {
@@ -632,26 +731,65 @@ __device__ __forceinline__ void observation(const ModelArgs& a, int k, double (&
       c[i] = a.cx[(size_t)k * d + i];
     }
   }
-}
-
-// select row `r` (< d) of a register matrix without dynamic indexing
-template <int d, int D>
-__device__ __forceinline__ void select_row(const double (&H)[d][D], int r, double (&h)[D]) {
+  __device__ __forceinline__ double dot(int i, const double* v, int st) const {
+    double acc = 0.0;
 #pragma unroll
-  for (int j = 0; j < D; ++j) h[j] = H[0][j];
-#pragma unroll
-  for (int i = 1; i < d; ++i) {
-#pragma unroll
-    for (int j = 0; j < D; ++j) h[j] = (r == i) ? H[i][j] : h[j];
+    for (int j = 0; j < D; ++j) acc = fma(H[i][j], v[j * st], acc);
+    return acc;
   }
+  template <int S, int n, int LD, int NO>
+  __device__ __forceinline__ void row_times_tile(int r, const double* T, int sub, double (&out)[NO]) const {
+    double h[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) h[j] = H[0][j];
+#pragma unroll
+    for (int i = 1; i < d; ++i) {
+#pragma unroll
+      for (int j = 0; j < D; ++j) h[j] = (r == i) ? H[i][j] : h[j];
+    }
+    pof::row_times_tile<S, D, n, LD>(h, T, sub, out);
+  }
+  __device__ __forceinline__ void left_mul(const double (&K)[d], double (&kh)[D]) const {
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      double acc = 0.0;
+#pragma unroll
+      for (int i = 0; i < d; ++i) acc = fma(K[i], H[i][j], acc);
+      kh[j] = acc;
+    }
+  }
+  template <int S, int n, int LD, int NO>
+  __device__ __forceinline__ void sub_kh_tile(const double (&kh)[D], const double* T, int sub,
+                                              double (&out)[NO]) const {
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+#pragma unroll
+      for (int t = 0; t < NO; ++t) {
+        const int cc = t * S + sub;
+        if ((t + 1) * S <= n) {
+          out[t] = fma(-kh[j], T[j * LD + cc], out[t]);
+        } else if (t * S < n) {
+          const double v = T[j * LD + ((cc < n) ? cc : 0)];
+          out[t] = fma(-kh[j], (cc < n) ? v : 0.0, out[t]);
+        }
+      }
+      pin(out);
+    }
+  }
+};
+
+// length of the (K H) coefficient vector of Obs::left_mul
+template <int D, int d, bool EK1>
+constexpr int kh_len() {
+  return EK1 ? 2 * d : D;
 }
 
 // ----------------------------------------------------------- filter element
 // _get_element (pof/parallel_filtsmooth/filter.py:50-81) for element k with
 // prior (x0m, x0L) when first, else the zero prior.  Row r < D of A, U, Z is
 // returned split over the row's lanes; b and eta (entry r) in every lane of row r.
-template <int D, int d, int SS>
-__device__ void filter_element(const Ctx<D, SS>& cx, const ModelArgs& a, int k, bool first,
+template <int D, int d, bool EK1, bool first, int SS>
+__device__ void filter_element(const Ctx<D, SS>& cx, const ModelArgs& a, int k,
                                double (&A)[Lay<D, SS>::NL], double& b, double (&U)[Lay<D, SS>::NL], double& eta,
                                double (&Z)[Lay<D, SS>::NL]) {
   using L = Lay<D, SS>;
@@ -659,9 +797,8 @@ __device__ void filter_element(const Ctx<D, SS>& cx, const ModelArgs& a, int k, 
   constexpr int NLd = padded<d, S>() / S;
   const int r = cx.row, s = cx.sub;
   const int rr = (r < D) ? r : 0;
-  double H[d][D];
-  double cv[d];
-  observation<D, d>(a, k, H, cv);
+  Obs<D, d, EK1> ob;
+  ob.load(a, k);
 
   double* T0 = cx.tile(0);
   double* T1 = cx.tile(1);
@@ -670,7 +807,7 @@ __device__ void filter_element(const Ctx<D, SS>& cx, const ModelArgs& a, int k, 
   double* V0 = cx.vec(0);
   const double* N1 = cx.sN1;
   double m1r = 0.0;
-  if (first) {
+  if constexpr (first) {
     // m1 = F m0, N1 = tria([F L0 | LQ])
     double row[2 * NL];
     {
@@ -710,10 +847,8 @@ __device__ void filter_element(const Ctx<D, SS>& cx, const ModelArgs& a, int k, 
   // Psi_ = [[H N1, R], [N1, 0]]  (R = 0): rows 0..d-1 and d..d+D-1, column blocks (D, d)
   double psi[NL + NLd];
   {
-    double h[D];
-    select_row<d, D>(H, r, h);
     double hn[NL];
-    row_times_tile<S, D, D, LD>(h, N1, s, hn);
+    ob.template row_times_tile<S, D, LD>(r, N1, s, hn);
     double nr[NL];
     const int rn = (r >= d && r < d + D) ? r - d : 0;
     load_dist<S, D>(nr, N1 + rn * LD, s);
@@ -750,33 +885,14 @@ __device__ void filter_element(const Ctx<D, SS>& cx, const ModelArgs& a, int k, 
   }
   wsync();
 
-  // HF = H F (split columns), hm1 = H m1 + c (every lane)
-  double HF[d][NL];
+  // hm1 = H m1 + c (every lane); m1 = 0 without a prior
   double hm1[d];
 #pragma unroll
-  for (int i = 0; i < d; ++i) {
-#pragma unroll
-    for (int t = 0; t < NL; ++t) HF[i][t] = 0.0;
-    hm1[i] = cv[i];
-  }
-#pragma unroll
-  for (int j = 0; j < D; ++j) {
-    double fj[NL];
-    load_dist<S, D>(fj, cx.sF + j * LD, s);
-    const double m1j = first ? V0[j] : 0.0;
-#pragma unroll
-    for (int i = 0; i < d; ++i) {
-#pragma unroll
-      for (int t = 0; t < NL; ++t) HF[i][t] = fma(H[i][j], fj[t], HF[i][t]);
-      hm1[i] = fma(H[i][j], m1j, hm1[i]);
-      pin(HF[i]);
-    }
-  }
-  pin(hm1);
+  for (int i = 0; i < d; ++i) hm1[i] = first ? ob.c[i] + ob.dot(i, V0, 1) : ob.c[i];
   // w = Psi11^-1 c
   double wv[d];
 #pragma unroll
-  for (int i = 0; i < d; ++i) wv[i] = cv[i];
+  for (int i = 0; i < d; ++i) wv[i] = ob.c[i];
   lsolve_lower<d, LD>(wv, T1);
 
   {
@@ -784,25 +900,21 @@ __device__ void filter_element(const Ctx<D, SS>& cx, const ModelArgs& a, int k, 
 #pragma unroll
     for (int i = 0; i < d; ++i) kr[i] = T3[rr * LD + i];
     load_dist<S, D>(U, T2 + rr * LD, s);
-    // A = F - K H F ; b = m1 - K (H m1 + c)
+    // A = F - (K H) F ; b = m1 - K (H m1 + c)
+    double kh[kh_len<D, d, EK1>()];
+    ob.left_mul(kr, kh);
     load_dist<S, D>(A, cx.sF + rr * LD, s);
+    ob.template sub_kh_tile<S, D, LD>(kh, cx.sF, s, A);
     double bb = m1r;
 #pragma unroll
-    for (int i = 0; i < d; ++i) {
-#pragma unroll
-      for (int t = 0; t < NL; ++t) A[t] = fma(-kr[i], HF[i][t], A[t]);
-      bb = fma(-kr[i], hm1[i], bb);
-    }
+    for (int i = 0; i < d; ++i) bb = fma(-kr[i], hm1[i], bb);
     b = bb;
   }
-  // Z row = (Psi11^-1 HF[:, r])^T ; eta = -Z Psi11^-1 c
+  // Z row = (Psi11^-1 (H F)[:, r])^T ; eta = -Z Psi11^-1 c
   double z[d];
 #pragma unroll
   for (int i = 0; i < d; ++i) {
-    double acc = 0.0;
-#pragma unroll
-    for (int j = 0; j < D; ++j) acc = fma(H[i][j], cx.sF[j * LD + rr], acc);
-    z[i] = acc;
+    z[i] = ob.dot(i, cx.sF + rr, LD);
     pin1(z[i]);
   }
   lsolve_lower<d, LD>(z, T1);

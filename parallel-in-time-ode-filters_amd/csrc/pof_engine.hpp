@@ -42,6 +42,8 @@ struct LaunchPlan {
   int m = 0;                 // active IVPs
   int fl = 0, sl = 0;        // filter / smoother levels (maximum over the active IVPs)
   int leaf_off = 0, leaf_total = 0;    // per IVP: Ns work items
+  int one_off = 0, one_total = 0;      // 1 (the first element)
+  int leaf1_off = 0, leaf1_total = 0;  // Ns - 1 (the other elements)
   int pts_off = 0, pts_total = 0;      // Ns + 1
   int chunk_off = 0, chunk_total = 0;  // ceil((Ns + 1) / kChunk)
   int fup_off[kMaxLevels] = {}, fup_total[kMaxLevels] = {};  // Ns >> (l + 1)

@@ -93,19 +93,21 @@ __device__ __forceinline__ ModelArgs view(const GroupArgs& g, int j) {
 }
 
 // ------------------------------------------------------------------ kernels
-// Leaf elements: worker k builds filter element k (k = 0 carries x0).
-template <int D, int d>
+// Leaf elements.  FIRST: element 0 of each IVP (it carries the prior x0;
+// one work item per IVP); otherwise work item i builds element i + 1.
+template <int D, int d, bool EK1, bool FIRST>
 __global__ __launch_bounds__((Geo<D, kLdsLeaf>::BLOCK)) void k_filter_leaf(GroupArgs g, LaunchMap lm) {
   extern __shared__ double smem[];
   auto cx = make_ctx<D, kLdsLeaf>(g.F, g.LQ, g.N1z, smem);
   int j, k;
   if (!locate(lm, cx.wid, j, k)) return;
+  k = FIRST ? 0 : k + 1;
   const ModelArgs a = view<D>(g, j);
   const FiltArrays out = at(g.fel, g.base[j], D);
   constexpr int S = decltype(cx)::S, NL = Lay<D, S>::NL;
   double A[NL], U[NL], Z[NL];
   double b, eta;
-  filter_element<D, d>(cx, a, k, k == 0, A, b, U, eta, Z);
+  filter_element<D, d, EK1, FIRST>(cx, a, k, A, b, U, eta, Z);
   const int r = cx.row;
   if (r < D) {
     const size_t o = (size_t)k * D + r;
@@ -175,7 +177,7 @@ __global__ __launch_bounds__((Geo<D, kLdsState>::BLOCK)) void k_filter_down(Grou
 // s < Ns the predicted observation statistics of filter.py:84-114:
 //   S = tria([H_s tria([F L_s | LQ]) | 0]), yhat = H_s F m_s + c_s,
 //   ssq_s = |S^-T yhat|^2 (utils.py:110-112), nll_s = -loglik(yhat, S) (utils.py:22-30).
-template <int D, int d>
+template <int D, int d, bool EK1>
 __global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK)) void k_filter_post(GroupArgs g, LaunchMap lm) {
   extern __shared__ double smem[];
   auto cx = make_ctx<D, kLdsPost>(g.F, g.LQ, g.N1z, smem);
@@ -216,9 +218,8 @@ __global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK)) void k_filter_post(Group
   double* Vm = cx.vec(0);
   double* Vfm = cx.vec(1);
   // global reads first: the filtered state and the linearisation point of obs model s
-  double H[d][D];
-  double cv[d];
-  observation<D, d>(a, s, H, cv);
+  Obs<D, d, EK1> ob;
+  ob.load(a, s);
   {
     double lr[NL];
     load_dist<S, D>(lr, Lp + rr * D, sb);
@@ -276,10 +277,8 @@ __global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK)) void k_filter_post(Group
   // LDS until the end so that no store is outstanding during the LQ below)
   double sr[NL + NLd];
   {
-    double h[D];
-    select_row<d, D>(H, r, h);
     double t1[NL];
-    row_times_tile<S, D, D, LD>(h, TP11, sb, t1);
+    ob.template row_times_tile<S, D, LD>(r, TP11, sb, t1);
 #pragma unroll
     for (int t = 0; t < NL; ++t) sr[t] = (r < d) ? t1[t] : 0.0;
 #pragma unroll
@@ -291,12 +290,7 @@ __global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK)) void k_filter_post(Group
   if (r == 0 && sb == 0) {
     double yh[d];
 #pragma unroll
-    for (int i = 0; i < d; ++i) {
-      double acc = cv[i];
-#pragma unroll
-      for (int k = 0; k < D; ++k) acc = fma(H[i][k], Vfm[k], acc);
-      yh[i] = acc;
-    }
+    for (int i = 0; i < d; ++i) yh[i] = ob.c[i] + ob.dot(i, Vfm, 1);
     double wz[d], yz[d];
 #pragma unroll
     for (int i = 0; i < d; ++i) {
@@ -561,9 +555,11 @@ __global__ void k_linearize(ModelArgs a, const double* m, int n, double* H, doub
   ModelArgs b = a;
   b.lin = m - D;  // observation(k) reads lin[k + 1]
   b.Hx = nullptr;
+  Obs<D, d, true> ob;
+  ob.load(b, k);
   double Hk[d][D];
-  double ck[d];
-  observation<D, d>(b, k, Hk, ck);
+  ob.dense(Hk);
+  const double (&ck)[d] = ob.c;
 #pragma unroll
   for (int i = 0; i < d; ++i) {
 #pragma unroll
@@ -629,7 +625,17 @@ template <int D, int d>
 hipError_t run_iteration(const GroupArgs& g, const LaunchPlan& p, double* scal, hipStream_t st, KernelTimer* t) {
   Rec rec(t, st);
   // ---- filter: leaf elements, up-sweep, down-sweep
-  POF_LAUNCH((k_filter_leaf<D, d>), kLdsLeaf, p.leaf_total, st, g, p.map(p.leaf_off, p.leaf_total));
+  if (g.affine) {
+    POF_LAUNCH((k_filter_leaf<D, d, false, true>), kLdsLeaf, p.one_total, st, g, p.map(p.one_off, p.one_total));
+    if (p.leaf1_total > 0)
+      POF_LAUNCH((k_filter_leaf<D, d, false, false>), kLdsLeaf, p.leaf1_total, st, g,
+                 p.map(p.leaf1_off, p.leaf1_total));
+  } else {
+    POF_LAUNCH((k_filter_leaf<D, d, true, true>), kLdsLeaf, p.one_total, st, g, p.map(p.one_off, p.one_total));
+    if (p.leaf1_total > 0)
+      POF_LAUNCH((k_filter_leaf<D, d, true, false>), kLdsLeaf, p.leaf1_total, st, g,
+                 p.map(p.leaf1_off, p.leaf1_total));
+  }
   rec(kKFilterLeaf, p.leaf_total);
   for (int l = 0; l + 1 < p.fl; ++l) {
     const int nw = p.fup_total[l];
@@ -644,7 +650,11 @@ hipError_t run_iteration(const GroupArgs& g, const LaunchPlan& p, double* scal, 
     rec(kKFilterDown, nw - p.fdn_ivps[l]);
   }
   // ---- smoother elements + observation statistics
-  POF_LAUNCH((k_filter_post<D, d>), kLdsPost, p.pts_total, st, g, p.map(p.pts_off, p.pts_total));
+  if (g.affine) {
+    POF_LAUNCH((k_filter_post<D, d, false>), kLdsPost, p.pts_total, st, g, p.map(p.pts_off, p.pts_total));
+  } else {
+    POF_LAUNCH((k_filter_post<D, d, true>), kLdsPost, p.pts_total, st, g, p.map(p.pts_off, p.pts_total));
+  }
   rec(kKFilterPost, p.leaf_total);
   // ---- smoother scan (reversed order)
   smooth_scan<D>(g, p, st, rec);
