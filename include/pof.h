@@ -49,6 +49,7 @@ extern "C" {
 /* initial trajectory (pof/convenience.py:76-92) */
 #define POF_INIT_PRIOR 0
 #define POF_INIT_CONSTANT 1
+#define POF_INIT_COARSE 2   /* sequential EKS on 100 points, nearest-left lookup (initialization.py:112-129) */
 
 #define POF_MAX_PARAMS 8
 
@@ -66,7 +67,7 @@ typedef struct {
   int32_t kind;                    /* POF_LOGISTIC ... */
   int32_t d;                       /* ODE dimension (len(y0)) */
   int32_t q;                       /* order = number of derivatives of the IWP prior, >= 1 */
-  int32_t init;                    /* POF_INIT_PRIOR / POF_INIT_CONSTANT */
+  int32_t init;                    /* POF_INIT_PRIOR / POF_INIT_CONSTANT / POF_INIT_COARSE */
   int64_t n_times;                 /* len(ts) = Ns + 1 >= 2 */
   const double* ts_host;           /* [n_times]; uniform grid assumed, dt = ts[1] - ts[0] */
   const double* y0_host;           /* [d] */
@@ -77,7 +78,7 @@ typedef struct {
   int32_t maxiters;    /* the loop runs while k <= maxiters (reference semantics) */
   int32_t fixed_iters; /* > 0: run exactly this many iterations, no convergence test */
   int32_t calibrate;   /* 1: returned factors scaled by sqrt(ssq) twice (solver.py:66-68 + step.py:42-44) */
-  int32_t reserved;
+  int32_t sequential;  /* 1: sequential filter + smoother per iteration (solve(sequential=True), step.py:39-40) */
 } pof_opts;
 
 typedef struct {
@@ -128,8 +129,23 @@ int pof_transitions(int d, int q, double dt, double* F_host, double* QL_host, do
 
 /* Initial trajectory means [n_times, D] on the host.  Replaces
  * pof.convenience.get_initial_trajectory (convenience.py:76-92) for "constant"
- * (PI-scaled) and "prior" (not PI-scaled, steps = ts[1:]). */
+ * (PI-scaled) and "prior" (not PI-scaled, steps = ts[1:]); "coarse" needs the
+ * GPU (pof_coarse_means) and returns POF_E_UNSUPPORTED here. */
 int pof_initial_means(const pof_problem* prob, double* means_host);
+
+/* "coarse" initial trajectory means [n_times, D] on the host: sequential EKS
+ * on 100 equispaced points on the GPU, looked up at floor(ts / coarse_dt),
+ * times PI.  Replaces pof.initialization.coarse_ekf_init (initialization.py:112-129)
+ * with convenience.py:80-83. */
+int pof_coarse_means(pof_ctx* ctx, const pof_problem* prob, double* means_host);
+
+/* Single-pass sequential EKS.  Replaces pof.solver.sequential_eks_solve
+ * (pof/solver.py:76-96): sequential square-root EKF (linearised at the
+ * predicted means) + RTS smoother; factors scaled by sqrt(ssq) when
+ * calibrate.  out->means_dev / chols_dev receive E0 (m, L); full_means_dev /
+ * full_chols_dev (return_full_states) receive P (m, L).  info: nll is the
+ * summed log-likelihood, as in the reference. */
+int pof_eks_solve(pof_ctx* ctx, const pof_problem* prob, int calibrate, pof_out* out, void* stream);
 
 /* Vector field f(y) at one point on the host (pof/ivp.py f(None, y)). */
 int pof_eval_field(int kind, int d, const double* params, const double* y_host, double* f_host);
@@ -141,20 +157,24 @@ int pof_linearize(pof_ctx* ctx, const pof_problem* prob, const double* m_dev, in
                   double* c_dev, void* stream);
 
 /* One IEKS iteration.  Replaces pof.step.ieks_step (step.py:33-45): linearise at
- * states_dev [Ns+1, D], parallel filter + smoother, chol *= sqrt(ssq).
- * Outputs full states [Ns+1, D] / [Ns+1, D, D] and nll, obj, ssq. */
-int pof_ieks_step(pof_ctx* ctx, const pof_problem* prob, const double* states_mean_dev, double* out_mean_dev,
-                  double* out_chol_dev, double* nll, double* obj, double* ssq, void* stream);
+ * states_dev [Ns+1, D], parallel (sequential = 0) or sequential (1) filter +
+ * smoother, chol *= sqrt(ssq).  Outputs full states [Ns+1, D] / [Ns+1, D, D]
+ * and nll, obj, ssq (nll is +sum log-likelihood on the sequential branch, as
+ * in the reference). */
+int pof_ieks_step(pof_ctx* ctx, const pof_problem* prob, const double* states_mean_dev, int sequential,
+                  double* out_mean_dev, double* out_chol_dev, double* nll, double* obj, double* ssq, void* stream);
 
 /* Noiseless linear filter + smoother with a constant transition (F, QL) and
  * observations y_k = H_k x_{k+1} + c_k = 0, k < Ns.  Replaces
  * pof.parallel_filtsmooth.linear_filtsmooth (parallel_filtsmooth/__init__.py:5-10,
- * filter.py:18-47, smoother.py:8-22).  x0 mean [D] / factor [D, D], F/QL [D, D]
- * on the host; H [Ns, d, D], c [Ns, d] and outputs on the device.  filt_* (may be
- * NULL) receive the filtered states [Ns+1, ...]. */
+ * filter.py:18-47, smoother.py:8-22) and, with sequential = 1,
+ * pof.sequential_filtsmooth.linear_filtsmooth (sequential_filtsmooth/__init__.py:13-16).
+ * x0 mean [D] / factor [D, D], F/QL [D, D] on the host; H [Ns, d, D], c [Ns, d]
+ * and outputs on the device.  filt_* (may be NULL) receive the filtered states
+ * [Ns+1, ...]. */
 int pof_linear_filtsmooth(pof_ctx* ctx, int D, int d, int64_t Ns, const double* x0_mean_host,
                           const double* x0_chol_host, const double* F_host, const double* QL_host,
-                          const double* H_dev, const double* c_dev, double* smooth_mean_dev,
+                          const double* H_dev, const double* c_dev, int sequential, double* smooth_mean_dev,
                           double* smooth_chol_dev, double* filt_mean_dev, double* filt_chol_dev, double* nll,
                           double* obj, double* ssq, void* stream);
 

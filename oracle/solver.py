@@ -87,13 +87,26 @@ def sequential_eks_solve(problem, ts, order, return_full_states=False, calibrate
     return MVNSqrt(mv(A, states.mean), A @ states.chol), info
 
 
+def jax_linspace(start, stop, num):
+    """``jnp.linspace(start, stop, num)`` (jax 0.3.23, endpoint=True): ``start (1 - k/div) + stop k/div``
+    for ``k < div = num - 1``, then ``stop`` (the rounding differs from numpy's ``start + k step``)."""
+    div = num - 1
+    step = np.arange(div, dtype=np.float64) / div
+    return np.concatenate([start * (1.0 - step) + stop * step, [stop]])
+
+
 def coarse_ekf_init(problem, q, ts, N=10):
-    """(``pof/initialization.py:112-129``): EKS on a coarse grid, nearest-left lookup."""
+    """(``pof/initialization.py:112-129``): EKS on a coarse grid, nearest-left lookup.
+
+    ``coarse_ts = jnp.linspace(ts[0], ts[-1], N)``; ``idx = floor(ts / coarse_dt)``
+    (not ``ts - ts[0]``, as in the reference); the gather ``l[idx]`` counts a
+    negative index from the end and clamps out-of-range indices (JAX indexing).
+    """
     ts = np.asarray(ts, dtype=np.float64)
-    coarse_ts = np.linspace(ts[0], ts[-1], N)
+    coarse_ts = jax_linspace(ts[0], ts[-1], N)
     coarse_dt = coarse_ts[1] - coarse_ts[0]
     out, _ = sequential_eks_solve(problem, coarse_ts, q, return_full_states=True)
-    idxs = np.floor(ts / coarse_dt).astype(int)
-    # jax clamps out-of-range gathers
+    idxs = np.floor(ts / coarse_dt).astype(np.int64)
+    idxs = np.where(idxs < 0, idxs + N, idxs)
     idxs = np.clip(idxs, 0, N - 1)
     return MVNSqrt(out.mean[idxs], out.chol[idxs])

@@ -326,6 +326,10 @@ static int initial_means(const pof_problem* pr, const Setup& s, std::vector<doub
     }
     return POF_OK;
   }
+  if (pr->init == POF_INIT_COARSE) {
+    err = "method=coarse runs a sequential EKS on the device (pof_coarse_means)";
+    return POF_E_UNSUPPORTED;
+  }
   err = "method=" + std::to_string(pr->init) + " not found";
   return POF_E_INIT;
 }
@@ -346,6 +350,8 @@ struct pof_ctx {
   long long prof_work[pof::kNumKernelClasses] = {};
   // batches kept for reuse by pof_solve / pof_ieks_step / pof_solve_batch, one per (D, d)
   std::vector<pof_batch*> cache;
+  // small batches for the coarse initial trajectory (EKS on 100 points), one per (D, d)
+  std::vector<pof_batch*> aux;
 };
 
 // A group of IVPs with one (D, d) sharing one device workspace.  IVP slots are
@@ -363,13 +369,14 @@ struct pof_batch {
     long long ticket = 0;
     long long start = 0, len = 0;
     int last_wchol = 0;
-    bool raw_scale = false;  // output factors unscaled (linear filter/smoother API)
+    int scale_mode = 0;  // output factor scale: 0 ssq (calibrate) / sqrt(ssq); 1 none; 2 sqrt(ssq) if calibrate
   };
 
   pof_ctx* ctx = nullptr;
   int D = 0, d = 0, q = 0;
   bool affine = false;
   bool cached = false;
+  int mode = -1;  // 0 parallel IEKS, 1 sequential IEKS, 2 sequential EKS single pass (set by the first admission)
   const pof::EngineFns* e = nullptr;
   long long entries = 0;
   int slots = 0;
@@ -381,7 +388,7 @@ struct pof_batch {
   // writable device tables (g holds the same pointers)
   double *F = nullptr, *LQ = nullptr, *N1z = nullptr, *Pd = nullptr, *x0m = nullptr, *x0L = nullptr;
   double *params = nullptr, *scale = nullptr, *scal_dev = nullptr, *scal_host = nullptr;
-  int *kind = nullptr, *Ns = nullptr, *cur = nullptr, *wchol = nullptr;
+  int *kind = nullptr, *Ns = nullptr, *cur = nullptr, *wchol = nullptr, *pfull = nullptr;
   long long* base = nullptr;
   double** outp = nullptr;  // [4, slots]: means, chols, full means, full chols
   int* maps_dev = nullptr;
@@ -389,6 +396,7 @@ struct pof_batch {
   // host mirrors
   std::vector<double> hPd, hx0m, hx0L, hparams, hscale;
   std::vector<int> hkind, hNs, hcur, hwchol;  // hcur: 0, uploaded at admission (the device flips it)
+  std::vector<int> hpfull;
   std::vector<long long> hbase;
   std::vector<double*> hout;
   std::vector<int> hmaps;
@@ -542,6 +550,7 @@ int batch_new(pof_ctx* ctx, int D, int d, long long entries, int slots, bool aff
   POF_A(b->F, DD) POF_A(b->LQ, DD) POF_A(b->N1z, DD) POF_A(b->Pd, S * D) POF_A(b->x0m, S * D)
   POF_A(b->x0L, S * DD) POF_A(b->params, S * kMaxParams) POF_A(b->scale, S) POF_A(b->scal_dev, S * 8)
   POF_A(b->kind, S) POF_A(b->Ns, S) POF_A(b->cur, S) POF_A(b->wchol, S) POF_A(b->base, S) POF_A(b->outp, 4 * S)
+  POF_A(b->pfull, S)
 #undef POF_A
   if (he == hipSuccess) he = hipHostMalloc((void**)&b->scal_host, S * 8 * sizeof(double));
   if (he == hipSuccess) he = hipStreamCreateWithFlags(&b->own_st, hipStreamNonBlocking);
@@ -579,6 +588,7 @@ int batch_new(pof_ctx* ctx, int D, int d, long long entries, int slots, bool aff
   g.out_full_means = b->outp + 2 * S;
   g.out_full_chols = b->outp + 3 * S;
   g.out_scale = b->scale;
+  g.out_pfull = b->pfull;
   g.Hx = nullptr;
   g.cx = nullptr;
   g.q = b->q;
@@ -592,6 +602,7 @@ int batch_new(pof_ctx* ctx, int D, int d, long long entries, int slots, bool aff
   b->hNs.assign(S, 1);
   b->hcur.assign(S, 0);
   b->hwchol.assign(S, 0);
+  b->hpfull.assign(S, 0);
   b->hbase.assign(S, 0);
   b->hout.assign(4 * S, nullptr);
   b->sl.assign(S, pof_batch::Slot{});
@@ -704,9 +715,14 @@ int settle(pof_batch* b) {
 
 int admit_raw(pof_batch* b, int Nsteps, const double* Pd, const double* x0m, const double* x0L, int kind,
               const double* params, std::vector<double>* m_host, const double* m_dev, const pof_opts* opts,
-              pof_out* out, bool raw_scale, long long* ticket) {
+              pof_out* out, int scale_mode, int mode, bool pfull, long long* ticket) {
   pof_ctx* ctx = b->ctx;
   if (b->launched) return set_err(ctx, POF_E_ARG, "admit between pof_batch_launch and pof_batch_finish");
+  bool busy = false;
+  for (const auto& sl : b->sl) busy = busy || sl.used;
+  if (busy && b->mode != mode)
+    return set_err(ctx, POF_E_ARG, "the batch runs mode %d solves (0 parallel, 1 sequential, 2 EKS), got %d", b->mode,
+                   mode);
   // the host tables change below: no earlier copy from them may still be pending
   if (settle(b) != POF_OK) return POF_E_HIP;
   int j = -1;
@@ -729,7 +745,8 @@ int admit_raw(pof_batch* b, int Nsteps, const double* Pd, const double* x0m, con
   s.ticket = b->next_ticket++;
   s.start = start;
   s.len = len;
-  s.raw_scale = raw_scale;
+  s.scale_mode = scale_mode;
+  b->mode = mode;
   std::copy(Pd, Pd + D, &b->hPd[(size_t)j * D]);
   std::copy(x0m, x0m + D, &b->hx0m[(size_t)j * D]);
   std::copy(x0L, x0L + DD, &b->hx0L[(size_t)j * DD]);
@@ -739,6 +756,7 @@ int admit_raw(pof_batch* b, int Nsteps, const double* Pd, const double* x0m, con
   b->hbase[j] = start;
   b->hcur[j] = 0;
   b->hwchol[j] = 0;
+  b->hpfull[j] = pfull ? 1 : 0;
   b->hout[0 * S + j] = out->means_dev;
   b->hout[1 * S + j] = out->chols_dev;
   b->hout[2 * S + j] = out->full_means_dev;
@@ -805,6 +823,7 @@ int batch_launch(pof_batch* b) {
                                b->st));
       HIP_TRY(hipMemcpyAsync(b->cur + j, &b->hcur[j], sizeof(int), hipMemcpyHostToDevice, b->st));
       HIP_TRY(hipMemcpyAsync(b->wchol + j, &b->hwchol[j], sizeof(int), hipMemcpyHostToDevice, b->st));
+      HIP_TRY(hipMemcpyAsync(b->pfull + j, &b->hpfull[j], sizeof(int), hipMemcpyHostToDevice, b->st));
     }
     b->dirty_slots.clear();
     HIP_TRY(hipMemcpyAsync(b->kind, b->hkind.data(), S * sizeof(int), hipMemcpyHostToDevice, b->st));
@@ -850,7 +869,11 @@ int batch_launch(pof_batch* b) {
     HIP_TRY(hipMemcpyAsync(b->wchol, b->hwchol.data(), (size_t)b->slots * sizeof(int), hipMemcpyHostToDevice, b->st));
   }
   KernelTimer* tm = ctx->profiling ? &ctx->timer : nullptr;
-  HIP_TRY(b->e->iteration(b->g, b->plan, b->scal_dev, b->st, tm));
+  if (b->mode >= 1) {
+    HIP_TRY(b->e->seq_iteration(b->g, b->plan, b->scal_dev, b->mode == 2 ? 1 : 0, b->st, tm));
+  } else {
+    HIP_TRY(b->e->iteration(b->g, b->plan, b->scal_dev, b->st, tm));
+  }
   hipLaunchKernelGGL(k_flip_cur, dim3((b->plan.m + 255) / 256), dim3(256), 0, b->st, b->cur, b->plan.dev, b->plan.m);
   HIP_TRY(hipGetLastError());
   HIP_TRY(collect_timing(ctx, b->st));
@@ -920,8 +943,12 @@ int batch_finish(pof_batch* b, int max_done, long long* done, int* n_done) {
   for (int j : fin) {
     const pof_batch::Slot& s = b->sl[j];
     b->hwchol[j] = 1;
-    // double calibration (step.py:42-44 inside the loop + solver.py:66-68)
-    b->hscale[j] = s.raw_scale ? 1.0 : (s.opts.calibrate ? s.ssq : std::sqrt(s.ssq));
+    // double calibration (step.py:42-44 inside the loop + solver.py:66-68); the
+    // linear filter/smoother API returns raw factors; the EKS calibrates once
+    // (solver.py:86-89)
+    b->hscale[j] = (s.scale_mode == 1)   ? 1.0
+                   : (s.scale_mode == 2) ? (s.opts.calibrate ? std::sqrt(s.ssq) : 1.0)
+                                         : (s.opts.calibrate ? s.ssq : std::sqrt(s.ssq));
   }
   b->hmaps.clear();
   LaunchPlan pc, po;
@@ -948,7 +975,11 @@ int batch_finish(pof_batch* b, int max_done, long long* done, int* n_done) {
   HIP_TRY(hipMemcpyAsync(b->scale, b->hscale.data(), S * sizeof(double), hipMemcpyHostToDevice, b->st));
   KernelTimer* tm = ctx->profiling ? &ctx->timer : nullptr;
   if (!need.empty()) {
-    HIP_TRY(b->e->smooth_chol(b->g, pc, b->st, tm));
+    if (b->mode >= 1) {
+      HIP_TRY(b->e->seq_smooth_chol(b->g, pc, b->st, tm));
+    } else {
+      HIP_TRY(b->e->smooth_chol(b->g, pc, b->st, tm));
+    }
     HIP_TRY(collect_timing(ctx, b->st));
   }
   HIP_TRY(b->e->output(b->g, po, b->st));
@@ -1020,6 +1051,85 @@ struct StreamScope {
   ~StreamScope() { b->st = saved; }
 };
 
+// Single-pass sequential EKS of `prob` on batch `b` (pof/solver.py:76-96):
+// one sequential filter (EKF linearised at the predicted means) + smoother
+// iteration, factors scaled by sqrt(ssq) when calibrate, outputs E0 (m, L)
+// and P (m, L).  Runs to completion.
+int eks_run(pof_batch* b, const pof_problem* prob, const Setup& s, int calibrate, pof_out* out) {
+  pof_opts o{};
+  o.maxiters = 1;
+  o.fixed_iters = 1;
+  o.calibrate = calibrate;
+  o.sequential = 1;
+  // the EKF does not read the initial means; zeros feed the convergence partials
+  int rc = admit_raw(b, s.Ns, s.P.data(), s.x0m.data(), s.x0L.data(), prob->kind, prob->params, nullptr, nullptr, &o,
+                     out, 2, 2, true, nullptr);
+  if (rc != POF_OK) return rc;
+  rc = run_to_completion(b);
+  out->iterations = 1.0;
+  return rc;
+}
+
+// Coarse initial trajectory (initialization.py:112-129 with convenience.py:80-83):
+// EKS on N_coarse = 100 equispaced points of [ts[0], ts[-1]], full states
+// P (m, L) looked up at idx = floor(ts[k] / coarse_dt) (a negative index counts
+// from the end, out-of-range indices clamp, as a JAX gather), then times PI of
+// the fine grid.  Means only: the solve reads nothing else of the initial
+// trajectory.  Runs on the context's own small batch, so `prob` may be
+// admitted to a batch with IVPs in flight.
+int coarse_means(pof_ctx* ctx, const pof_problem* prob, const Setup& fine, std::vector<double>& m) {
+  constexpr int kCoarse = 100;
+  const int D = fine.D;
+  pof_batch* b = nullptr;
+  for (pof_batch* x : ctx->aux)
+    if (x->D == D && x->d == fine.d) b = x;
+  if (!b) {
+    int rc = batch_new(ctx, D, fine.d, entries_for(kCoarse), 1, false, &b);
+    if (rc != POF_OK) return rc;
+    ctx->aux.push_back(b);
+  }
+  const double t0 = prob->ts_host[0], t1 = prob->ts_host[prob->n_times - 1];
+  std::vector<double> cts(kCoarse);
+  // jnp.linspace (endpoint): t0 (1 - k/div) + t1 k/div for k < div = N - 1, then t1
+  const int div = kCoarse - 1;
+  for (int k = 0; k < div; ++k) {
+    const double st = (double)k / (double)div;
+    const double w0 = t0 * (1.0 - st);  // two rounded products, then the sum (no fused multiply-add)
+    const double w1 = t1 * st;
+    cts[k] = w0 + w1;
+  }
+  cts[div] = t1;
+  pof_problem cp = *prob;
+  cp.init = POF_INIT_CONSTANT;
+  cp.n_times = kCoarse;
+  cp.ts_host = cts.data();
+  Setup cs;
+  int rc = build_setup(&cp, cs, ctx->err);
+  if (rc != POF_OK) return rc;
+  double* full = nullptr;
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipMalloc((void**)&full, (size_t)kCoarse * D * sizeof(double)));
+  pof_out po{};
+  po.full_means_dev = full;
+  rc = eks_run(b, &cp, cs, 1, &po);
+  std::vector<double> cm((size_t)kCoarse * D);
+  hipError_t he = hipSuccess;
+  if (rc == POF_OK) he = hipMemcpy(cm.data(), full, cm.size() * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(full);
+  if (rc != POF_OK) return rc;
+  if (he != hipSuccess) return set_err(ctx, POF_E_HIP, "coarse means copy: %s", hipGetErrorString(he));
+  const double cdt = cts[1] - cts[0];
+  const size_t N = (size_t)fine.Ns + 1;
+  m.assign(N * D, 0.0);
+  for (size_t k = 0; k < N; ++k) {
+    long long idx = (long long)std::floor(prob->ts_host[k] / cdt);
+    if (idx < 0) idx += kCoarse;
+    idx = std::min<long long>(std::max<long long>(idx, 0), kCoarse - 1);
+    for (int i = 0; i < D; ++i) m[k * D + i] = fine.PI[i] * cm[(size_t)idx * D + i];
+  }
+  return POF_OK;
+}
+
 int admit_problem(pof_batch* b, const pof_problem* prob, const pof_opts* opts, pof_out* out, long long* ticket) {
   pof_ctx* ctx = b->ctx;
   Setup s;
@@ -1027,10 +1137,10 @@ int admit_problem(pof_batch* b, const pof_problem* prob, const pof_opts* opts, p
   if (rc != POF_OK) return rc;
   if (s.D != b->D || s.d != b->d) return set_err(ctx, POF_E_ARG, "problem (D=%d, d=%d) does not match the batch", s.D, s.d);
   std::vector<double> m0;
-  rc = initial_means(prob, s, m0, ctx->err);
+  rc = (prob->init == POF_INIT_COARSE) ? coarse_means(ctx, prob, s, m0) : initial_means(prob, s, m0, ctx->err);
   if (rc != POF_OK) return rc;
   return admit_raw(b, s.Ns, s.P.data(), s.x0m.data(), s.x0L.data(), prob->kind, prob->params, &m0, nullptr, opts, out,
-                   false, ticket);
+                   0, opts->sequential ? 1 : 0, false, ticket);
 }
 
 }  // namespace
@@ -1055,6 +1165,8 @@ void pof_ctx_destroy(pof_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   for (pof_batch* b : ctx->cache) batch_free(b);
   ctx->cache.clear();
+  for (pof_batch* b : ctx->aux) batch_free(b);
+  ctx->aux.clear();
   if (ctx->timer.ready)
     for (int i = 0; i <= pof::kMaxTimed; ++i) (void)hipEventDestroy(ctx->timer.ev[i]);
   delete ctx;
@@ -1183,8 +1295,34 @@ int pof_solve(pof_ctx* ctx, const pof_problem* prob, const pof_opts* opts, pof_o
   return run_to_completion(b);
 }
 
-int pof_ieks_step(pof_ctx* ctx, const pof_problem* prob, const double* states_mean_dev, double* out_mean_dev,
-                  double* out_chol_dev, double* nll, double* obj, double* ssq, void* stream) {
+int pof_coarse_means(pof_ctx* ctx, const pof_problem* prob, double* means_host) {
+  if (!ctx || !prob || !means_host) return set_err(ctx, POF_E_ARG, "null argument");
+  pof::Setup s;
+  int rc = pof::build_setup(prob, s, ctx->err);
+  if (rc != POF_OK) return rc;
+  if (!engine_or_err(ctx, s.D, s.d)) return POF_E_UNSUPPORTED;
+  std::vector<double> m;
+  rc = coarse_means(ctx, prob, s, m);
+  if (rc != POF_OK) return rc;
+  std::copy(m.begin(), m.end(), means_host);
+  return POF_OK;
+}
+
+int pof_eks_solve(pof_ctx* ctx, const pof_problem* prob, int calibrate, pof_out* out, void* stream) {
+  if (!ctx || !prob || !out) return set_err(ctx, POF_E_ARG, "null argument");
+  pof::Setup s;
+  int rc = pof::build_setup(prob, s, ctx->err);
+  if (rc != POF_OK) return rc;
+  if (!engine_or_err(ctx, s.D, s.d)) return POF_E_UNSUPPORTED;
+  pof_batch* b = nullptr;
+  rc = cached_batch(ctx, s.D, s.d, entries_for(prob->n_times), 1, &b);
+  if (rc != POF_OK) return rc;
+  StreamScope scope(b, stream);
+  return eks_run(b, prob, s, calibrate, out);
+}
+
+int pof_ieks_step(pof_ctx* ctx, const pof_problem* prob, const double* states_mean_dev, int sequential,
+                  double* out_mean_dev, double* out_chol_dev, double* nll, double* obj, double* ssq, void* stream) {
   if (!ctx || !states_mean_dev) return set_err(ctx, POF_E_ARG, "null argument");
   pof::Setup s;
   int rc = pof::build_setup(prob, s, ctx->err);
@@ -1201,8 +1339,9 @@ int pof_ieks_step(pof_ctx* ctx, const pof_problem* prob, const double* states_me
   pof_out po{};
   po.full_means_dev = out_mean_dev;
   po.full_chols_dev = out_chol_dev;
+  o.sequential = sequential ? 1 : 0;
   rc = admit_raw(b, s.Ns, s.P.data(), s.x0m.data(), s.x0L.data(), prob->kind, prob->params, nullptr,
-                 states_mean_dev, &o, &po, false, nullptr);
+                 states_mean_dev, &o, &po, 0, o.sequential, false, nullptr);
   if (rc != POF_OK) return rc;
   rc = run_to_completion(b);
   if (rc != POF_OK) return rc;
@@ -1214,7 +1353,7 @@ int pof_ieks_step(pof_ctx* ctx, const pof_problem* prob, const double* states_me
 
 int pof_linear_filtsmooth(pof_ctx* ctx, int D, int d, int64_t Ns, const double* x0_mean_host,
                           const double* x0_chol_host, const double* F_host, const double* QL_host,
-                          const double* H_dev, const double* c_dev, double* smooth_mean_dev,
+                          const double* H_dev, const double* c_dev, int sequential, double* smooth_mean_dev,
                           double* smooth_chol_dev, double* filt_mean_dev, double* filt_chol_dev, double* nll,
                           double* obj, double* ssq, void* stream) {
   if (!ctx) return POF_E_ARG;
@@ -1250,8 +1389,9 @@ int pof_linear_filtsmooth(pof_ctx* ctx, int D, int d, int64_t Ns, const double* 
   po.full_means_dev = smooth_mean_dev;
   po.full_chols_dev = smooth_chol_dev;
   // the means buffer only feeds the convergence partials here; it starts at zero
-  rc = admit_raw(b, (int)Ns, P.data(), x0_mean_host, x0_chol_host, 0, nullptr, nullptr, nullptr, &o, &po, true,
-                 nullptr);
+  o.sequential = sequential ? 1 : 0;
+  rc = admit_raw(b, (int)Ns, P.data(), x0_mean_host, x0_chol_host, 0, nullptr, nullptr, nullptr, &o, &po, 1,
+                 o.sequential, false, nullptr);
   if (rc != POF_OK) return rc;
   rc = batch_launch(b);
   if (rc != POF_OK) return rc;

@@ -63,13 +63,18 @@ struct Lay {
 
 // LDS layout kinds: how many tiles / broadcast vectors a worker of each
 // kernel needs and whether the block stages the model constants F, LQ, N1z.
-enum LdsKind : int { kLdsFull = 0, kLdsState, kLdsSmooth, kLdsLeaf, kLdsPost };
+enum LdsKind : int { kLdsFull = 0, kLdsState, kLdsSmooth, kLdsLeaf, kLdsPost, kLdsSeq };
 
 template <int K>
 struct LdsNeeds {
-  static constexpr int NT = (K == kLdsFull) ? 6 : (K == kLdsState || K == kLdsLeaf) ? 4 : (K == kLdsPost) ? 5 : 2;
-  static constexpr int NV = (K == kLdsFull) ? 8 : (K == kLdsState) ? 4 : (K == kLdsPost) ? 2 : 1;
-  static constexpr bool CONSTS = (K == kLdsLeaf || K == kLdsPost);
+  static constexpr int NT = (K == kLdsFull) ? 6
+                            : (K == kLdsState || K == kLdsLeaf) ? 4
+                            : (K == kLdsPost || K == kLdsSeq) ? 5
+                                                              : 2;
+  static constexpr int NV = (K == kLdsFull || K == kLdsState || K == kLdsSeq) ? ((K == kLdsFull) ? 8 : 4)
+                            : (K == kLdsPost) ? 2
+                                              : 1;
+  static constexpr bool CONSTS = (K == kLdsLeaf || K == kLdsPost || K == kLdsSeq);
 };
 
 // Row split per kernel kind: the scan combines use the widest split (their
@@ -77,7 +82,7 @@ struct LdsNeeds {
 // of a row repeats, keep one lane per row and two workers per wavefront.
 template <int D, int K>
 constexpr int kind_split() {
-  return (K == kLdsLeaf || K == kLdsPost) ? 1 : split_of<D>();
+  return (K == kLdsLeaf || K == kLdsPost || K == kLdsSeq) ? 1 : split_of<D>();
 }
 
 template <int D_, int K = kLdsFull>
@@ -614,8 +619,9 @@ struct Obs<D, d, true> {
   double hd[d];
   double c[d];
 
-  __device__ __forceinline__ void load(const ModelArgs& a, int k) {
-    const double* m = a.lin + (size_t)(k + 1) * D;
+  __device__ __forceinline__ void load(const ModelArgs& a, int k) { load_at(a, a.lin + (size_t)(k + 1) * D); }
+  // linearise at the (preconditioned) state mean m
+  __device__ __forceinline__ void load_at(const ModelArgs& a, const double* m) {
     double y[d];
 #pragma unroll
     for (int i = 0; i < d; ++i) y[i] = a.Pd[i * q1] * m[i * q1];

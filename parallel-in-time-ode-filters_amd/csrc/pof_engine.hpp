@@ -67,6 +67,10 @@ struct EngineFns {
   hipError_t (*output)(const GroupArgs& g, const LaunchPlan& p, hipStream_t s);
   // filtered states of IVP ivp[0] (x0, then the level-0 filter prefixes)
   hipError_t (*filtered)(const GroupArgs& g, const LaunchPlan& p, double* fm, double* fL, hipStream_t s);
+  // sequential filter + smoother iteration (at_pred: the EKF/EKS single pass)
+  hipError_t (*seq_iteration)(const GroupArgs& g, const LaunchPlan& p, double* scal, int at_pred, hipStream_t s,
+                              KernelTimer* t);
+  hipError_t (*seq_smooth_chol)(const GroupArgs& g, const LaunchPlan& p, hipStream_t s, KernelTimer* t);
   // EK1 linearisation at n points: H [n, d, D], c [n, d]
   hipError_t (*linearize)(const ModelArgs& a, const double* m, int n, double* H, double* c, hipStream_t s);
 };

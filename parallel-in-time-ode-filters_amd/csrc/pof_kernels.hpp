@@ -506,6 +506,7 @@ __global__ void k_output(GroupArgs g, LaunchMap lm) {
   double* chol_out = g.out_chols[j];
   double* full_means = g.out_full_means[j];
   double* full_chols = g.out_full_chols[j];
+  const bool pfull = g.out_pfull[j] != 0;
   const double* m = spre0.m + (size_t)(Ns - t) * D;
   const double* L = spre0.L + (size_t)(Ns - t) * D * D;
 #pragma unroll
@@ -519,11 +520,15 @@ __global__ void k_output(GroupArgs g, LaunchMap lm) {
   }
   if (full_means) {
 #pragma unroll
-    for (int i = 0; i < D; ++i) full_means[(size_t)t * D + i] = m[i];
+    for (int i = 0; i < D; ++i) full_means[(size_t)t * D + i] = pfull ? Pd[i] * m[i] : m[i];
   }
   if (full_chols) {
 #pragma unroll
-    for (int i = 0; i < D * D; ++i) full_chols[(size_t)t * D * D + i] = scale * L[i];
+    for (int i = 0; i < D; ++i) {
+      const double f = pfull ? scale * Pd[i] : scale;
+#pragma unroll
+      for (int c = 0; c < D; ++c) full_chols[((size_t)t * D + i) * D + c] = f * L[i * D + c];
+    }
   }
 }
 
@@ -565,6 +570,255 @@ __global__ void k_linearize(ModelArgs a, const double* m, int n, double* H, doub
 #pragma unroll
     for (int j = 0; j < D; ++j) H[((size_t)k * d + i) * D + j] = Hk[i][j];
     c[(size_t)k * d + i] = ck[i];
+  }
+}
+
+// --------------------------------------------------------------- sequential
+// Sequential square-root Kalman filter, one IVP per worker, one lane per row
+// (pof/sequential_filtsmooth/filter.py:9-92): for k = 0..Ns-1, predict with
+// (F, QL) (_sqrt_predict, 59-66), linearise (EK1 at lin[k+1]; at the
+// predicted mean when AT_PRED, the EKF of sequential_eks_solve, filter.py:9-30;
+// or the explicit affine model), update (_sqrt_update, 69-92).  Filtered state
+// k+1 goes to level 0 of the filter prefix arrays (entry k), the step's
+// |I^-T y|^2 and log-likelihood to t_ssq[k] and t_nll[k].
+template <int D, int d, bool EK1, bool AT_PRED>
+__global__ __launch_bounds__((Geo<D, kLdsSeq>::BLOCK)) void k_seq_filter(GroupArgs g, LaunchMap lm) {
+  extern __shared__ double smem[];
+  auto cx = make_ctx<D, kLdsSeq>(g.F, g.LQ, g.N1z, smem);
+  constexpr int LD = Lay<D, 1>::LD;
+  int j, i0;
+  if (!locate(lm, cx.wid, j, i0)) return;
+  const ModelArgs a = view<D>(g, j);
+  const long long base = g.base[j];
+  const StateArrays fp = at(g.fpre, base, D);
+  double* t_ssq = g.t_ssq + base;
+  double* t_nll = g.t_nll + base;
+  const int Ns = a.Ns;
+  const int r = cx.row;
+  const bool act = r < D;
+  const int rr = act ? r : 0;
+  double* TL = cx.tile(0);  // L_k, then L_{k+1}
+  double* TP = cx.tile(1);  // predicted factor
+  double* TI = cx.tile(2);  // I = S[:d, :d]
+  double* TG = cx.tile(3);  // G = S[d:, :d]
+  double* Vm = cx.vec(0);   // m_k
+  double* Vp = cx.vec(1);   // predicted mean
+  if (act) {
+#pragma unroll
+    for (int c = 0; c < D; ++c) TL[r * LD + c] = a.x0L[r * D + c];
+    Vm[r] = a.x0m[r];
+  }
+  wsync();
+  for (int k = 0; k < Ns; ++k) {
+    // predict: mp = F m, Lp = tria([F L | QL])
+    {
+      double fr[D];
+      load_row(fr, cx.sF + rr * LD);
+      double fl[D];
+      row_times_tile<1, D, D, LD>(fr, TL, 0, fl);
+      double mp = 0.0;
+#pragma unroll
+      for (int c = 0; c < D; ++c) mp = fma(fr[c], Vm[c], mp);
+      double row[2 * D];
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        row[c] = act ? fl[c] : 0.0;
+        row[D + c] = act ? cx.sLQ[rr * LD + c] : 0.0;
+      }
+      lq<1, D, D, D>(row, r, 0, cx.bc());
+      if (act) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) TP[r * LD + c] = row[c];
+        Vp[r] = mp;
+      }
+    }
+    wsync();
+    Obs<D, d, EK1> ob;
+    if constexpr (AT_PRED) {
+      ob.load_at(a, Vp);
+    } else {
+      ob.load(a, k);
+    }
+    // update: S = tria([[H Lp, R], [Lp, 0]]), R = 0; column blocks (D, d)
+    double psi[D + d];
+    {
+      double hn[D];
+      ob.template row_times_tile<1, D, LD>(r, TP, 0, hn);
+      const int rn = (r >= d && r < d + D) ? r - d : 0;
+#pragma unroll
+      for (int c = 0; c < D; ++c) psi[c] = (r < d) ? hn[c] : ((r < d + D) ? TP[rn * LD + c] : 0.0);
+#pragma unroll
+      for (int c = 0; c < d; ++c) psi[D + c] = 0.0;
+    }
+    lq<1, d + D, D, d>(psi, r, 0, cx.bc());
+    if (r < d) {
+#pragma unroll
+      for (int c = 0; c < d; ++c) TI[r * LD + c] = psi[c];
+    } else if (r < d + D) {
+#pragma unroll
+      for (int c = 0; c < d; ++c) TG[(r - d) * LD + c] = psi[c];
+#pragma unroll
+      for (int c = d; c < D + d; ++c) TL[(r - d) * LD + (c - d)] = psi[c];
+    }
+    wsync();
+    // y = 0 - (H mp + c); m_{k+1} = mp + G I^-1 y
+    double z[d], w[d];
+#pragma unroll
+    for (int i = 0; i < d; ++i) {
+      z[i] = -(ob.c[i] + ob.dot(i, Vp, 1));
+      w[i] = z[i];
+    }
+    lsolve_lower<d, LD>(z, TI);   // I z = y
+    lsolve_lowerT<d, LD>(w, TI);  // I^T w = y (whiten)
+    double mn = Vp[rr];
+#pragma unroll
+    for (int i = 0; i < d; ++i) mn = fma(TG[rr * LD + i], z[i], mn);
+    wsync();
+    if (act) {
+      Vm[r] = mn;
+      fp.m[(size_t)k * D + r] = mn;
+#pragma unroll
+      for (int c = 0; c < D; ++c) fp.L[((size_t)k * D + r) * D + c] = TL[r * LD + c];
+    }
+    if (r == 0) {
+      double ss = 0.0, nn = 0.0, ld = 0.0;
+#pragma unroll
+      for (int i = 0; i < d; ++i) {
+        ss = fma(w[i], w[i], ss);
+        nn = fma(z[i], z[i], nn);
+        ld += log(fabs(TI[i * LD + i]));
+      }
+      t_ssq[k] = ss;
+      // mvn_loglikelihood (utils.py:22-30): -|I^-1 y|^2 / 2 - sum log|I_ii| - d log(2 pi) / 2
+      t_nll[k] = -0.5 * nn - (ld + d * log(2.0 * M_PI) / 2.0);
+    }
+    wsync();
+  }
+}
+
+// Sequential square-root RTS smoother, one IVP per worker
+// (pof/sequential_filtsmooth/smoother.py:8-48, _sqrt_smooth 31-48), backward
+// over the filtered states of k_seq_filter.  Smoothed state t goes to level 0
+// of the smoother prefix arrays at entry Ns - t (the parallel smoother's
+// layout, read by k_smooth_post and k_output); factors only when wchol is set.
+template <int D>
+__global__ __launch_bounds__((Geo<D, kLdsSeq>::BLOCK)) void k_seq_smooth(GroupArgs g, LaunchMap lm) {
+  extern __shared__ double smem[];
+  auto cx = make_ctx<D, kLdsSeq>(g.F, g.LQ, g.N1z, smem);
+  constexpr int LD = Lay<D, 1>::LD;
+  int j, i0;
+  if (!locate(lm, cx.wid, j, i0)) return;
+  const long long base = g.base[j];
+  const int Ns = g.Ns[j];
+  const bool wc = g.wchol[j] != 0;
+  const StateArrays fp = at(g.fpre, base, D);
+  const StateArrays sp = at(g.spre, base, D);
+  const double* x0m = g.x0m + (size_t)j * D;
+  const double* x0L = g.x0L + (size_t)j * D * D;
+  const int r = cx.row;
+  const bool act = r < D;
+  const int rr = act ? r : 0;
+  double* TLs = cx.tile(0);  // smoothed factor of state k+1
+  double* TP11 = cx.tile(1);
+  double* TE = cx.tile(2);   // gain
+  double* TD = cx.tile(3);   // Phi22
+  double* TF = cx.tile(4);   // filtered factor of state k
+  double* Vs = cx.vec(0);    // smoothed mean of state k+1
+  double* Vf = cx.vec(1);    // filtered mean of state k
+  double* Vfm = cx.vec(2);   // F m_f
+  // state Ns: smoothed = filtered
+  {
+    const double* m = (Ns == 0) ? x0m : fp.m + (size_t)(Ns - 1) * D;
+    const double* L = (Ns == 0) ? x0L : fp.L + (size_t)(Ns - 1) * D * D;
+    if (act) {
+      const double mv = m[r];
+      Vs[r] = mv;
+      sp.m[r] = mv;
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        const double v = L[r * D + c];
+        TLs[r * LD + c] = v;
+        if (wc) sp.L[r * D + c] = v;
+      }
+    }
+  }
+  wsync();
+  for (int k = Ns - 1; k >= 0; --k) {
+    const double* mf = (k == 0) ? x0m : fp.m + (size_t)(k - 1) * D;
+    const double* Lf = (k == 0) ? x0L : fp.L + (size_t)(k - 1) * D * D;
+    if (act) {
+      Vf[r] = mf[r];
+#pragma unroll
+      for (int c = 0; c < D; ++c) TF[r * LD + c] = Lf[r * D + c];
+    }
+    wsync();
+    // Phi = tria([[F Lf, QL], [Lf, 0]])
+    double ph[2 * D];
+    {
+      double fr[D];
+      load_row(fr, cx.sF + rr * LD);
+      double t1[D];
+      row_times_tile<1, D, D, LD>(fr, TF, 0, t1);
+      double fm = 0.0;
+#pragma unroll
+      for (int c = 0; c < D; ++c) fm = fma(fr[c], Vf[c], fm);
+      if (act) Vfm[r] = fm;
+      const int r2 = (r >= D && r < 2 * D) ? r - D : 0;
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        ph[c] = act ? t1[c] : ((r < 2 * D) ? TF[r2 * LD + c] : 0.0);
+        ph[D + c] = act ? cx.sLQ[rr * LD + c] : 0.0;
+      }
+    }
+    lq<1, 2 * D, D, D>(ph, r, 0, cx.bc());
+    if (act) {
+#pragma unroll
+      for (int c = 0; c < D; ++c) TP11[r * LD + c] = ph[c];
+    }
+    wsync();
+    // gain = Phi21 Phi11^-1, Phi22 (rows D..2D-1 move to r - D)
+    if (r >= D && r < 2 * D) {
+      double e[D];
+#pragma unroll
+      for (int c = 0; c < D; ++c) e[c] = ph[c];
+      rsolve_lower<D, LD>(e, TP11);
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        TE[(r - D) * LD + c] = e[c];
+        TD[(r - D) * LD + c] = ph[D + c];
+      }
+    }
+    wsync();
+    // mean = mf + gain (ms - F mf); chol = tria([Phi22 | gain Ls])
+    double mean = Vf[rr];
+#pragma unroll
+    for (int c = 0; c < D; ++c) mean = fma(TE[rr * LD + c], Vs[c] - Vfm[c], mean);
+    double cr[2 * D];
+    if (wc) {
+      double ge[D];
+      load_row(ge, TE + rr * LD);
+      double t1[D];
+      row_times_tile<1, D, D, LD>(ge, TLs, 0, t1);
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        cr[c] = act ? TD[rr * LD + c] : 0.0;
+        cr[D + c] = act ? t1[c] : 0.0;
+      }
+      lq<1, D, D, D>(cr, r, 0, cx.bc());
+    }
+    wsync();
+    if (act) {
+      Vs[r] = mean;
+      sp.m[(size_t)(Ns - k) * D + r] = mean;
+      if (wc) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+          TLs[r * LD + c] = cr[c];
+          sp.L[((size_t)(Ns - k) * D + r) * D + c] = cr[c];
+        }
+      }
+    }
+    wsync();
   }
 }
 
@@ -665,6 +919,43 @@ hipError_t run_iteration(const GroupArgs& g, const LaunchPlan& p, double* scal, 
   hipLaunchKernelGGL(k_reduce_part, dim3(p.chunk_total), dim3(256), 0, st, g, p.map(p.chunk_off, p.chunk_total));
   hipLaunchKernelGGL(k_reduce_final, dim3(p.m), dim3(64), 0, st, g, p.map(p.chunk_off, p.chunk_total), d, scal);
   rec(kKReduce, p.leaf_total);
+  return hipGetLastError();
+}
+
+// One sequential IEKS iteration (pof/step.py:33-45 with sequential=True) or,
+// with at_pred, the single pass of sequential_eks_solve (pof/solver.py:76-96):
+// sequential filter and smoother, then the smoothed means, objective,
+// convergence partials and per-IVP reduction of run_iteration.
+template <int D, int d>
+hipError_t run_seq_iteration(const GroupArgs& g, const LaunchPlan& p, double* scal, int at_pred, hipStream_t st,
+                             KernelTimer* t) {
+  Rec rec(t, st);
+  const LaunchMap one = p.map(p.one_off, p.one_total);
+  if (g.affine) {
+    POF_LAUNCH((k_seq_filter<D, d, false, false>), kLdsSeq, p.one_total, st, g, one);
+  } else if (at_pred) {
+    POF_LAUNCH((k_seq_filter<D, d, true, true>), kLdsSeq, p.one_total, st, g, one);
+  } else {
+    POF_LAUNCH((k_seq_filter<D, d, true, false>), kLdsSeq, p.one_total, st, g, one);
+  }
+  rec(kKFilterLeaf, p.leaf_total);
+  POF_LAUNCH((k_seq_smooth<D>), kLdsSeq, p.one_total, st, g, one);
+  rec(kKSmoothUp, p.leaf_total);
+  hipLaunchKernelGGL((k_smooth_post<D>), dim3((p.pts_total + 127) / 128), dim3(128), 0, st, g,
+                     p.map(p.pts_off, p.pts_total));
+  rec(kKSmoothPost, p.pts_total);
+  hipLaunchKernelGGL(k_reduce_part, dim3(p.chunk_total), dim3(256), 0, st, g, p.map(p.chunk_off, p.chunk_total));
+  hipLaunchKernelGGL(k_reduce_final, dim3(p.m), dim3(64), 0, st, g, p.map(p.chunk_off, p.chunk_total), d, scal);
+  rec(kKReduce, p.leaf_total);
+  return hipGetLastError();
+}
+
+// Smoothed factors of the last sequential iteration (after means-only passes).
+template <int D, int d>
+hipError_t run_seq_smooth_chol(const GroupArgs& g, const LaunchPlan& p, hipStream_t st, KernelTimer* t) {
+  Rec rec(t, st);
+  POF_LAUNCH((k_seq_smooth<D>), kLdsSeq, p.one_total, st, g, p.map(p.one_off, p.one_total));
+  rec(kKSmoothUp, p.leaf_total);
   return hipGetLastError();
 }
 

@@ -77,6 +77,7 @@ struct GroupArgs {
   double* const* out_full_means;
   double* const* out_full_chols;
   const double* out_scale;    // [slots] factor scale of the output
+  const int* out_pfull;       // [slots] 1: full-state outputs are P (m, L) instead of (m, L)
   const double* Hx;           // affine mode (one IVP at entry 0): [Ns, d, D]
   const double* cx;           // [Ns, d]
   int q;

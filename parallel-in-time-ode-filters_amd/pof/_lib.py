@@ -34,6 +34,7 @@ KINDS = {
 }
 INIT_PRIOR = 0
 INIT_CONSTANT = 1
+INIT_COARSE = 2
 MAX_PARAMS = 8
 STATUS = {0: "converged", 1: "maxiter", 2: "nan", 3: "fixed"}
 KERNEL_CLASSES = ["filter_leaf", "filter_up", "filter_down", "filter_post", "smooth_up", "smooth_down",
@@ -58,7 +59,7 @@ class Opts(ctypes.Structure):
         ("maxiters", ctypes.c_int32),
         ("fixed_iters", ctypes.c_int32),
         ("calibrate", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("sequential", ctypes.c_int32),
     ]
 
 
@@ -94,13 +95,15 @@ SIGNATURES = {
     "pof_setup": (ctypes.c_int, [_P(Problem), _dp, _dp, _dp, _dp, _dp, _dp]),
     "pof_transitions": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _d, _dp, _dp, _dp, _dp]),
     "pof_initial_means": (ctypes.c_int, [_P(Problem), _dp]),
+    "pof_coarse_means": (ctypes.c_int, [_vp, _P(Problem), _dp]),
+    "pof_eks_solve": (ctypes.c_int, [_vp, _P(Problem), ctypes.c_int, _P(Out), _vp]),
     "pof_eval_field": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]),
     "pof_linearize": (ctypes.c_int, [_vp, _P(Problem), _vp, ctypes.c_int64, _vp, _vp, _vp]),
-    "pof_ieks_step": (ctypes.c_int, [_vp, _P(Problem), _vp, _vp, _vp, _dp, _dp, _dp, _vp]),
+    "pof_ieks_step": (ctypes.c_int, [_vp, _P(Problem), _vp, ctypes.c_int, _vp, _vp, _dp, _dp, _dp, _vp]),
     "pof_linear_filtsmooth": (
         ctypes.c_int,
-        [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int64, _dp, _dp, _dp, _dp, _vp, _vp, _vp, _vp, _vp, _vp,
-         _dp, _dp, _dp, _vp],
+        [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int64, _dp, _dp, _dp, _dp, _vp, _vp, ctypes.c_int, _vp, _vp,
+         _vp, _vp, _dp, _dp, _dp, _vp],
     ),
     "pof_batch_entries_for": (ctypes.c_int64, [ctypes.c_int64]),
     "pof_batch_bytes": (ctypes.c_int64, [ctypes.c_int, ctypes.c_int64, ctypes.c_int32]),

@@ -9,7 +9,7 @@ from .observations import AffineModel, NonlinearModel, linearize
 from .transitions import IWP, TransitionModel, discretize_transitions, projection_matrix
 from .utils import MVNSqrt, as_f64_device, as_f64_host
 
-INIT_METHODS = {"prior": _lib.INIT_PRIOR, "constant": _lib.INIT_CONSTANT}
+INIT_METHODS = {"prior": _lib.INIT_PRIOR, "constant": _lib.INIT_CONSTANT, "coarse": _lib.INIT_COARSE}
 
 
 class ProblemSpec:
@@ -97,17 +97,23 @@ def set_up_solver(*, f, y0, ts, order):
 def get_initial_trajectory(setup, method="prior"):
     """(``convenience.py:76-92``): device means ``[len(ts), D]`` and zero factors.
 
-    ``constant`` is multiplied by ``PI``; ``prior`` is returned without
-    preconditioning, as in the reference.  Only the means enter the IEKS.
+    ``constant`` and ``coarse`` are multiplied by ``PI``; ``prior`` is returned
+    without preconditioning, as in the reference.  ``coarse`` runs the
+    sequential EKS on 100 points on the GPU.  Only the means enter the IEKS.
     """
+    import torch
+
     spec = setup["spec"]
     if method not in INIT_METHODS:
         raise Exception(f"method={method} not found")
     spec = ProblemSpec(spec.f, spec.y0, spec.ts, spec.order, init=method)
     m = np.empty((spec.ts.shape[0], spec.D))
     prob = spec.cstruct()
-    _lib.check(_lib.load().pof_initial_means(ctypes.byref(prob), _lib.hptr(m)), what="pof_initial_means")
-    import torch
+    if method == "coarse":
+        ctx = _lib.context(torch.cuda.current_device())
+        _lib.check(_lib.load().pof_coarse_means(ctx, ctypes.byref(prob), _lib.hptr(m)), ctx, "pof_coarse_means")
+    else:
+        _lib.check(_lib.load().pof_initial_means(ctypes.byref(prob), _lib.hptr(m)), what="pof_initial_means")
 
     mean = as_f64_device(m)
     return MVNSqrt(mean, torch.zeros(mean.shape + (spec.D,), dtype=torch.float64, device=mean.device))

@@ -55,6 +55,30 @@ def constant_init(*, y0, order, ts, f):
     return MVNSqrt(mean, torch.zeros(mean.shape + (spec.D,), dtype=torch.float64, device=mean.device))
 
 
+def coarse_ekf_init(*, f, y0, order, ts, N=100):
+    """Sequential EKS on ``N = 100`` equispaced points, looked up at ``floor(ts / coarse_dt)``
+    (``initialization.py:112-129``); returns ``P``-scaled means (not ``PI``-multiplied), zero factors.
+
+    The reference's default is ``N=10`` but its only caller passes ``N=100``
+    (``convenience.py:81``); that is the size the device path implements.
+    """
+    import torch
+
+    if N != 100:
+        raise NotImplementedError("the coarse EKS runs on N = 100 points (convenience.py:81)")
+    spec = ProblemSpec(f, y0, ts, order, init="coarse")
+    m = np.empty((spec.ts.shape[0], spec.D))
+    prob = spec.cstruct()
+    ctx = _lib.context(torch.cuda.current_device())
+    _lib.check(_lib.load().pof_coarse_means(ctx, ctypes.byref(prob), _lib.hptr(m)), ctx, "pof_coarse_means")
+    from .transitions import IWP, nordsieck_preconditioner
+
+    _, PI = nordsieck_preconditioner(IWP(spec.d, spec.order), spec.ts[1] - spec.ts[0])
+    m = m / np.diag(PI)[None, :]
+    mean = as_f64_device(m)
+    return MVNSqrt(mean, torch.zeros(mean.shape + (spec.D,), dtype=torch.float64, device=mean.device))
+
+
 def prior_init(*, f, y0, order, ts):
     """Prior predictions of x0 to every ``t_k`` with steps ``ts[1:]`` (``initialization.py:83-89``); means only."""
     import torch

@@ -26,7 +26,7 @@ def _constant_transition(dtm):
     return np.ascontiguousarray(F), np.ascontiguousarray(QL)
 
 
-def _run(x0, dtm, dom, want_filter):
+def _run(x0, dtm, dom, want_filter, sequential=False):
     import torch
 
     F, QL = _constant_transition(dtm)
@@ -52,7 +52,7 @@ def _run(x0, dtm, dom, want_filter):
     _lib.check(
         _lib.load().pof_linear_filtsmooth(
             ctx, D, d, Ns, _lib.hptr(m0), _lib.hptr(L0), _lib.hptr(F), _lib.hptr(QL), _lib.dptr(H), _lib.dptr(c),
-            _lib.dptr(sm), _lib.dptr(sL), _lib.dptr(fm), _lib.dptr(fL), ctypes.byref(nll), ctypes.byref(obj),
+            1 if sequential else 0, _lib.dptr(sm), _lib.dptr(sL), _lib.dptr(fm), _lib.dptr(fL), ctypes.byref(nll), ctypes.byref(obj),
             ctypes.byref(ssq), _lib.stream_ptr(dev)),
         ctx, "pof_linear_filtsmooth",
     )

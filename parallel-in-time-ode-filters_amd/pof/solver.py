@@ -17,12 +17,11 @@ def solve(*, f, y0, ts, order, init="prior", calibrate=True, maxiters=10_000, se
     ``fixed_iters`` runs exactly that many iterations without the convergence
     test (benchmark mode); ``return_full_states`` also returns the
     preconditioned states in ``info["states"]``.  ``info["status"]`` tells
-    how the loop ended.
+    how the loop ended.  ``sequential=True`` runs each iteration as the
+    sequential filter + smoother (``step.py:39-40``), one GPU worker per IVP.
     """
     import torch
 
-    if sequential:
-        raise NotImplementedError("sequential=True is not implemented on the GPU")
     spec = ProblemSpec(f, y0, ts, order, init)
     N, d, D = spec.ts.shape[0], spec.d, spec.D
     dev = torch.device("cuda", torch.cuda.current_device())
@@ -34,6 +33,7 @@ def solve(*, f, y0, ts, order, init="prior", calibrate=True, maxiters=10_000, se
     opts.maxiters = int(maxiters)
     opts.fixed_iters = int(fixed_iters or 0)
     opts.calibrate = 1 if calibrate else 0
+    opts.sequential = 1 if sequential else 0
     out = _lib.Out()
     out.means_dev = means.data_ptr()
     out.chols_dev = chols.data_ptr()
@@ -59,5 +59,41 @@ def solve(*, f, y0, ts, order, init="prior", calibrate=True, maxiters=10_000, se
 
 
 def sequential_eks_solve(*, f, y0, ts, order, return_full_states=False, calibrate=True):
-    """Single-pass sequential EKS (``pof/solver.py:76-96``): not implemented on the GPU."""
-    raise NotImplementedError("sequential_eks_solve is not implemented on the GPU")
+    """Single-pass sequential EKS (``pof/solver.py:76-96``) on the GPU.
+
+    Square-root EKF linearised at each predicted mean, then the RTS smoother;
+    factors scaled by ``sqrt(sigma_squared)`` when ``calibrate``.  Returns
+    ``(MVNSqrt(E0 m, E0 L), info)``, or ``P (m, L)`` (``[len(ts), D]``,
+    ``[len(ts), D, D]``) with ``return_full_states``.  ``info["nll"]`` is the
+    summed log-likelihood, as in the reference.
+    """
+    import torch
+
+    spec = ProblemSpec(f, y0, ts, order, "constant")
+    N, d, D = spec.ts.shape[0], spec.d, spec.D
+    dev = torch.device("cuda", torch.cuda.current_device())
+    out = _lib.Out()
+    if return_full_states:
+        means = torch.empty((N, D), dtype=torch.float64, device=dev)
+        chols = torch.empty((N, D, D), dtype=torch.float64, device=dev)
+        out.full_means_dev = means.data_ptr()
+        out.full_chols_dev = chols.data_ptr()
+    else:
+        means = torch.empty((N, d), dtype=torch.float64, device=dev)
+        chols = torch.empty((N, d, D), dtype=torch.float64, device=dev)
+        out.means_dev = means.data_ptr()
+        out.chols_dev = chols.data_ptr()
+    prob = spec.cstruct()
+    ctx = _lib.context(dev.index)
+    _lib.check(
+        _lib.load().pof_eks_solve(ctx, ctypes.byref(prob), 1 if calibrate else 0, ctypes.byref(out),
+                                  _lib.stream_ptr(dev)),
+        ctx, "pof_eks_solve",
+    )
+    info = {
+        "nll": float(out.nll),
+        "obj": float(out.obj),
+        "sigma_squared": float(out.sigma_squared),
+        "calibrated": bool(calibrate),
+    }
+    return MVNSqrt(means, chols), info

@@ -13,15 +13,16 @@ def linearize_at_previous_states(om, prev_states):
 
 
 def ieks_step(*, om=None, dtm=None, x0=None, states, calibrate=True, sequential=False, setup=None):
-    """(``step.py:33-45``): linearise at ``states.mean[1:]``, parallel filter + smoother, chol *= sqrt(ssq).
+    """(``step.py:33-45``): linearise at ``states.mean[1:]``, filter + smoother, chol *= sqrt(ssq).
 
-    The native step needs the problem description; pass ``setup=`` (from
-    ``set_up_solver``).  Returns ``(MVNSqrt(mean[N+1, D], chol[N+1, D, D]), nll, obj, ssq)``.
+    Parallel (associative scans) or, with ``sequential=True``, the sequential
+    square-root filter and RTS smoother (``step.py:39-40``; there ``nll`` is
+    the summed log-likelihood, as in the reference).  The native step needs the
+    problem description; pass ``setup=`` (from ``set_up_solver``).  Returns
+    ``(MVNSqrt(mean[N+1, D], chol[N+1, D, D]), nll, obj, ssq)``.
     """
     import torch
 
-    if sequential:
-        raise NotImplementedError("sequential=True is not implemented on the GPU")
     if setup is None:
         raise TypeError("ieks_step needs setup= (the dictionary returned by pof.convenience.set_up_solver)")
     spec = setup["spec"]
@@ -37,7 +38,8 @@ def ieks_step(*, om=None, dtm=None, x0=None, states, calibrate=True, sequential=
     prob = spec.cstruct()
     ctx = _lib.context(m.device.index)
     _lib.check(
-        _lib.load().pof_ieks_step(ctx, ctypes.byref(prob), _lib.dptr(m), _lib.dptr(om_), _lib.dptr(oL),
+        _lib.load().pof_ieks_step(ctx, ctypes.byref(prob), _lib.dptr(m), 1 if sequential else 0,
+                                  _lib.dptr(om_), _lib.dptr(oL),
                                   ctypes.byref(nll), ctypes.byref(obj), ctypes.byref(ssq), _lib.stream_ptr(m.device)),
         ctx, "pof_ieks_step",
     )

@@ -321,3 +321,31 @@ def test_maxiters_semantics():
     p = op.fitzhughnagumo()
     ys, info = osol.solve(p, np.linspace(0, 100, 64), 1, init="constant", maxiters=3)
     assert info["iterations"] == 4.0
+
+
+def test_jax_linspace_endpoints_and_rounding():
+    """jnp.linspace's formula: exact endpoints, within two ulps of numpy's."""
+    for a, b, n in ((0.0, 100.0, 100), (1.0, 11.0, 100), (-3.5, 7.25, 17)):
+        x = osol.jax_linspace(a, b, n)
+        y = np.linspace(a, b, n)
+        assert x[0] == a and x[-1] == b and x.shape == (n,)
+        assert np.all(np.abs(x - y) <= 2 * np.spacing(np.maximum(np.abs(x), np.abs(y))))
+    # the two formulas do differ: 0 (1 - k/99) + 100 k/99 vs k (100/99)
+    assert np.any(osol.jax_linspace(0.0, 100.0, 100) != np.linspace(0.0, 100.0, 100))
+
+
+def test_coarse_init_nearest_left_lookup():
+    """coarse_ekf_init picks, for every fine t, the coarse EKS state k = floor(t / coarse_dt)
+    (clamped; a negative index counts from the end): checked by brute force on the coarse grid."""
+    p = op.logistic()
+    N = 100
+    for t0, t1, n in ((0.0, 10.0, 257), (1.0, 11.0, 301)):
+        ts = np.linspace(t0, t1, n)
+        got = osol.coarse_ekf_init(p, 2, ts, N=N)
+        cts = osol.jax_linspace(t0, t1, N)
+        out, _ = osol.sequential_eks_solve(p, cts, 2, return_full_states=True)
+        cdt = cts[1] - cts[0]
+        for k, t in enumerate(ts):
+            j = max(i for i in range(-N, 2 * N) if i * cdt <= t) if t >= 0 else None
+            j = min(max(j, 0), N - 1)
+            np.testing.assert_array_equal(got.mean[k], out.mean[j])
