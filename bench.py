@@ -320,7 +320,7 @@ def main():
             print(json.dumps(batched), file=sys.stderr, flush=True)
         # mean Ns of log-uniform [256, 65536] over [256, 16384]: 3.04x more work per request
         rate = args.serve_rate or 0.7 * batched["solves_per_s"] / 3.04
-        served = served_run(args.serve_requests, rate, maxiters=100)
+        served = served_run(args.serve_requests, rate, maxiters=100) if args.serve_requests > 0 else None
         if args.verbose:
             print(json.dumps(served), file=sys.stderr, flush=True)
     cpu = None if args.no_cpu_baseline else cpu_baseline()

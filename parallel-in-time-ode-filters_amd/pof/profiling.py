@@ -25,7 +25,9 @@ COPY_KERNEL = "__amd_rocclr_copyBuffer"
 
 
 def short_name(name):
-    return name.split("(")[0].replace("void ", "")
+    """Kernel name without ``void`` and the argument list (``(anonymous namespace)::`` kept)."""
+    name = name.replace("void ", "").replace("(anonymous namespace)::", "anon::")
+    return name.split("(")[0]
 
 
 def counter_means(path):
