@@ -156,6 +156,24 @@ int pof_eval_field(int kind, int d, const double* params, const double* y_host, 
 int pof_linearize(pof_ctx* ctx, const pof_problem* prob, const double* m_dev, int64_t n, double* H_dev,
                   double* c_dev, void* stream);
 
+/* Linearisations of the residual r(x) = E1 x - f(E0 x) (method codes) */
+#define POF_LIN_EK1 0          /* observations.py:35-40: H = E1 - J E0, cholR = 0 */
+#define POF_LIN_EK0 1          /* observations.py:52-61: H = E1 of the IWP without the preconditioner, cholR = 0 */
+#define POF_LIN_UNCERTAIN 2    /* observations.py:43-49: EK1 with cholR = tria(H L) */
+#define POF_LIN_REGULARIZED 3  /* observations.py:64-89: H = [J; I], b = [r(m); -m], cholR = blockdiag(0, I/sqrt(lam)) */
+#define POF_LIN_UNSCENTED 4    /* linearization/unscented.py:19-70 */
+
+/* Linearise at n device states (m [n, D], L [n, D, D]; L is read by UNCERTAIN and
+ * UNSCENTED only).  Replaces pof.observations.{linearize, linearize_ek0,
+ * uncertain_linearize, linearize_regularized} and
+ * pof.linearization.unscented.linearize_unscented, one state each, vectorised.
+ * params (may be NULL): {lam, alpha, beta, kappa}; kappa NaN = 3 + D (defaults
+ * lam 1, alpha 1, beta 0).  Outputs H [n, dO, D], c [n, dO], R [n, dO, dO] with
+ * dO = d, or d + D for REGULARIZED. */
+int pof_linearize_model(pof_ctx* ctx, const pof_problem* prob, int method, const double* params,
+                        const double* m_dev, const double* L_dev, int64_t n, double* H_dev, double* c_dev,
+                        double* R_dev, void* stream);
+
 /* One IEKS iteration.  Replaces pof.step.ieks_step (step.py:33-45): linearise at
  * states_dev [Ns+1, D], parallel (sequential = 0) or sequential (1) filter +
  * smoother, chol *= sqrt(ssq).  Outputs full states [Ns+1, D] / [Ns+1, D, D]
@@ -169,12 +187,13 @@ int pof_ieks_step(pof_ctx* ctx, const pof_problem* prob, const double* states_me
  * pof.parallel_filtsmooth.linear_filtsmooth (parallel_filtsmooth/__init__.py:5-10,
  * filter.py:18-47, smoother.py:8-22) and, with sequential = 1,
  * pof.sequential_filtsmooth.linear_filtsmooth (sequential_filtsmooth/__init__.py:13-16).
- * x0 mean [D] / factor [D, D], F/QL [D, D] on the host; H [Ns, d, D], c [Ns, d]
- * and outputs on the device.  filt_* (may be NULL) receive the filtered states
- * [Ns+1, ...]. */
+ * x0 mean [D] / factor [D, D], F/QL [D, D] on the host; H [Ns, d, D], c [Ns, d],
+ * the observation noise factors R [Ns, d, d] (NULL = noiseless) and outputs on
+ * the device.  filt_* (may be NULL) receive the filtered states [Ns+1, ...]. */
 int pof_linear_filtsmooth(pof_ctx* ctx, int D, int d, int64_t Ns, const double* x0_mean_host,
                           const double* x0_chol_host, const double* F_host, const double* QL_host,
-                          const double* H_dev, const double* c_dev, int sequential, double* smooth_mean_dev,
+                          const double* H_dev, const double* c_dev, const double* R_dev, int sequential,
+                          double* smooth_mean_dev,
                           double* smooth_chol_dev, double* filt_mean_dev, double* filt_chol_dev, double* nll,
                           double* obj, double* ssq, void* stream);
 

@@ -1280,6 +1280,49 @@ int pof_linearize(pof_ctx* ctx, const pof_problem* prob, const double* m_dev, in
   return POF_OK;
 }
 
+int pof_linearize_model(pof_ctx* ctx, const pof_problem* prob, int method, const double* params,
+                        const double* m_dev, const double* L_dev, int64_t n, double* H_dev, double* c_dev,
+                        double* R_dev, void* stream) {
+  if (!ctx) return POF_E_ARG;
+  pof::Setup s;
+  int rc = pof::build_setup(prob, s, ctx->err);
+  if (rc != POF_OK) return rc;
+  if (method < POF_LIN_EK1 || method > POF_LIN_UNSCENTED) return set_err(ctx, POF_E_ARG, "unknown method %d", method);
+  const bool needs_L = method == POF_LIN_UNCERTAIN || method == POF_LIN_UNSCENTED;
+  if (n < 0 || n > 0x3fffffff || (n > 0 && (!m_dev || !H_dev || !c_dev || !R_dev || (needs_L && !L_dev))))
+    return set_err(ctx, POF_E_ARG, "bad linearize_model args");
+  pof::LinParams lp{};
+  lp.lam = params ? params[0] : 1.0;
+  lp.alpha = params ? params[1] : 1.0;
+  lp.beta = params ? params[2] : 0.0;
+  lp.kappa = params ? params[3] : 0.0;
+  lp.kappa_set = (params && !std::isnan(params[3])) ? 1 : 0;
+  if (method == POF_LIN_REGULARIZED && !(lp.lam > 0.0)) return set_err(ctx, POF_E_ARG, "lam must be > 0");
+  const pof::EngineFns* e = engine_or_err(ctx, s.D, s.d);
+  if (!e) return POF_E_UNSUPPORTED;
+  if (n == 0) return POF_OK;
+  hipStream_t st = (hipStream_t)stream;
+  HIP_TRY(hipSetDevice(ctx->device));
+  // Pd, then the per-state work slices (uncertain / unscented)
+  const size_t wper = (size_t)pof::lin_work_host(s.D, s.d);
+  const size_t wn = needs_L ? (size_t)n * wper : 0;
+  double* buf = nullptr;
+  HIP_TRY(hipMalloc((void**)&buf, (s.D + wn) * sizeof(double)));
+  hipError_t he = hipMemcpyAsync(buf, s.P.data(), s.D * sizeof(double), hipMemcpyHostToDevice, st);
+  pof::ModelArgs a{};
+  a.Pd = buf;
+  a.kind = prob->kind;
+  a.q = s.q;
+  a.Ns = (int)n;
+  for (int i = 0; i < pof::kMaxParams; ++i) a.params[i] = prob->params[i];
+  if (he == hipSuccess)
+    he = e->linearize_alt(a, method, lp, m_dev, L_dev, (int)n, H_dev, c_dev, R_dev, wn ? buf + s.D : nullptr, st);
+  if (he == hipSuccess) he = hipStreamSynchronize(st);
+  (void)hipFree(buf);
+  if (he != hipSuccess) return set_err(ctx, POF_E_HIP, "linearize_model failed: %s", hipGetErrorString(he));
+  return POF_OK;
+}
+
 int pof_solve(pof_ctx* ctx, const pof_problem* prob, const pof_opts* opts, pof_out* out, void* stream) {
   if (!ctx || !opts || !out) return set_err(ctx, POF_E_ARG, "null argument");
   pof::Setup s;
@@ -1353,7 +1396,8 @@ int pof_ieks_step(pof_ctx* ctx, const pof_problem* prob, const double* states_me
 
 int pof_linear_filtsmooth(pof_ctx* ctx, int D, int d, int64_t Ns, const double* x0_mean_host,
                           const double* x0_chol_host, const double* F_host, const double* QL_host,
-                          const double* H_dev, const double* c_dev, int sequential, double* smooth_mean_dev,
+                          const double* H_dev, const double* c_dev, const double* R_dev, int sequential,
+                          double* smooth_mean_dev,
                           double* smooth_chol_dev, double* filt_mean_dev, double* filt_chol_dev, double* nll,
                           double* obj, double* ssq, void* stream) {
   if (!ctx) return POF_E_ARG;
@@ -1380,6 +1424,7 @@ int pof_linear_filtsmooth(pof_ctx* ctx, int D, int d, int64_t Ns, const double* 
   HIP_TRY(hipMemcpy(b->N1z, N1z.data(), DD * 8, hipMemcpyHostToDevice));
   b->g.Hx = H_dev;
   b->g.cx = c_dev;
+  b->g.Rx = R_dev;
   std::vector<double> P(D, 1.0);
   pof_opts o{};
   o.maxiters = 1;

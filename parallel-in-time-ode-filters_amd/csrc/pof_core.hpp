@@ -640,6 +640,8 @@ struct Obs<D, d, true> {
       c[i] = acc - f[i];
     }
   }
+  // the noise factor R of EK1 is zero (observations.py:39)
+  static __device__ __forceinline__ constexpr double r_at(int, int) { return 0.0; }
   // sum_j H[i][j] v[j st]
   __device__ __forceinline__ double dot(int i, const double* v, int st) const {
     double acc = 0.0;
@@ -727,15 +729,29 @@ template <int D, int d>
 struct Obs<D, d, false> {
   double H[d][D];
   double c[d];
+  double R[d][d];
 
   __device__ __forceinline__ void load(const ModelArgs& a, int k) {
     const double* Hk = a.Hx + (size_t)k * d * D;
+    const double* Rk = a.Rx ? a.Rx + (size_t)k * d * d : nullptr;
 #pragma unroll
     for (int i = 0; i < d; ++i) {
 #pragma unroll
       for (int j = 0; j < D; ++j) H[i][j] = Hk[i * D + j];
       c[i] = a.cx[(size_t)k * d + i];
+#pragma unroll
+      for (int j = 0; j < d; ++j) R[i][j] = Rk ? Rk[i * d + j] : 0.0;
     }
+  }
+  // R[r][col] for a lane's row r and column col (0 outside d x d)
+  __device__ __forceinline__ double r_at(int r, int col) const {
+    double v = 0.0;
+#pragma unroll
+    for (int i = 0; i < d; ++i) {
+#pragma unroll
+      for (int j = 0; j < d; ++j) v = (r == i && col == j) ? R[i][j] : v;
+    }
+    return v;
   }
   __device__ __forceinline__ double dot(int i, const double* v, int st) const {
     double acc = 0.0;
@@ -850,7 +866,7 @@ __device__ void filter_element(const Ctx<D, SS>& cx, const ModelArgs& a, int k,
     N1 = T0;
   }
 
-  // Psi_ = [[H N1, R], [N1, 0]]  (R = 0): rows 0..d-1 and d..d+D-1, column blocks (D, d)
+  // Psi_ = [[H N1, R], [N1, 0]]  (R = 0 for EK1): rows 0..d-1 and d..d+D-1, column blocks (D, d)
   double psi[NL + NLd];
   {
     double hn[NL];
@@ -861,7 +877,7 @@ __device__ void filter_element(const Ctx<D, SS>& cx, const ModelArgs& a, int k,
 #pragma unroll
     for (int t = 0; t < NL; ++t) psi[t] = (r < d) ? hn[t] : ((r < d + D) ? nr[t] : 0.0);
 #pragma unroll
-    for (int t = 0; t < NLd; ++t) psi[NL + t] = 0.0;
+    for (int t = 0; t < NLd; ++t) psi[NL + t] = (r < d) ? ob.r_at(r, t * S + s) : 0.0;
   }
   lq<S, d + D, D, d>(psi, r, s, cx.bc());
 

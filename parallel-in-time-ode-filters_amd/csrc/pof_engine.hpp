@@ -73,6 +73,10 @@ struct EngineFns {
   hipError_t (*seq_smooth_chol)(const GroupArgs& g, const LaunchPlan& p, hipStream_t s, KernelTimer* t);
   // EK1 linearisation at n points: H [n, d, D], c [n, d]
   hipError_t (*linearize)(const ModelArgs& a, const double* m, int n, double* H, double* c, hipStream_t s);
+  // linearisation of the given LinMethod at n states (m [n, D], L [n, D, D]):
+  // H [n, dO, D], c [n, dO], R [n, dO, dO]; work: n * lin_work<D, d>() doubles
+  hipError_t (*linearize_alt)(const ModelArgs& a, int method, const LinParams& p, const double* m, const double* L,
+                              int n, double* H, double* c, double* R, double* work, hipStream_t s);
 };
 
 const EngineFns* find_engine(int D, int d);

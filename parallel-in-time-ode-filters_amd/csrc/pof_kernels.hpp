@@ -20,6 +20,7 @@
 
 #include "pof_core.hpp"
 #include "pof_engine.hpp"
+#include "pof_linearization.hpp"
 
 namespace pof {
 
@@ -83,6 +84,7 @@ __device__ __forceinline__ ModelArgs view(const GroupArgs& g, int j) {
   a.lin = g.affine ? nullptr : ((g.cur[j] ? g.means1 : g.means0) + (size_t)base * D);
   a.Hx = g.Hx;
   a.cx = g.cx;
+  a.Rx = g.Rx;
   a.kind = g.kind[j];
   a.q = g.q;
   a.Ns = g.Ns[j];
@@ -282,7 +284,7 @@ __global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK)) void k_filter_post(Group
 #pragma unroll
     for (int t = 0; t < NL; ++t) sr[t] = (r < d) ? t1[t] : 0.0;
 #pragma unroll
-    for (int t = 0; t < NLd; ++t) sr[NL + t] = 0.0;
+    for (int t = 0; t < NLd; ++t) sr[NL + t] = (r < d) ? ob.r_at(r, t * S + sb) : 0.0;
   }
   lq<S, d, D, d>(sr, r, sb, cx.bc());
   if (r < d) store_dist<S, d>(TS + r * LD, sr, sb);
@@ -639,7 +641,7 @@ __global__ __launch_bounds__((Geo<D, kLdsSeq>::BLOCK)) void k_seq_filter(GroupAr
     } else {
       ob.load(a, k);
     }
-    // update: S = tria([[H Lp, R], [Lp, 0]]), R = 0; column blocks (D, d)
+    // update: S = tria([[H Lp, R], [Lp, 0]]) (R = 0 for EK1); column blocks (D, d)
     double psi[D + d];
     {
       double hn[D];
@@ -648,7 +650,7 @@ __global__ __launch_bounds__((Geo<D, kLdsSeq>::BLOCK)) void k_seq_filter(GroupAr
 #pragma unroll
       for (int c = 0; c < D; ++c) psi[c] = (r < d) ? hn[c] : ((r < d + D) ? TP[rn * LD + c] : 0.0);
 #pragma unroll
-      for (int c = 0; c < d; ++c) psi[D + c] = 0.0;
+      for (int c = 0; c < d; ++c) psi[D + c] = (r < d) ? ob.r_at(r, c) : 0.0;
     }
     lq<1, d + D, D, d>(psi, r, 0, cx.bc());
     if (r < d) {
@@ -976,6 +978,14 @@ hipError_t run_filtered(const GroupArgs& g, const LaunchPlan& p, double* fm, dou
 template <int D, int d>
 hipError_t run_linearize(const ModelArgs& a, const double* m, int n, double* H, double* c, hipStream_t st) {
   hipLaunchKernelGGL((k_linearize<D, d>), dim3((n + 127) / 128), dim3(128), 0, st, a, m, n, H, c);
+  return hipGetLastError();
+}
+
+template <int D, int d>
+hipError_t run_linearize_alt(const ModelArgs& a, int method, const LinParams& p, const double* m, const double* L,
+                             int n, double* H, double* c, double* R, double* work, hipStream_t st) {
+  hipLaunchKernelGGL((k_linearize_alt<D, d>), dim3((n + 63) / 64), dim3(64), 0, st, a, method, p, m, L, n, H, c, R,
+                     work);
   return hipGetLastError();
 }
 

@@ -16,12 +16,30 @@ struct ModelArgs {
   const double* lin;    // [Ns+1, D] linearisation points (EK1 mode), or null
   const double* Hx;     // [Ns, d, D] explicit observation matrices (affine mode), or null
   const double* cx;     // [Ns, d]    explicit observation offsets (affine mode)
+  const double* Rx;     // [Ns, d, d] explicit observation noise factors (affine mode), or null = 0
   int kind;             // ProblemKind (EK1 mode)
   int q;                // IWP order; D = d (q + 1)
   int Ns;               // number of steps (elements)
   int pad;
   double params[kMaxParams];
 };
+
+// linearisations of the residual model (pof/observations.py, pof/linearization/unscented.py)
+enum LinMethod : int { kLinEK1 = 0, kLinEK0 = 1, kLinUncertain = 2, kLinRegularized = 3, kLinUnscented = 4 };
+
+struct LinParams {
+  double lam;     // regularisation weight (linearize_regularized)
+  double alpha;   // unscented spread
+  double beta;    // unscented prior weight
+  double kappa;   // unscented kappa; used when kappa_set
+  int kappa_set;  // 0: kappa = 3 + D (unscented.py:47-48)
+  int pad;
+};
+
+// doubles of per-state work buffer of the uncertain / unscented linearisations
+__host__ __device__ constexpr int lin_work_host(int D, int d) {
+  return 2 * (2 * D + 1) * d + 3 * D * d + 3 * d * d + 2 * d + 8;
+}
 
 // filter element / aggregate arrays (structure of arrays; one entry per element)
 struct FiltArrays {
@@ -80,6 +98,7 @@ struct GroupArgs {
   const int* out_pfull;       // [slots] 1: full-state outputs are P (m, L) instead of (m, L)
   const double* Hx;           // affine mode (one IVP at entry 0): [Ns, d, D]
   const double* cx;           // [Ns, d]
+  const double* Rx;           // [Ns, d, d] or null (noiseless)
   int q;
   int affine;
 };

@@ -35,6 +35,7 @@ KINDS = {
 INIT_PRIOR = 0
 INIT_CONSTANT = 1
 INIT_COARSE = 2
+LIN_METHODS = {"ek1": 0, "ek0": 1, "uncertain": 2, "regularized": 3, "unscented": 4}
 MAX_PARAMS = 8
 STATUS = {0: "converged", 1: "maxiter", 2: "nan", 3: "fixed"}
 KERNEL_CLASSES = ["filter_leaf", "filter_up", "filter_down", "filter_post", "smooth_up", "smooth_down",
@@ -99,11 +100,13 @@ SIGNATURES = {
     "pof_eks_solve": (ctypes.c_int, [_vp, _P(Problem), ctypes.c_int, _P(Out), _vp]),
     "pof_eval_field": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]),
     "pof_linearize": (ctypes.c_int, [_vp, _P(Problem), _vp, ctypes.c_int64, _vp, _vp, _vp]),
+    "pof_linearize_model": (ctypes.c_int, [_vp, _P(Problem), ctypes.c_int, _dp, _vp, _vp, ctypes.c_int64, _vp, _vp,
+                                           _vp, _vp]),
     "pof_ieks_step": (ctypes.c_int, [_vp, _P(Problem), _vp, ctypes.c_int, _vp, _vp, _dp, _dp, _dp, _vp]),
     "pof_linear_filtsmooth": (
         ctypes.c_int,
-        [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int64, _dp, _dp, _dp, _dp, _vp, _vp, ctypes.c_int, _vp, _vp,
-         _vp, _vp, _dp, _dp, _dp, _vp],
+        [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int64, _dp, _dp, _dp, _dp, _vp, _vp, _vp, ctypes.c_int, _vp,
+         _vp, _vp, _vp, _dp, _dp, _dp, _vp],
     ),
     "pof_batch_entries_for": (ctypes.c_int64, [ctypes.c_int64]),
     "pof_batch_bytes": (ctypes.c_int64, [ctypes.c_int, ctypes.c_int64, ctypes.c_int32]),

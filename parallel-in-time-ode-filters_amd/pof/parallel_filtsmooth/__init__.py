@@ -5,8 +5,9 @@
 (``smoother.py:8-22``) in ``libpof_hip.so`` and returns
 ``(MVNSqrt(means[N+1, D], chols[N+1, D, D]), nll, obj, ssq)``
 (``__init__.py:5-10``).  The transition model must be constant over time
-(the solver always builds it so, ``convenience.py:19``) and the observations
-noiseless (``cholR = 0``, as produced by EK1).
+(the solver always builds it so, ``convenience.py:19``).  The observations
+may carry a noise factor ``cholR`` (``[N, d, d]``, from
+``uncertain_linearize`` or ``linearize_unscented``); EK1 produces ``0``.
 """
 import ctypes
 
@@ -33,11 +34,14 @@ def _run(x0, dtm, dom, want_filter, sequential=False):
     H = as_f64_device(dom.H if hasattr(dom, "H") else dom[0])
     c = as_f64_device(dom.b if hasattr(dom, "b") else dom[1])
     cholR = dom.cholR if hasattr(dom, "cholR") else dom[2]
-    if cholR is not None:
-        R = cholR if isinstance(cholR, torch.Tensor) else torch.as_tensor(np.asarray(cholR))
-        if bool((R != 0).any()):
-            raise NotImplementedError("only noiseless observations (cholR = 0) are supported")
     Ns, d, D = H.shape
+    R = None
+    if cholR is not None:
+        R = as_f64_device(cholR)
+        if tuple(R.shape) != (Ns, d, d):
+            raise ValueError(f"cholR has shape {tuple(R.shape)}, expected {(Ns, d, d)}")
+        if not bool((R != 0).any()):
+            R = None  # noiseless: the kernels' R = 0 path
     m0 = as_f64_host(x0.mean).reshape(D)
     L0 = as_f64_host(x0.chol).reshape(D, D)
     dev = H.device
@@ -52,7 +56,7 @@ def _run(x0, dtm, dom, want_filter, sequential=False):
     _lib.check(
         _lib.load().pof_linear_filtsmooth(
             ctx, D, d, Ns, _lib.hptr(m0), _lib.hptr(L0), _lib.hptr(F), _lib.hptr(QL), _lib.dptr(H), _lib.dptr(c),
-            1 if sequential else 0, _lib.dptr(sm), _lib.dptr(sL), _lib.dptr(fm), _lib.dptr(fL), ctypes.byref(nll), ctypes.byref(obj),
+            _lib.dptr(R), 1 if sequential else 0, _lib.dptr(sm), _lib.dptr(sL), _lib.dptr(fm), _lib.dptr(fL), ctypes.byref(nll), ctypes.byref(obj),
             ctypes.byref(ssq), _lib.stream_ptr(dev)),
         ctx, "pof_linear_filtsmooth",
     )

@@ -136,19 +136,20 @@ def test_associative_scan_matches_fold():
 
 
 # ------------------------------------------------------ brute-force conditioning
-def _random_model(rng, d, q, Ns, prior_chol=True):
+def _random_model(rng, d, q, Ns, prior_chol=True, noise=False):
     D = d * (q + 1)
     F, QL = om.preconditioned_discretize(om.IWP(d, q))
     H = rng.standard_normal((Ns, d, D))
     c = rng.standard_normal((Ns, d))
     m0 = rng.standard_normal(D)
     L0 = np.tril(rng.standard_normal((D, D))) if prior_chol else np.zeros((D, D))
-    return F, QL, om.AffineModel(H, c, np.zeros((Ns, d, d))), om.MVNSqrt(m0, L0)
+    R = np.tril(rng.standard_normal((Ns, d, d))) * 0.5 if noise else np.zeros((Ns, d, d))
+    return F, QL, om.AffineModel(H, c, R), om.MVNSqrt(m0, L0)
 
 
 def _brute(F, QL, obs, x0):
-    """Dense joint Gaussian over x_0..x_Ns, conditioned on H_k x_{k+1} + c_k = 0."""
-    H, c, _ = obs
+    """Dense joint Gaussian over x_0..x_Ns, conditioned on H_k x_{k+1} + c_k + R_k e_k = 0."""
+    H, c, cholR = obs
     Ns, d, D = H.shape
     n = (Ns + 1) * D
     # x = B w + mu with w ~ N(0, I): x_0 = m0 + L0 w0, x_{k+1} = F x_k + QL w_{k+1}
@@ -166,10 +167,13 @@ def _brute(F, QL, obs, x0):
     for k in range(Ns):
         G[k * d : (k + 1) * d, (k + 1) * D : (k + 2) * D] = H[k]
     y = G @ mu + c.reshape(-1)
+    Rn = np.zeros((Ns * d, Ns * d))
+    for k in range(Ns):
+        Rn[k * d : (k + 1) * d, k * d : (k + 1) * d] = cholR[k] @ cholR[k].T
 
     def condition(rows):
         Gs = G[rows]
-        Syy = Gs @ S @ Gs.T
+        Syy = Gs @ S @ Gs.T + Rn[np.ix_(rows, rows)]
         K = S @ Gs.T @ np.linalg.inv(Syy)
         return mu - K @ y[rows], S - K @ Gs @ S
 
@@ -181,15 +185,16 @@ def _brute(F, QL, obs, x0):
     ms, Ps = condition(np.arange(Ns * d))
     sm_m = [ms[k * D : (k + 1) * D] for k in range(Ns + 1)]
     sm_P = [Ps[k * D : (k + 1) * D, k * D : (k + 1) * D] for k in range(Ns + 1)]
-    Syy = G @ S @ G.T
+    Syy = G @ S @ G.T + Rn
     nll = 0.5 * y @ np.linalg.solve(Syy, y) + 0.5 * np.linalg.slogdet(2 * np.pi * Syy)[1]
     return np.array(filt_m), np.array(filt_P), np.array(sm_m), np.array(sm_P), nll
 
 
-@pytest.mark.parametrize("d,q,Ns,seed", [(1, 2, 5, 0), (2, 1, 6, 1), (2, 2, 4, 2), (1, 3, 4, 3)])
-def test_parallel_filtsmooth_brute_force(d, q, Ns, seed):
+@pytest.mark.parametrize("d,q,Ns,seed,noise", [(1, 2, 5, 0, False), (2, 1, 6, 1, False), (2, 2, 4, 2, False),
+                                               (1, 3, 4, 3, False), (2, 1, 5, 8, True), (1, 2, 6, 9, True)])
+def test_parallel_filtsmooth_brute_force(d, q, Ns, seed, noise):
     rng = np.random.default_rng(seed)
-    F, QL, obs, x0 = _random_model(rng, d, q, Ns)
+    F, QL, obs, x0 = _random_model(rng, d, q, Ns, noise=noise)
     fm, fP, sm, sP, nll = _brute(F, QL, obs, x0)
     filt, pnll, _, pssq = opar.linear_noiseless_filtering(x0, F, QL, obs)
     out, obj = opar.smoothing(F, QL, filt)
@@ -206,14 +211,16 @@ def test_parallel_filtsmooth_brute_force(d, q, Ns, seed):
         pm = np.einsum("ij,nj->ni", F, fm[:-1])
         yhat = np.einsum("nij,nj->ni", H, pm) + c
         Spred = np.einsum("nij,njk,nlk->nil", H, np.einsum("ij,njk,lk->nil", F, fP[:-1], F) + QL @ QL.T, H)
+        Spred = Spred + np.einsum("nij,nkj->nik", obs.cholR, obs.cholR)
         ssq = np.mean(yhat[:, 0] ** 2 / Spred[:, 0, 0])
         assert abs(pssq - ssq) < 1e-8 * ssq
 
 
-@pytest.mark.parametrize("d,q,Ns,seed", [(2, 2, 23, 5), (3, 1, 17, 6), (1, 3, 30, 7)])
-def test_parallel_equals_sequential(d, q, Ns, seed):
+@pytest.mark.parametrize("d,q,Ns,seed,noise", [(2, 2, 23, 5, False), (3, 1, 17, 6, False), (1, 3, 30, 7, False),
+                                               (2, 2, 19, 10, True)])
+def test_parallel_equals_sequential(d, q, Ns, seed, noise):
     rng = np.random.default_rng(seed)
-    F, QL, obs, x0 = _random_model(rng, d, q, Ns)
+    F, QL, obs, x0 = _random_model(rng, d, q, Ns, noise=noise)
     p, pnll, pobj, pssq = opar.linear_filtsmooth(x0, F, QL, obs)
     s, snll, sobj, sssq = oseq.linear_filtsmooth(x0, F, QL, obs)
     cov = lambda L: np.einsum("nij,nkj->nik", L, L)
