@@ -116,6 +116,15 @@ int pof_profile_read(const pof_ctx* ctx, int n_classes, double* ms, long long* l
 /* Full parallel IEKS solve.  Replaces pof.solver.solve (pof/solver.py:11-73). */
 int pof_solve(pof_ctx* ctx, const pof_problem* prob, const pof_opts* opts, pof_out* out, void* stream);
 
+/* The same solve on the large-state engine: every scan level as batched
+ * primitives (a Householder LQ kernel, one workgroup per matrix, and rocBLAS
+ * strided-batched DGEMM / DTRSM), for state dimensions beyond the per-worker
+ * kernels (Lorenz-96 with d = 32, q = 3: D = 128).  pof_solve uses it by itself
+ * for such problems; it runs any (D, d) whose field it can linearise (every
+ * pof_supported pair, and Lorenz-96 for D <= 256).  Parallel IEKS only;
+ * init "prior" or "constant". */
+int pof_solve_large(pof_ctx* ctx, const pof_problem* prob, const pof_opts* opts, pof_out* out, void* stream);
+
 /* Solver set-up on the host.  Replaces pof.convenience.set_up_solver (pof/convenience.py:13-45):
  * F, QL [D, D] (transitions.py:37-50), P, PI diagonals [D] (transitions.py:53-68),
  * x0 mean [D] and factor [D, D] (preconditioned Taylor-mode init, initialization.py:15-22,

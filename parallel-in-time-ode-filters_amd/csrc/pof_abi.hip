@@ -19,6 +19,7 @@
 
 #include "../../include/pof.h"
 #include "pof_engine.hpp"
+#include "pof_large.hpp"
 #include "pof_problems.hpp"
 
 namespace pof {
@@ -352,6 +353,8 @@ struct pof_ctx {
   std::vector<pof_batch*> cache;
   // small batches for the coarse initial trajectory (EKS on 100 points), one per (D, d)
   std::vector<pof_batch*> aux;
+  // large-state engine (batched primitives), created on first use
+  pof::large::Engine* large = nullptr;
 };
 
 // A group of IVPs with one (D, d) sharing one device workspace.  IVP slots are
@@ -1167,6 +1170,8 @@ void pof_ctx_destroy(pof_ctx* ctx) {
   ctx->cache.clear();
   for (pof_batch* b : ctx->aux) batch_free(b);
   ctx->aux.clear();
+  pof::large::engine_destroy(ctx->large);
+  ctx->large = nullptr;
   if (ctx->timer.ready)
     for (int i = 0; i <= pof::kMaxTimed; ++i) (void)hipEventDestroy(ctx->timer.ev[i]);
   delete ctx;
@@ -1323,11 +1328,55 @@ int pof_linearize_model(pof_ctx* ctx, const pof_problem* prob, int method, const
   return POF_OK;
 }
 
+// The whole solve on the large-state engine (batched primitives); `small` is the
+// per-worker engine of (D, d) when there is one (its EK1 kernel is reused).
+static int solve_large(pof_ctx* ctx, const pof_problem* prob, const pof_opts* opts, pof_out* out, void* stream) {
+  pof::Setup s;
+  int rc = pof::build_setup(prob, s, ctx->err);
+  if (rc != POF_OK) return rc;
+  if (opts->sequential) return set_err(ctx, POF_E_UNSUPPORTED, "the large-state engine runs the parallel IEKS only");
+  if (s.Ns > 65534) return set_err(ctx, POF_E_ARG, "the large-state engine takes at most 65534 steps");
+  std::vector<double> m0;
+  rc = (prob->init == POF_INIT_COARSE) ? POF_E_UNSUPPORTED : pof::initial_means(prob, s, m0, ctx->err);
+  if (rc == POF_E_UNSUPPORTED) return set_err(ctx, rc, "init=coarse is not available on the large-state engine");
+  if (rc != POF_OK) return rc;
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (!ctx->large) {
+    ctx->large = pof::large::engine_create(ctx->device);
+    if (!ctx->large) return set_err(ctx, POF_E_HIP, "rocBLAS handle creation failed");
+  }
+  pof::large::Problem lp{};
+  lp.D = s.D;
+  lp.d = s.d;
+  lp.q = s.q;
+  lp.Ns = s.Ns;
+  lp.kind = prob->kind;
+  lp.params = prob->params;
+  lp.F = s.F.data();
+  lp.QL = s.LQ.data();
+  lp.N1z = s.N1z.data();
+  lp.P = s.P.data();
+  lp.x0m = s.x0m.data();
+  lp.x0L = s.x0L.data();
+  lp.m0 = m0.data();
+  std::string err;
+  rc = pof::large::solve(ctx->large, lp, *opts, out, pof::find_engine(s.D, s.d), (hipStream_t)stream, err);
+  if (rc != POF_OK) return set_err(ctx, rc, "%s", err.c_str());
+  return POF_OK;
+}
+
+int pof_solve_large(pof_ctx* ctx, const pof_problem* prob, const pof_opts* opts, pof_out* out, void* stream) {
+  if (!ctx || !prob || !opts || !out) return set_err(ctx, POF_E_ARG, "null argument");
+  return solve_large(ctx, prob, opts, out, stream);
+}
+
 int pof_solve(pof_ctx* ctx, const pof_problem* prob, const pof_opts* opts, pof_out* out, void* stream) {
   if (!ctx || !opts || !out) return set_err(ctx, POF_E_ARG, "null argument");
   pof::Setup s;
   int rc = pof::build_setup(prob, s, ctx->err);
   if (rc != POF_OK) return rc;
+  if (!pof::find_engine(s.D, s.d) && pof::large::supports(prob->kind, s.D, s.d))
+    return solve_large(ctx, prob, opts, out, stream);
   if (!engine_or_err(ctx, s.D, s.d)) return POF_E_UNSUPPORTED;
   pof_batch* b = nullptr;
   rc = cached_batch(ctx, s.D, s.d, entries_for(prob->n_times), 1, &b);

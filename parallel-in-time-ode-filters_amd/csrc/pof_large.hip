@@ -1,0 +1,868 @@
+// Large-state IEKS engine (state dimension beyond the per-worker kernels,
+// e.g. Lorenz-96 d = 32, q = 3: D = 128).
+//
+// Each combine of a scan level is a sequence of batched primitives over every
+// combine of that level: a Householder LQ kernel (one workgroup per matrix,
+// LAPACK dlarfg's reflector convention, the same as the per-worker lq()),
+// rocBLAS strided-batched DGEMM / DTRSM for the plain products and triangular
+// solves (FP64 MFMA on gfx950), and small copy / fill / reduction kernels.  The
+// host drives JAX's odd/even associative_scan recursion (pof/parallel_filtsmooth
+// filter.py:35, smoother.py:15-17) over strided views, so every factor is built
+// by the same sequence of operations as in the reference.
+//
+// Matrices are row-major float64.  A rocBLAS call on a row-major C = op(A) op(B)
+// is the column-major C^T = op(B)^T op(A)^T; a row-major lower-triangular L is
+// the column-major upper L^T.
+#include "pof_large.hpp"
+
+#include <rocblas/rocblas.h>
+
+#include "pof_problems.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+namespace pof {
+namespace large {
+
+namespace {
+
+// ------------------------------------------------------------------ views
+// A strided batch of row-major matrices (or vectors, ld = 1): item i at p + i * s.
+struct V {
+  double* p;
+  long long s;  // doubles between items (0: one matrix shared by every item)
+  int ld;       // doubles between rows
+  V at(int r, int c) const { return V{p + (size_t)r * ld + c, s, ld}; }
+  V item(long long i) const { return V{p + i * s, s, ld}; }
+  V every(int k) const { return V{p, s * k, ld}; }
+};
+
+// ------------------------------------------------------------------ kernels
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+constexpr int kLqThreads = 256;
+constexpr int kLqMaxN = 512;
+
+// In-place Householder LQ of the m x n matrix of item blockIdx.x: afterwards its
+// first min(m, n) columns hold tria(A) = qr(A^T).R^T (utils.py:33-41) with
+// LAPACK's reflectors (beta = -sign(alpha) ||x||, tau = (beta - alpha) / beta,
+// v = x / (alpha - beta), no reflector for a zero tail).  Pivot row i's tail is
+// reduced by the whole block; every later row is updated by one wave.
+__global__ __launch_bounds__(kLqThreads) void k_lq(double* A, long long stride, int m, int n, int ld) {
+  __shared__ double v[kLqMaxN];
+  __shared__ double red[kLqThreads / 64];
+  double* a = A + (long long)blockIdx.x * stride;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  constexpr int NW = kLqThreads / 64;
+  const int K = m < n ? m : n;
+  for (int i = 0; i < K && i < n - 1; ++i) {
+    double* pr = a + (size_t)i * ld;
+    double s = 0.0;
+    for (int j = i + 1 + tid; j < n; j += kLqThreads) s = fma(pr[j], pr[j], s);
+    s = wave_sum(s);
+    if (lane == 0) red[wv] = s;
+    __syncthreads();
+    const double sig = (red[0] + red[1]) + (red[2] + red[3]);
+    const double alpha = pr[i];
+    double tau = 0.0, beta = alpha, scal = 0.0;
+    if (sig != 0.0) {
+      const double nrm = sqrt(fma(alpha, alpha, sig));
+      beta = -copysign(nrm, alpha);
+      tau = (beta - alpha) / beta;
+      scal = 1.0 / (alpha - beta);
+    }
+    for (int j = i + 1 + tid; j < n; j += kLqThreads) v[j] = pr[j] * scal;
+    __syncthreads();
+    if (tau != 0.0) {
+      for (int r = i + 1 + wv; r < m; r += NW) {
+        double* rr = a + (size_t)r * ld;
+        double w = 0.0;
+        for (int j = i + 1 + lane; j < n; j += 64) w = fma(rr[j], v[j], w);
+        w = wave_sum(w) + rr[i];
+        const double tw = tau * w;
+        for (int j = i + 1 + lane; j < n; j += 64) rr[j] = fma(-tw, v[j], rr[j]);
+        if (lane == 0) rr[i] -= tw;
+      }
+    }
+    if (tid == 0) pr[i] = beta;
+    for (int j = i + 1 + tid; j < n; j += kLqThreads) pr[j] = 0.0;
+    __syncthreads();
+  }
+}
+
+// dst item b (rows x cols) = alpha * src item b' (r, c), or src (c, r) when trans;
+// b' = cnt - 1 - b when rev.  grid (ceil(rows * cols / 256), cnt)
+__global__ void k_copy(double* dst, long long ds, int dld, const double* src, long long ss, int sld, int rows,
+                       int cols, int cnt, int trans, int rev, double alpha) {
+  const int b = blockIdx.y;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= cnt || e >= rows * cols) return;
+  const int r = e / cols, c = e % cols;
+  const long long bs = rev ? (long long)(cnt - 1 - b) : (long long)b;
+  const double x = trans ? src[bs * ss + (size_t)c * sld + r] : src[bs * ss + (size_t)r * sld + c];
+  dst[(long long)b * ds + (size_t)r * dld + c] = alpha * x;
+}
+
+// dst item (rows x cols) = dv on the diagonal, 0 elsewhere
+__global__ void k_fill(double* dst, long long ds, int dld, int rows, int cols, int cnt, double dv) {
+  const int b = blockIdx.y;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= cnt || e >= rows * cols) return;
+  const int r = e / cols, c = e % cols;
+  dst[(long long)b * ds + (size_t)r * dld + c] = (r == c) ? dv : 0.0;
+}
+
+// EK1 of the Lorenz-96 residual at n states (rows of H as in the per-worker
+// kernels: -J[i][l] P[l q1] at column l q1, + P[i q1 + 1] at column i q1 + 1;
+// c = J y - f(y)).  One thread per (state, row).
+__global__ void k_lin_l96(const double* m, const double* P, double Fo, int n, int d, int q1, double* H, double* c) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)n * d) return;
+  const long long k = e / d;
+  const int i = (int)(e % d);
+  const int D = d * q1;
+  const double* mk = m + k * D;
+  const int ip1 = (i + 1) % d, im1 = (i + d - 1) % d, im2 = (i + d - 2) % d;
+  const double yi = P[i * q1] * mk[i * q1];
+  const double yp1 = P[ip1 * q1] * mk[ip1 * q1];
+  const double ym1 = P[im1 * q1] * mk[im1 * q1];
+  const double ym2 = P[im2 * q1] * mk[im2 * q1];
+  const double f = (yp1 - ym2) * ym1 - yi + Fo;
+  // J[i][l] for the four columns, summed in ascending l for c
+  int cols[4] = {ip1, im1, im2, i};
+  double jv[4] = {ym1, yp1 - ym2, -ym1, -1.0};
+  double ys[4] = {yp1, ym1, ym2, yi};
+  for (int a = 0; a < 4; ++a)
+    for (int b2 = a + 1; b2 < 4; ++b2)
+      if (cols[b2] < cols[a]) {
+        int t = cols[a]; cols[a] = cols[b2]; cols[b2] = t;
+        double u = jv[a]; jv[a] = jv[b2]; jv[b2] = u;
+        u = ys[a]; ys[a] = ys[b2]; ys[b2] = u;
+      }
+  double acc = 0.0;
+  for (int a = 0; a < 4; ++a) acc = fma(jv[a], ys[a], acc);
+  double* Hr = H + (k * d + i) * D;
+  for (int j = 0; j < D; ++j) Hr[j] = 0.0;
+  for (int a = 0; a < 4; ++a) Hr[cols[a] * q1] = -jv[a] * P[cols[a] * q1];
+  Hr[i * q1 + 1] += P[i * q1 + 1];
+  c[k * d + i] = acc - f;
+}
+
+// Observation terms of state s from S (d x d lower, leading dimension ldS) and
+// yhat: ssq_s = |S^-T y|^2 (utils.py:110-112), nll_s = -loglik (utils.py:22-30).
+__global__ void k_obs_terms(const double* S, long long sS, int ldS, const double* y, long long sy, int d, int n,
+                            double* ssq, double* nll, double* tmp) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const double* Ss = S + (long long)s * sS;
+  const double* ys = y + (long long)s * sy;
+  double* w = tmp + (long long)s * 2 * d;
+  double* z = w + d;
+  double ld = 0.0;
+  for (int i = 0; i < d; ++i) {
+    double v = ys[i];
+    for (int t = 0; t < i; ++t) v = fma(-Ss[(size_t)i * ldS + t], z[t], v);
+    z[i] = v / Ss[(size_t)i * ldS + i];
+    ld += log(fabs(Ss[(size_t)i * ldS + i]));
+  }
+  for (int i = d - 1; i >= 0; --i) {
+    double v = ys[i];
+    for (int t = i + 1; t < d; ++t) v = fma(-Ss[(size_t)t * ldS + i], w[t], v);
+    w[i] = v / Ss[(size_t)i * ldS + i];
+  }
+  double ss = 0.0, nn = 0.0;
+  for (int i = 0; i < d; ++i) {
+    ss = fma(w[i], w[i], ss);
+    nn = fma(z[i], z[i], nn);
+  }
+  ssq[s] = ss;
+  nll[s] = 0.5 * nn + (ld + d * log(2.0 * M_PI) / 2.0);
+}
+
+// out[s] = |x_s|^2 (x: n vectors of length D, stride D)
+__global__ void k_sqnorm(const double* x, int D, int n, double* out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  double a = 0.0;
+  for (int i = 0; i < D; ++i) a = fma(x[(size_t)s * D + i], x[(size_t)s * D + i], a);
+  out[s] = a;
+}
+
+// means-close test of the convergence rule (convergence_criteria.py:10-12):
+// out[t] = 1 if any |old - new| > 1e-8 + 1e-13 |new| (or NaN) in state t
+__global__ void k_viol(const double* mnew, const double* mold, int D, int n, double* out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  double v = 0.0;
+  for (int i = 0; i < D; ++i) {
+    const double o = mold[(size_t)t * D + i], x = mnew[(size_t)t * D + i];
+    v = (fabs(o - x) <= 1e-8 + 1e-13 * fabs(x)) ? v : 1.0;
+  }
+  out[t] = v;
+}
+
+// Fixed-order reductions of n terms: out[0] = sum a, out[1] = sum b, out[2] = sum c, out[3] = max v.
+__global__ __launch_bounds__(256) void k_reduce4(const double* a, const double* b, const double* c, int nabc,
+                                                 const double* v, int nv, double* out) {
+  __shared__ double sh[4][256];
+  const int t = threadIdx.x;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  for (int i = t; i < nabc; i += 256) {
+    s0 += a[i];
+    s1 += b[i];
+    s2 += c[i];
+  }
+  for (int i = t; i < nv; i += 256) s3 = fmax(s3, v[i]);
+  sh[0][t] = s0;
+  sh[1][t] = s1;
+  sh[2][t] = s2;
+  sh[3][t] = s3;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      sh[0][t] += sh[0][t + w];
+      sh[1][t] += sh[1][t + w];
+      sh[2][t] += sh[2][t + w];
+      sh[3][t] = fmax(sh[3][t], sh[3][t + w]);
+    }
+    __syncthreads();
+  }
+  if (t < 4) out[t] = sh[t][0];
+}
+
+// Output projection (solver.py:66-71): means_out[t][i] = P_i0 m_t[i0],
+// chol_out[t][i][:] = scale P_i0 L_t[i0][:], i0 = i q1; full outputs m_t and scale L_t.
+__global__ void k_project(const double* m, const double* L, const double* P, int D, int d, int n, double scale,
+                          double* means, double* chols, double* fmeans, double* fchols) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int q1 = D / d;
+  if (e >= (long long)n * D) return;
+  const long long t = e / D;
+  const int r = (int)(e % D);
+  if (fmeans) fmeans[t * D + r] = m[t * D + r];
+  if (fchols)
+    for (int c = 0; c < D; ++c) fchols[(t * D + r) * D + c] = scale * L[(t * D + r) * D + c];
+  if (r % q1 == 0) {
+    const int i = r / q1;
+    if (means) means[t * d + i] = P[r] * m[t * D + r];
+    if (chols)
+      for (int c = 0; c < D; ++c) chols[(t * d + i) * D + c] = (scale * P[r]) * L[(t * D + r) * D + c];
+  }
+}
+
+// ------------------------------------------------------------- host helpers
+struct Ops {
+  rocblas_handle h;
+  hipStream_t st;
+  int err = 0;
+  std::string msg;
+
+  void fail(const char* what, int code) {
+    if (!err) {
+      err = code;
+      char b[160];
+      std::snprintf(b, sizeof b, "%s failed (%d)", what, code);
+      msg = b;
+    }
+  }
+  static rocblas_operation op(bool t) { return t ? rocblas_operation_transpose : rocblas_operation_none; }
+  // row-major C (M x N) = alpha op(A) op(B) + beta C
+  void gemm(bool tA, bool tB, int M, int N, int K, double alpha, V A, V B, double beta, V C, int cnt) {
+    if (err || cnt <= 0) return;
+    const rocblas_status s =
+        rocblas_dgemm_strided_batched(h, op(tB), op(tA), N, M, K, &alpha, B.p, B.ld, B.s, A.p, A.ld, A.s, &beta,
+                                      C.p, C.ld, C.s, cnt);
+    if (s != rocblas_status_success) fail("rocblas_dgemm_strided_batched", (int)s);
+  }
+  // op(L) X = X (X: M x N, L lower M x M), in place
+  void trsm_left(bool tL, int M, int N, V L, V X, int cnt) {
+    if (err || cnt <= 0) return;
+    const double one = 1.0;
+    const rocblas_status s =
+        rocblas_dtrsm_strided_batched(h, rocblas_side_right, rocblas_fill_upper, op(tL), rocblas_diagonal_non_unit,
+                                      N, M, &one, L.p, L.ld, L.s, X.p, X.ld, X.s, cnt);
+    if (s != rocblas_status_success) fail("rocblas_dtrsm_strided_batched", (int)s);
+  }
+  // X op(L) = X (X: M x N, L lower N x N), in place
+  void trsm_right(bool tL, int M, int N, V L, V X, int cnt) {
+    if (err || cnt <= 0) return;
+    const double one = 1.0;
+    const rocblas_status s =
+        rocblas_dtrsm_strided_batched(h, rocblas_side_left, rocblas_fill_upper, op(tL), rocblas_diagonal_non_unit,
+                                      N, M, &one, L.p, L.ld, L.s, X.p, X.ld, X.s, cnt);
+    if (s != rocblas_status_success) fail("rocblas_dtrsm_strided_batched", (int)s);
+  }
+  void lq(V A, int m, int n, int cnt) {
+    if (err || cnt <= 0) return;
+    if (n > kLqMaxN) {
+      fail("lq: too many columns", -1);
+      return;
+    }
+    hipLaunchKernelGGL(k_lq, dim3(cnt), dim3(kLqThreads), 0, st, A.p, A.s, m, n, A.ld);
+    check("k_lq");
+  }
+  void copy(V dst, V src, int rows, int cols, int cnt, bool trans = false, bool rev = false, double alpha = 1.0) {
+    if (err || cnt <= 0) return;
+    const int e = rows * cols;
+    hipLaunchKernelGGL(k_copy, dim3((e + 255) / 256, cnt), dim3(256), 0, st, dst.p, dst.s, dst.ld, src.p, src.s,
+                       src.ld, rows, cols, cnt, trans ? 1 : 0, rev ? 1 : 0, alpha);
+    check("k_copy");
+  }
+  void fill(V dst, int rows, int cols, int cnt, double dv) {
+    if (err || cnt <= 0) return;
+    const int e = rows * cols;
+    hipLaunchKernelGGL(k_fill, dim3((e + 255) / 256, cnt), dim3(256), 0, st, dst.p, dst.s, dst.ld, rows, cols, cnt,
+                       dv);
+    check("k_fill");
+  }
+  void check(const char* what) {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) fail(what, (int)e);
+  }
+};
+
+// bump allocator over the engine's workspace
+struct Arena {
+  char* base;
+  size_t cap, off = 0;
+  double* take(size_t n) {
+    const size_t bytes = ((n * sizeof(double)) + 255) & ~(size_t)255;
+    if (off + bytes > cap) return nullptr;
+    double* p = (double*)(base + off);
+    off += bytes;
+    return p;
+  }
+};
+
+// filter elements (A, b, U, eta, Z) as strided views
+struct FE {
+  V A, b, U, eta, Z;
+  FE every(int k) const { return FE{A.every(k), b.every(k), U.every(k), eta.every(k), Z.every(k)}; }
+  FE item(long long i) const { return FE{A.item(i), b.item(i), U.item(i), eta.item(i), Z.item(i)}; }
+};
+// smoother elements (g, E, D)
+struct SE {
+  V g, E, Dm;
+  SE every(int k) const { return SE{g.every(k), E.every(k), Dm.every(k)}; }
+  SE item(long long i) const { return SE{g.item(i), E.item(i), Dm.item(i)}; }
+};
+
+FE fe_alloc(Arena& ar, int D, long long n) {
+  const long long DD = (long long)D * D;
+  FE e;
+  e.A = V{ar.take(n * DD), DD, D};
+  e.b = V{ar.take(n * D), D, 1};
+  e.U = V{ar.take(n * DD), DD, D};
+  e.eta = V{ar.take(n * D), D, 1};
+  e.Z = V{ar.take(n * DD), DD, D};
+  return e;
+}
+bool fe_ok(const FE& e) { return e.A.p && e.b.p && e.U.p && e.eta.p && e.Z.p; }
+SE se_alloc(Arena& ar, int D, long long n) {
+  const long long DD = (long long)D * D;
+  SE e;
+  e.g = V{ar.take(n * D), D, 1};
+  e.E = V{ar.take(n * DD), DD, D};
+  e.Dm = V{ar.take(n * DD), DD, D};
+  return e;
+}
+bool se_ok(const SE& e) { return e.g.p && e.E.p && e.Dm.p; }
+
+// Filter combine (earlier e1, later e2) -> out, filter.py:117-142 with the
+// operand order of the reference.
+void filter_fn(Ops& o, Arena& ar, int D, const FE& e1, const FE& e2, const FE& out, int cnt) {
+  if (cnt <= 0 || o.err) return;
+  const size_t mark = ar.off;
+  const long long DD = (long long)D * D;
+  V X{ar.take((size_t)cnt * 4 * DD), 4 * DD, 2 * D};
+  V Y{ar.take((size_t)cnt * 2 * DD), 2 * DD, 2 * D};
+  V T1{ar.take((size_t)cnt * DD), DD, D}, T2{ar.take((size_t)cnt * DD), DD, D};
+  V T3{ar.take((size_t)cnt * DD), DD, D}, T4{ar.take((size_t)cnt * DD), DD, D};
+  V v1{ar.take((size_t)cnt * D), D, 1}, v2{ar.take((size_t)cnt * D), D, 1}, v3{ar.take((size_t)cnt * D), D, 1};
+  if (!X.p || !Y.p || !T1.p || !T2.p || !T3.p || !T4.p || !v1.p || !v2.p || !v3.p) {
+    o.fail("workspace", -2);
+    return;
+  }
+  // Xi = tria([[U1^T Z2, I], [Z2, 0]])
+  o.gemm(true, false, D, D, D, 1.0, e1.U, e2.Z, 0.0, X.at(0, 0), cnt);
+  o.fill(X.at(0, D), D, D, cnt, 1.0);
+  o.copy(X.at(D, 0), e2.Z, D, D, cnt);
+  o.fill(X.at(D, D), D, D, cnt, 0.0);
+  o.lq(X, 2 * D, 2 * D, cnt);
+  const V Xi11 = X.at(0, 0), Xi21 = X.at(D, 0), Xi22 = X.at(D, D);
+  // M = Xi11^-1 (U1^T A2^T); A = A2 A1 - M^T (Xi21^T A1)
+  o.gemm(true, true, D, D, D, 1.0, e1.U, e2.A, 0.0, T1, cnt);
+  o.trsm_left(false, D, D, Xi11, T1, cnt);
+  o.gemm(true, false, D, D, D, 1.0, Xi21, e1.A, 0.0, T2, cnt);
+  o.gemm(false, false, D, D, D, 1.0, e2.A, e1.A, 0.0, out.A, cnt);
+  o.gemm(true, false, D, D, D, -1.0, T1, T2, 1.0, out.A, cnt);
+  // m_ = Xi11^-1 U1^T; b = A2 (v2 - m_^T Xi21^T v2) + b2, v2 = b1 + U1 (U1^T eta2)
+  o.copy(T3, e1.U, D, D, cnt, true);
+  o.trsm_left(false, D, D, Xi11, T3, cnt);
+  o.gemm(true, false, D, 1, D, 1.0, e1.U, e2.eta, 0.0, v1, cnt);
+  o.copy(v2, e1.b, D, 1, cnt);
+  o.gemm(false, false, D, 1, D, 1.0, e1.U, v1, 1.0, v2, cnt);
+  o.gemm(true, false, D, 1, D, 1.0, Xi21, v2, 0.0, v3, cnt);
+  o.gemm(true, false, D, 1, D, -1.0, T3, v3, 1.0, v2, cnt);
+  o.copy(out.b, e2.b, D, 1, cnt);
+  o.gemm(false, false, D, 1, D, 1.0, e2.A, v2, 1.0, out.b, cnt);
+  // U = tria([M^T | U2])
+  o.copy(Y.at(0, 0), T1, D, D, cnt, true);
+  o.copy(Y.at(0, D), e2.U, D, D, cnt);
+  o.lq(Y, D, 2 * D, cnt);
+  o.copy(out.U, Y.at(0, 0), D, D, cnt);
+  // _e = Xi11^-T Xi21^T; eta = A1^T (w2 - _e^T U1^T w2) + eta1, w2 = eta2 - Z2 (Z2^T b1)
+  o.copy(T4, Xi21, D, D, cnt, true);
+  o.trsm_left(true, D, D, Xi11, T4, cnt);
+  o.gemm(true, false, D, 1, D, 1.0, e2.Z, e1.b, 0.0, v1, cnt);
+  o.copy(v2, e2.eta, D, 1, cnt);
+  o.gemm(false, false, D, 1, D, -1.0, e2.Z, v1, 1.0, v2, cnt);
+  o.gemm(true, false, D, 1, D, 1.0, e1.U, v2, 0.0, v3, cnt);
+  o.gemm(true, false, D, 1, D, -1.0, T4, v3, 1.0, v2, cnt);
+  o.copy(out.eta, e1.eta, D, 1, cnt);
+  o.gemm(true, false, D, 1, D, 1.0, e1.A, v2, 1.0, out.eta, cnt);
+  // Z = tria([A1^T Xi22 | Z1])
+  o.gemm(true, false, D, D, D, 1.0, e1.A, Xi22, 0.0, Y.at(0, 0), cnt);
+  o.copy(Y.at(0, D), e1.Z, D, D, cnt);
+  o.lq(Y, D, 2 * D, cnt);
+  o.copy(out.Z, Y.at(0, 0), D, D, cnt);
+  ar.off = mark;
+}
+
+// Smoother combine (smoother.py:53-63; under reverse=True e1 is the later aggregate)
+void smoother_fn(Ops& o, Arena& ar, int D, const SE& e1, const SE& e2, const SE& out, int cnt, bool with_chol) {
+  if (cnt <= 0 || o.err) return;
+  const size_t mark = ar.off;
+  o.copy(out.g, e2.g, D, 1, cnt);
+  o.gemm(false, false, D, 1, D, 1.0, e2.E, e1.g, 1.0, out.g, cnt);
+  o.gemm(false, false, D, D, D, 1.0, e2.E, e1.E, 0.0, out.E, cnt);
+  if (with_chol) {
+    const long long DD = (long long)D * D;
+    V Y{ar.take((size_t)cnt * 2 * DD), 2 * DD, 2 * D};
+    if (!Y.p) {
+      o.fail("workspace", -2);
+      return;
+    }
+    o.gemm(false, false, D, D, D, 1.0, e2.E, e1.Dm, 0.0, Y.at(0, 0), cnt);
+    o.copy(Y.at(0, D), e2.Dm, D, D, cnt);
+    o.lq(Y, D, 2 * D, cnt);
+    o.copy(out.Dm, Y.at(0, 0), D, D, cnt);
+  }
+  ar.off = mark;
+}
+
+// JAX's associative_scan (odd/even recursion) of n elements `el` into `out`
+// (inclusive prefixes; out must not alias el).
+template <typename E, typename Fn, typename Alloc, typename Ok>
+void scan(Ops& o, Arena& ar, int D, const E& el, const E& out, long long n, Fn fn, Alloc alloc, Ok ok,
+          void (*copy1)(Ops&, int, const E&, const E&)) {
+  if (o.err || n <= 0) return;
+  if (n == 1) {
+    copy1(o, D, out, el);
+    return;
+  }
+  const size_t mark = ar.off;
+  const long long h = n / 2;
+  E red = alloc(ar, D, h);
+  if (!ok(red)) {
+    o.fail("workspace", -2);
+    return;
+  }
+  fn(o, ar, D, el.every(2), el.item(1).every(2), red, (int)h);
+  // odd positions of out hold the scan of the pair sums
+  const E odd = out.item(1).every(2);
+  scan(o, ar, D, red, odd, h, fn, alloc, ok, copy1);
+  // even positions: out[2j] = odd[j - 1] o el[2j], j >= 1
+  const long long ne = (n % 2 == 0) ? h - 1 : h;
+  fn(o, ar, D, odd, el.item(2).every(2), out.item(2).every(2), (int)ne);
+  copy1(o, D, out, el);
+  ar.off = mark;
+}
+
+void fe_copy1(Ops& o, int D, const FE& dst, const FE& src) {
+  o.copy(dst.A, src.A, D, D, 1);
+  o.copy(dst.b, src.b, D, 1, 1);
+  o.copy(dst.U, src.U, D, D, 1);
+  o.copy(dst.eta, src.eta, D, 1, 1);
+  o.copy(dst.Z, src.Z, D, D, 1);
+}
+void se_copy1(Ops& o, int D, const SE& dst, const SE& src) {
+  o.copy(dst.g, src.g, D, 1, 1);
+  o.copy(dst.E, src.E, D, D, 1);
+  o.copy(dst.Dm, src.Dm, D, D, 1);
+}
+
+}  // namespace
+
+struct Engine {
+  int device = 0;
+  rocblas_handle h = nullptr;
+  char* buf = nullptr;
+  size_t cap = 0;
+};
+
+Engine* engine_create(int device) {
+  Engine* e = new Engine();
+  e->device = device;
+  if (rocblas_create_handle(&e->h) != rocblas_status_success) {
+    delete e;
+    return nullptr;
+  }
+  rocblas_set_pointer_mode(e->h, rocblas_pointer_mode_host);
+  return e;
+}
+
+void engine_destroy(Engine* e) {
+  if (!e) return;
+  if (e->buf) (void)hipFree(e->buf);
+  if (e->h) rocblas_destroy_handle(e->h);
+  delete e;
+}
+
+bool supports(int kind, int D, int d) {
+  return kind == kLorenz96 && d >= 4 && D % d == 0 && D / d >= 2 && 2 * D <= kLqMaxN;
+}
+
+namespace {
+
+// workspace doubles for one solve
+size_t workspace_doubles(int D, int d, long long Ns) {
+  const size_t DD = (size_t)D * D;
+  const size_t n1 = (size_t)Ns + 1;
+  size_t w = 0;
+  w += 2 * n1 * D;                          // two mean trajectories
+  w += (size_t)Ns * ((size_t)d * D + d);    // H, c
+  w += 2 * (size_t)Ns * (3 * DD + 2 * D);   // leaf elements + scan outputs (A, eta, Z); b, U go to fm / fL
+  w += n1 * (D + DD);                       // filtered fm, fL
+  w += 2 * (size_t)Ns * (3 * DD + 2 * D);   // scan recursion (pair sums over all levels)
+  w += 4 * n1 * (D + 2 * DD);               // smoother elements natural / reversed / scan out / recursion
+  w += n1 * (D + DD);                       // smoothed natural order
+  // per-call temporaries at the widest level (the filter combine), and the post step
+  const size_t h = (size_t)(Ns / 2 + 1);
+  w += h * (4 * DD + 2 * DD + 4 * DD + 3 * D) + 4096;
+  w += (size_t)Ns * (4 * DD + (size_t)d * D + 2 * D + 3 * d + 4) + 4096;
+  w += 8 * n1 + 64 * 1024;
+  return w + w / 10;
+}
+
+}  // namespace
+
+int solve(Engine* e, const Problem& p, const pof_opts& opts, pof_out* out, const EngineFns* small, hipStream_t st,
+          std::string& err) {
+  const int D = p.D, d = p.d, q1 = p.q + 1;
+  const long long Ns = p.Ns;
+  const long long DD = (long long)D * D;
+  if (Ns < 1) {
+    err = "large engine needs Ns >= 1";
+    return POF_E_ARG;
+  }
+  if (!small && !supports(p.kind, D, d)) {
+    err = "large-state engine: no device linearisation for this field and (D, d)";
+    return POF_E_UNSUPPORTED;
+  }
+  if (2 * D > kLqMaxN) {
+    err = "large-state engine: D too large";
+    return POF_E_UNSUPPORTED;
+  }
+  // workspace
+  const size_t need = workspace_doubles(D, d, Ns) * sizeof(double);
+  if (e->cap < need) {
+    if (e->buf) (void)hipFree(e->buf);
+    e->buf = nullptr;
+    e->cap = 0;
+    if (hipMalloc((void**)&e->buf, need) != hipSuccess) {
+      err = "large-state engine: workspace allocation failed";
+      return POF_E_HIP;
+    }
+    e->cap = need;
+  }
+  Arena ar{e->buf, e->cap, 0};
+  Ops o{e->h, st};
+  rocblas_set_stream(e->h, st);
+  // constants: F, QL, N1z, P, x0 (one block)
+  double* cst = ar.take(4 * DD + 2 * D);
+  double* mA = ar.take((size_t)(Ns + 1) * D);
+  double* mB = ar.take((size_t)(Ns + 1) * D);
+  double* H = ar.take((size_t)Ns * d * D);
+  double* cc = ar.take((size_t)Ns * d);
+  double* fm = ar.take((size_t)(Ns + 1) * D);
+  double* fL = ar.take((size_t)(Ns + 1) * DD);
+  double* sm = ar.take((size_t)(Ns + 1) * D);
+  double* sL = ar.take((size_t)(Ns + 1) * DD);
+  double* terms = ar.take(4 * (size_t)(Ns + 1) + 2 * (size_t)Ns * d + 8);
+  if (!cst || !mA || !mB || !H || !cc || !fm || !fL || !sm || !sL || !terms) {
+    err = "large-state engine: workspace too small";
+    return POF_E_HIP;
+  }
+  double* dF = cst;
+  double* dQL = cst + DD;
+  double* dN1z = cst + 2 * DD;
+  double* dL0 = cst + 3 * DD;
+  double* dP = cst + 4 * DD;
+  double* dm0 = dP + D;
+  double* t_ssq = terms;
+  double* t_nll = t_ssq + (Ns + 1);
+  double* t_obj = t_nll + (Ns + 1);
+  double* t_viol = t_obj + (Ns + 1);
+  double* t_tmp = t_viol + (Ns + 1);
+  double* red = t_tmp + 2 * Ns * d;
+  hipError_t he = hipSuccess;
+  auto up = [&](double* dst, const double* src, size_t n) {
+    if (he == hipSuccess) he = hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, st);
+  };
+  up(dF, p.F, DD);
+  up(dQL, p.QL, DD);
+  up(dN1z, p.N1z, DD);
+  up(dL0, p.x0L, DD);
+  up(dP, p.P, D);
+  up(dm0, p.x0m, D);
+  up(mA, p.m0, (size_t)(Ns + 1) * D);
+  if (he != hipSuccess) {
+    err = std::string("large-state engine upload: ") + hipGetErrorString(he);
+    return POF_E_HIP;
+  }
+  const V Fv{dF, 0, D}, QLv{dQL, 0, D}, N1zv{dN1z, 0, D};
+  const size_t base_mark = ar.off;
+
+  double* mcur = mA;
+  double* mprev = mB;
+  int k = 0;
+  double nll = 0.0, obj = 0.0, ssq = 0.0;
+  int status = POF_STATUS_CONVERGED;
+  ModelArgs la{};
+  la.Pd = dP;
+  la.kind = p.kind;
+  la.q = p.q;
+  for (int i = 0; i < kMaxParams; ++i) la.params[i] = p.params[i];
+  for (;;) {
+    ar.off = base_mark;
+    // ---- EK1 at the current means (step.py:12-22)
+    if (small) {
+      la.Ns = (int)Ns;
+      he = small->linearize(la, mcur + D, (int)Ns, H, cc, st);
+    } else {
+      const long long nt = Ns * d;
+      hipLaunchKernelGGL(k_lin_l96, dim3((unsigned)((nt + 127) / 128)), dim3(128), 0, st, mcur + D, dP,
+                         p.params[0], (int)Ns, d, q1, H, cc);
+      he = hipGetLastError();
+    }
+    if (he != hipSuccess) o.fail("linearize", (int)he);
+    const V Hv{H, (long long)d * D, D}, cv{cc, d, 1};
+    // ---- filter elements (filter.py:50-81): all outputs of element k in leaf[k]
+    FE leaf = fe_alloc(ar, D, Ns);
+    if (!fe_ok(leaf)) o.fail("workspace", -2);
+    {
+      const size_t mark = ar.off;
+      const int n = (int)Ns;
+      V Pb{ar.take((size_t)n * (d + D) * D), (long long)(d + D) * D, D};
+      V Kb{ar.take((size_t)n * D * d), (long long)D * d, d};
+      V HF{ar.take((size_t)n * d * D), (long long)d * D, D};
+      V Zt{ar.take((size_t)n * d * D), (long long)d * D, D};
+      V X2{ar.take((size_t)n * d * D), (long long)d * D, D};
+      V t0{ar.take((size_t)d + 8), 0, 1};
+      V m1{ar.take((size_t)D + 8), 0, 1};
+      V N10{ar.take((size_t)2 * DD), 2 * DD, 2 * D};
+      if (!Pb.p || !Kb.p || !HF.p || !Zt.p || !X2.p || !t0.p || !m1.p || !N10.p) o.fail("workspace", -2);
+      // element 0: m1 = F m0, N1 = tria([F L0 | QL]); elements k >= 1: m1 = 0, N1 = tria([0 | QL])
+      o.gemm(false, false, D, 1, D, 1.0, Fv, V{dm0, 0, 1}, 0.0, m1, 1);
+      o.gemm(false, false, D, D, D, 1.0, Fv, V{dL0, 0, D}, 0.0, N10.at(0, 0), 1);
+      o.copy(N10.at(0, D), QLv, D, D, 1);
+      o.lq(N10, D, 2 * D, 1);
+      for (int part = 0; part < 2; ++part) {
+        const int k0 = part == 0 ? 0 : 1;
+        const int cnt = part == 0 ? 1 : n - 1;
+        if (cnt <= 0) continue;
+        const V N1 = part == 0 ? V{N10.p, 0, 2 * D} : N1zv;
+        const V Hk = Hv.item(k0), ck = cv.item(k0);
+        const FE ok = leaf.item(k0);
+        const V P0 = Pb.item(k0), K0 = Kb.item(k0), HF0 = HF.item(k0), Z0 = Zt.item(k0), X0 = X2.item(k0);
+        // Psi = tria([[H N1], [N1]]) (the zero column block of filter.py:63-66 changes no pivot)
+        o.gemm(false, false, d, D, D, 1.0, Hk, N1, 0.0, P0.at(0, 0), cnt);
+        o.copy(P0.at(d, 0), N1, D, D, cnt);
+        o.lq(P0, d + D, D, cnt);
+        // U = Psi22 = [Psi[d:, d:D] | 0]
+        o.copy(ok.U, P0.at(d, d), D, D - d, cnt);
+        o.fill(ok.U.at(0, D - d), D, d, cnt, 0.0);
+        // K = Psi21 Psi11^-1
+        o.copy(K0, P0.at(d, 0), D, d, cnt);
+        o.trsm_right(false, D, d, P0.at(0, 0), K0, cnt);
+        // A = F - K (H F)
+        o.gemm(false, false, d, D, D, 1.0, Hk, Fv, 0.0, HF0, cnt);
+        o.copy(ok.A, Fv, D, D, cnt);
+        o.gemm(false, false, D, D, d, -1.0, K0, HF0, 1.0, ok.A, cnt);
+        // b = m1 - K (H m1 + c)
+        if (part == 0) {
+          o.copy(t0, ck, d, 1, 1);
+          o.gemm(false, false, d, 1, D, 1.0, Hk, m1, 1.0, t0, 1);
+          o.copy(ok.b, m1, D, 1, 1);
+          o.gemm(false, false, D, 1, d, -1.0, K0, t0, 1.0, ok.b, 1);
+        } else {
+          o.gemm(false, false, D, 1, d, -1.0, K0, ck, 0.0, ok.b, cnt);
+        }
+        // Z = [(Psi11^-1 H F)^T | 0]; eta = -(Psi11^-T Psi11^-1 H F)^T c
+        o.copy(Z0, HF0, d, D, cnt);
+        o.trsm_left(false, d, D, P0.at(0, 0), Z0, cnt);
+        o.copy(ok.Z, Z0, D, d, cnt, true);
+        o.fill(ok.Z.at(0, d), D, D - d, cnt, 0.0);
+        o.copy(X0, Z0, d, D, cnt);
+        o.trsm_left(true, d, D, P0.at(0, 0), X0, cnt);
+        o.gemm(true, false, D, 1, d, -1.0, X0, ck, 0.0, ok.eta, cnt);
+      }
+      ar.off = mark;
+    }
+    // ---- filter scan (filter.py:35): prefixes land in fm / fL (items 1..Ns) and scratch
+    {
+      FE outp;
+      outp.b = V{fm + D, D, 1};
+      outp.U = V{fL + DD, DD, D};
+      outp.A = V{ar.take((size_t)Ns * DD), DD, D};
+      outp.eta = V{ar.take((size_t)Ns * D), D, 1};
+      outp.Z = V{ar.take((size_t)Ns * DD), DD, D};
+      if (!fe_ok(outp)) o.fail("workspace", -2);
+      scan<FE>(o, ar, D, leaf, outp, Ns, filter_fn, fe_alloc, fe_ok, fe_copy1);
+      o.copy(V{fm, D, 1}, V{dm0, 0, 1}, D, 1, 1);
+      o.copy(V{fL, DD, D}, V{dL0, 0, D}, D, D, 1);
+    }
+    // ---- smoother elements and observation statistics (smoother.py:25-50, filter.py:84-114)
+    SE snat = se_alloc(ar, D, Ns + 1);
+    if (!se_ok(snat)) o.fail("workspace", -2);
+    {
+      const size_t mark = ar.off;
+      const int n = (int)Ns;
+      V Phi{ar.take((size_t)n * 4 * DD), 4 * DD, 2 * D};
+      V Fm{ar.take((size_t)n * D), D, 1};
+      V Sb{ar.take((size_t)n * d * D), (long long)d * D, D};
+      V yv{ar.take((size_t)n * d), d, 1};
+      if (!Phi.p || !Fm.p || !Sb.p || !yv.p) o.fail("workspace", -2);
+      const V fmv{fm, D, 1}, fLv{fL, DD, D};
+      o.gemm(false, false, D, D, D, 1.0, Fv, fLv, 0.0, Phi.at(0, 0), n);
+      o.copy(Phi.at(0, D), QLv, D, D, n);
+      o.copy(Phi.at(D, 0), fLv, D, D, n);
+      o.fill(Phi.at(D, D), D, D, n, 0.0);
+      o.lq(Phi, 2 * D, 2 * D, n);
+      // E = Phi21 Phi11^-1, D = Phi22, g = m - E F m
+      o.copy(snat.E, Phi.at(D, 0), D, D, n);
+      o.trsm_right(false, D, D, Phi.at(0, 0), snat.E, n);
+      o.copy(snat.Dm, Phi.at(D, D), D, D, n);
+      o.gemm(false, false, D, 1, D, 1.0, Fv, fmv, 0.0, Fm, n);
+      o.copy(snat.g, fmv, D, 1, n);
+      o.gemm(false, false, D, 1, D, -1.0, snat.E, Fm, 1.0, snat.g, n);
+      // S = tria(H Phi11), yhat = H F m + c
+      o.gemm(false, false, d, D, D, 1.0, Hv, Phi.at(0, 0), 0.0, Sb, n);
+      o.lq(Sb, d, D, n);
+      o.copy(yv, cv, d, 1, n);
+      o.gemm(false, false, d, 1, D, 1.0, Hv, Fm, 1.0, yv, n);
+      if (!o.err) {
+        hipLaunchKernelGGL(k_obs_terms, dim3((n + 127) / 128), dim3(128), 0, st, Sb.p, Sb.s, Sb.ld, yv.p, yv.s, d, n,
+                           t_ssq, t_nll, t_tmp);
+        o.check("k_obs_terms");
+      }
+      // terminal element (m_Ns, 0, L_Ns)
+      o.copy(snat.g.item(Ns), fmv.item(Ns), D, 1, 1);
+      o.fill(snat.E.item(Ns), D, D, 1, 0.0);
+      o.copy(snat.Dm.item(Ns), fLv.item(Ns), D, D, 1);
+      ar.off = mark;
+    }
+    // ---- reverse scan (smoother.py:15-17): reversed inputs, scan, reversed outputs
+    {
+      const size_t mark = ar.off;
+      const long long n = Ns + 1;
+      SE srev = se_alloc(ar, D, n), sout = se_alloc(ar, D, n);
+      if (!se_ok(srev) || !se_ok(sout)) o.fail("workspace", -2);
+      o.copy(srev.g, snat.g, D, 1, (int)n, false, true);
+      o.copy(srev.E, snat.E, D, D, (int)n, false, true);
+      o.copy(srev.Dm, snat.Dm, D, D, (int)n, false, true);
+      auto sfn = [](Ops& oo, Arena& aa, int DD2, const SE& a, const SE& b, const SE& c, int cnt) {
+        smoother_fn(oo, aa, DD2, a, b, c, cnt, true);
+      };
+      scan<SE>(o, ar, D, srev, sout, n, sfn, se_alloc, se_ok, se_copy1);
+      o.copy(V{sm, D, 1}, sout.g, D, 1, (int)n, false, true);
+      o.copy(V{sL, DD, D}, sout.Dm, D, D, (int)n, false, true);
+      ar.off = mark;
+    }
+    // ---- objective (utils.py:97-101) and the convergence partials
+    {
+      const size_t mark = ar.off;
+      V r{ar.take((size_t)Ns * D), D, 1};
+      if (!r.p) o.fail("workspace", -2);
+      o.copy(r, V{sm, D, 1}, D, 1, (int)Ns);
+      o.gemm(false, false, D, 1, D, -1.0, Fv, V{sm + D, D, 1}, 1.0, r, (int)Ns);
+      o.trsm_left(false, D, 1, QLv, r, (int)Ns);
+      if (!o.err) {
+        hipLaunchKernelGGL(k_sqnorm, dim3((unsigned)((Ns + 127) / 128)), dim3(128), 0, st, r.p, D, (int)Ns, t_obj);
+        hipLaunchKernelGGL(k_viol, dim3((unsigned)((Ns + 128) / 128)), dim3(128), 0, st, sm, mcur, D, (int)(Ns + 1),
+                           t_viol);
+        hipLaunchKernelGGL(k_reduce4, dim3(1), dim3(256), 0, st, t_ssq, t_nll, t_obj, (int)Ns, t_viol, (int)(Ns + 1),
+                           red);
+        o.check("reductions");
+      }
+      ar.off = mark;
+    }
+    if (o.err) {
+      err = "large-state engine: " + o.msg;
+      return POF_E_HIP;
+    }
+    double hr[4] = {0, 0, 0, 0};
+    he = hipMemcpyAsync(hr, red, 4 * sizeof(double), hipMemcpyDeviceToHost, st);
+    if (he == hipSuccess) he = hipStreamSynchronize(st);
+    if (he != hipSuccess) {
+      err = std::string("large-state engine: ") + hipGetErrorString(he);
+      return POF_E_HIP;
+    }
+    k += 1;
+    const double obj_old = obj;
+    ssq = hr[0] / (double)Ns / (double)d;
+    nll = hr[1];
+    obj = hr[2];
+    const bool means_close = hr[3] == 0.0;
+    const bool isnan_ = std::isnan(obj) || std::isnan(nll);
+    // the smoothed means feed the next linearisation
+    he = hipMemcpyAsync(mprev, sm, (size_t)(Ns + 1) * D * sizeof(double), hipMemcpyDeviceToDevice, st);
+    std::swap(mcur, mprev);
+    bool finished = false;
+    if (opts.fixed_iters > 0) {
+      if (k >= opts.fixed_iters) {
+        status = POF_STATUS_FIXED;
+        finished = true;
+      }
+    } else {
+      const bool obj_conv = std::fabs(obj_old - obj) <= 1e-9 + 1e-6 * std::fabs(obj);
+      if (!(k <= opts.maxiters)) {
+        status = POF_STATUS_MAXITER;
+        finished = true;
+      } else if (isnan_ || obj_conv || means_close) {
+        status = isnan_ ? POF_STATUS_NAN : POF_STATUS_CONVERGED;
+        finished = true;
+      }
+    }
+    if (finished) break;
+  }
+  // ---- outputs: factors calibrated twice (step.py:42-44 + solver.py:66-68)
+  const double scale = opts.calibrate ? ssq : std::sqrt(ssq);
+  const long long ne = (Ns + 1) * D;
+  hipLaunchKernelGGL(k_project, dim3((unsigned)((ne + 127) / 128)), dim3(128), 0, st, sm, sL, dP, D, d,
+                     (int)(Ns + 1), scale, out->means_dev, out->chols_dev, out->full_means_dev, out->full_chols_dev);
+  he = hipGetLastError();
+  if (he == hipSuccess) he = hipStreamSynchronize(st);
+  if (he != hipSuccess) {
+    err = std::string("large-state engine output: ") + hipGetErrorString(he);
+    return POF_E_HIP;
+  }
+  out->iterations = (double)k;
+  out->nll = nll;
+  out->obj = obj;
+  out->sigma_squared = ssq;
+  out->calibrated = opts.calibrate ? 1 : 0;
+  out->status = status;
+  return POF_OK;
+}
+
+}  // namespace large
+}  // namespace pof

@@ -7,7 +7,8 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG.parent / "csrc"
 SOURCES = ["pof_inst_d1.hip", "pof_inst_d2.hip", "pof_inst_d3.hip", "pof_inst_d4.hip", "pof_registry.hip",
-           "pof_abi.hip"]
+           "pof_large.hip", "pof_abi.hip"]
+ROCM_LIB = "/opt/rocm/lib"
 FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
@@ -47,7 +48,8 @@ def build(verbose=False, jobs=None):
         if verbose and text.strip():
             print(text)
     tmp = out.with_suffix(".so.tmp")
-    cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
+    cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp),
+           f"-L{ROCM_LIB}", "-lrocblas", f"-Wl,-rpath,{ROCM_LIB}"]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout.decode(errors='replace')}")

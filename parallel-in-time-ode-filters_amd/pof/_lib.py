@@ -93,6 +93,7 @@ SIGNATURES = {
     "pof_profile_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pof_profile_read": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _P(ctypes.c_longlong), _P(ctypes.c_longlong)]),
     "pof_solve": (ctypes.c_int, [_vp, _P(Problem), _P(Opts), _P(Out), _vp]),
+    "pof_solve_large": (ctypes.c_int, [_vp, _P(Problem), _P(Opts), _P(Out), _vp]),
     "pof_setup": (ctypes.c_int, [_P(Problem), _dp, _dp, _dp, _dp, _dp, _dp]),
     "pof_transitions": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _d, _dp, _dp, _dp, _dp]),
     "pof_initial_means": (ctypes.c_int, [_P(Problem), _dp]),

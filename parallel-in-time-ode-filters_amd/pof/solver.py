@@ -7,7 +7,7 @@ from .utils import MVNSqrt
 
 
 def solve(*, f, y0, ts, order, init="prior", calibrate=True, maxiters=10_000, sequential=False,
-          fixed_iters=None, return_full_states=False):
+          fixed_iters=None, return_full_states=False, engine="auto"):
     """Parallel-in-time IEKS solve on the GPU (``pof/solver.py:11-73``).
 
     Same keyword arguments and result as the reference:
@@ -19,6 +19,9 @@ def solve(*, f, y0, ts, order, init="prior", calibrate=True, maxiters=10_000, se
     preconditioned states in ``info["states"]``.  ``info["status"]`` tells
     how the loop ended.  ``sequential=True`` runs each iteration as the
     sequential filter + smoother (``step.py:39-40``), one GPU worker per IVP.
+    ``engine="large"`` forces the large-state engine (batched LQ / GEMM
+    primitives per scan level), which ``"auto"`` picks by itself for state
+    dimensions beyond the per-worker kernels (Lorenz-96, D = 128).
     """
     import torch
 
@@ -41,10 +44,11 @@ def solve(*, f, y0, ts, order, init="prior", calibrate=True, maxiters=10_000, se
     out.full_chols_dev = fL.data_ptr() if fL is not None else None
     prob = spec.cstruct()
     ctx = _lib.context(dev.index)
-    _lib.check(
-        _lib.load().pof_solve(ctx, ctypes.byref(prob), ctypes.byref(opts), ctypes.byref(out), _lib.stream_ptr(dev)),
-        ctx, "pof_solve",
-    )
+    if engine not in ("auto", "large"):
+        raise ValueError(f"engine={engine!r}: expected 'auto' or 'large'")
+    fn = _lib.load().pof_solve_large if engine == "large" else _lib.load().pof_solve
+    _lib.check(fn(ctx, ctypes.byref(prob), ctypes.byref(opts), ctypes.byref(out), _lib.stream_ptr(dev)), ctx,
+               "pof_solve")
     info = {
         "iterations": float(out.iterations),
         "nll": float(out.nll),
