@@ -6,10 +6,12 @@ IWP order q = 3 (state dimension D = 12), float64 throughout, constant
 initial trajectory, exactly 10 IEKS iterations per solve.  Inputs are tiny
 (y0, ts); everything else lives in HBM.
 
-Also measured (same line, objects "batched" and "served"): a batch of 2048
-mixed IVPs solved together vs the plain one-IVP-per-launch scan, and a
-Poisson request stream served with continuous batching vs a FIFO plain
-per-request scan (p50/p99 latency, solves/s).
+Also measured (same line, objects "batched", "served" and "precision"): a
+batch of 2048 mixed IVPs solved together vs the plain one-IVP-per-launch
+scan; a Poisson request stream served with continuous batching vs a FIFO
+plain per-request scan (p50/p99 latency, solves/s); and Lorenz-96 with
+d = 32, q = 3 (D = 128) on the large-state engine (ms per iteration at
+t in [0, 5] with 8192 steps, errors against the oracle on a short prefix).
 
 Prints one JSON line: solves/s over K timed solves, the roofline of the
 dominant kernel class (algorithmic flops per launch over its mean launch
@@ -272,6 +274,63 @@ def served_run(n_req, rate, maxiters, seed=5):
     }
 
 
+def precision_run(ns=8192, iters=3, check_ns=32, check_iters=3):
+    """Lorenz-96 (d = 32, F = 8, y0 = F + 0.01 N(0, 1) from seed 0), t in [0, 5] with 8192 steps, q = 3:
+    D = 128 on the large-state engine (batched LQ / GEMM primitives per scan level), float64.
+
+    Timed: one solve of `iters` fixed iterations after an untimed one.  Errors:
+    the first `check_ns` steps (same dt) after `check_iters` iterations against
+    the oracle: max relative mean error (pof.workloads.max_rel_err), the
+    covariance error ||c c^T - c_ref c_ref^T|| / ||c_ref c_ref^T|| after dividing
+    out sigma_squared^2 (sign-free; sigma_squared itself depends on factor signs
+    for d >= 2), and the relative sigma_squared difference.
+    """
+    import numpy as np
+    import torch
+
+    import pof.ivp
+    import pof.solver
+    from oracle import problems as op
+    from oracle import solver as osol
+    from pof import workloads
+
+    ivp = pof.ivp.lorenz96(d=32)
+    ts = np.linspace(0.0, 5.0 * ns / 8192, ns + 1)
+
+    def run(n_it):
+        return pof.solver.solve(f=ivp.f, y0=ivp.y0, ts=ts, order=3, init="constant", fixed_iters=n_it)
+
+    run(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ys, info = run(iters)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    finite = bool(torch.isfinite(ys.mean).all())
+    tsc = ts[: check_ns + 1]
+    yc, ic = pof.solver.solve(f=ivp.f, y0=ivp.y0, ts=tsc, order=3, init="constant", fixed_iters=check_iters)
+    torch.cuda.synchronize()
+    ref, rinfo = osol.solve(op.lorenz96(d=32), tsc, 3, init="constant", fixed_iters=check_iters)
+    elem, norm = workloads.max_rel_err(yc.mean.cpu().numpy(), ref.mean)
+    c = yc.chol.cpu().numpy() / ic["sigma_squared"]
+    cr = ref.chol / rinfo["sigma_squared"]
+    cc = np.einsum("nij,nkj->nik", c, c)
+    ccr = np.einsum("nij,nkj->nik", cr, cr)
+    cov_err = float(np.linalg.norm(cc - ccr) / np.linalg.norm(ccr))
+    return {
+        "workload": f"Lorenz-96 d=32 F=8 seed 0, t=[0,{5.0 * ns / 8192:g}] Ns={ns} q=3 (D=128), f64, constant init, "
+                    f"{iters} fixed iterations, large-state engine",
+        "ms_per_iteration": 1e3 * t / iters,
+        "seconds": t,
+        "finite": finite,
+        "check": f"first {check_ns} steps, {check_iters} fixed iterations, vs the FP64 oracle",
+        "max_rel_err_means": elem,
+        "normwise_err_means": norm,
+        "cov_rel_err": cov_err,
+        "sigma_squared_rel_diff": abs(ic["sigma_squared"] - rinfo["sigma_squared"]) / abs(rinfo["sigma_squared"]),
+    }
+
+
 def copy_probe():
     """A plain device copy of known size (2^25 float64 = 256 MiB read + 256 MiB written per launch).
 
@@ -298,6 +357,10 @@ def main():
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--copy-probe", action="store_true", help="only run the HBM counter calibration copy")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched and served measurements")
+    ap.add_argument("--no-precision", action="store_true", help="skip the Lorenz-96 D = 128 measurement")
+    ap.add_argument("--precision-only", action="store_true",
+                    help="only the Lorenz-96 D = 128 measurement (printed as its own JSON object)")
+    ap.add_argument("--precision-ns", type=int, default=8192)
     ap.add_argument("--batch-ivps", type=int, default=2048)
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--serve-rate", type=float, default=None,
@@ -309,6 +372,12 @@ def main():
         sys.exit(2)
     if args.copy_probe:
         copy_probe()
+        return
+    if args.precision_only:
+        import torch
+
+        torch.cuda.set_device(0)
+        print(json.dumps({"precision": precision_run(args.precision_ns)}), flush=True)
         return
     g = gpu_run(args.steps, args.warmup)
     if args.verbose:
@@ -323,6 +392,9 @@ def main():
         served = served_run(args.serve_requests, rate, maxiters=100) if args.serve_requests > 0 else None
         if args.verbose:
             print(json.dumps(served), file=sys.stderr, flush=True)
+    precision = None if args.no_precision else precision_run(args.precision_ns)
+    if args.verbose and precision:
+        print(json.dumps(precision), file=sys.stderr, flush=True)
     cpu = None if args.no_cpu_baseline else cpu_baseline()
     ms_step = 1e3 * g["seconds"] / args.steps
     out = {
@@ -353,6 +425,7 @@ def main():
         "finite": g["finite"],
         "batched": batched,
         "served": served,
+        "precision": precision,
     }
     print(json.dumps(out), flush=True)
 

@@ -4,8 +4,8 @@
 // Each combine of a scan level is a sequence of batched primitives over every
 // combine of that level: a Householder LQ kernel (one workgroup per matrix,
 // LAPACK dlarfg's reflector convention, the same as the per-worker lq()),
-// rocBLAS strided-batched DGEMM / DTRSM for the plain products and triangular
-// solves (FP64 MFMA on gfx950), and small copy / fill / reduction kernels.  The
+// rocBLAS strided-batched DGEMM for the plain products (FP64 MFMA on gfx950),
+// a blocked triangular-solve kernel, and small copy / fill / reduction kernels.  The
 // host drives JAX's odd/even associative_scan recursion (pof/parallel_filtsmooth
 // filter.py:35, smoother.py:15-17) over strided views, so every factor is built
 // by the same sequence of operations as in the reference.
@@ -94,6 +94,196 @@ __global__ __launch_bounds__(kLqThreads) void k_lq(double* A, long long stride, 
     if (tid == 0) pr[i] = beta;
     for (int j = i + 1 + tid; j < n; j += kLqThreads) pr[j] = 0.0;
     __syncthreads();
+  }
+}
+
+constexpr int kNB = 16;  // rows per panel of the blocked LQ
+
+// One panel of the blocked LQ (LAPACK dgelqf: dgelq2 on the panel, dlarft
+// forward / rowwise): rows i0 .. i0 + np - 1 of item blockIdx.x, columns
+// i0 .. n - 1, factored by one wave with the panel in registers (lane holds
+// columns lane + 64 q) and the reflectors of k_lq.  Writes the panel's rows of
+// L back, the reflectors V (np x (n - i0), leading 1, zeros before it) and the
+// triangular factor T (np x np) with H_1 ... H_np = I - V^T T V.  Pivot column
+// k < kNB < 64 lives in slot 0 of lane k.
+template <int NQ>
+__global__ __launch_bounds__(64) void k_lq_panel(double* A, long long sA, int n, int ld, int i0, int np, double* Vb,
+                                                 long long sV, int ldV, double* Tb, long long sT, int ldT) {
+  __shared__ double G[kNB][kNB];
+  __shared__ double taus[kNB];
+  const int lane = threadIdx.x;
+  double* a = A + (long long)blockIdx.x * sA + (size_t)i0 * ld + i0;
+  const int nc = n - i0;
+  double P[kNB][NQ];
+#pragma unroll
+  for (int r = 0; r < kNB; ++r) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int c = lane + 64 * q;
+      P[r][q] = (r < np && c < nc) ? a[(size_t)r * ld + c] : 0.0;
+    }
+  }
+  double betas[kNB];
+#pragma unroll
+  for (int k = 0; k < kNB; ++k) {
+    betas[k] = 0.0;
+    if (k < np) {
+      double sum = 0.0;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const int c = lane + 64 * q;
+        sum = (c > k && c < nc) ? fma(P[k][q], P[k][q], sum) : sum;
+      }
+      const double sig = wave_sum(sum);
+      const double alpha = __shfl(P[k][0], k, 64);
+      double tau = 0.0, beta = alpha, scal = 0.0;
+      if (sig != 0.0) {
+        const double nrm = sqrt(fma(alpha, alpha, sig));
+        beta = -copysign(nrm, alpha);
+        tau = (beta - alpha) / beta;
+        scal = 1.0 / (alpha - beta);
+      }
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const int c = lane + 64 * q;
+        P[k][q] = (c > k && c < nc) ? P[k][q] * scal : P[k][q];
+      }
+      betas[k] = beta;
+      if (lane == 0) taus[k] = tau;
+      if (tau != 0.0) {
+#pragma unroll
+        for (int r = k + 1; r < kNB; ++r) {
+          if (r < np) {
+            double w = 0.0;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+              const int c = lane + 64 * q;
+              w = (c > k && c < nc) ? fma(P[r][q], P[k][q], w) : w;
+            }
+            w = wave_sum(w) + __shfl(P[r][0], k, 64);
+            const double tw = tau * w;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+              const int c = lane + 64 * q;
+              P[r][q] = (c > k && c < nc) ? fma(-tw, P[k][q], P[r][q]) : P[r][q];
+            }
+            if (lane == k) P[r][0] -= tw;
+          }
+        }
+      }
+    }
+  }
+  // L rows back to A; V rows out
+  double* Vo = Vb + (long long)blockIdx.x * sV;
+#pragma unroll
+  for (int k = 0; k < kNB; ++k) {
+    if (k < np) {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const int c = lane + 64 * q;
+        if (c < nc) {
+          a[(size_t)k * ld + c] = (c < k) ? P[k][q] : ((c == k) ? betas[k] : 0.0);
+          Vo[(size_t)k * ldV + c] = (c < k) ? 0.0 : ((c == k) ? 1.0 : P[k][q]);
+        }
+      }
+    }
+  }
+  // G[l][k] = v_l . v_k (l < k) = v_l[k] + sum_{c > k} v_l[c] v_k[c]
+#pragma unroll
+  for (int k = 1; k < kNB; ++k) {
+#pragma unroll
+    for (int l = 0; l < k; ++l) {
+      if (k < np) {
+        double w = 0.0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          const int c = lane + 64 * q;
+          w = (c > k && c < nc) ? fma(P[l][q], P[k][q], w) : w;
+        }
+        w = wave_sum(w) + __shfl(P[l][0], k, 64);
+        if (lane == 0) G[l][k] = w;
+      }
+    }
+  }
+  __syncthreads();
+  // T (dlarft): lane j holds row j; T[j][k] = sum_{l = j}^{k - 1} T[j][l] (-tau_k G[l][k]), T[k][k] = tau_k
+  double trow[kNB];
+#pragma unroll
+  for (int k = 0; k < kNB; ++k) {
+    double acc = 0.0;
+#pragma unroll
+    for (int l = 0; l < k; ++l) acc = (l >= lane) ? fma(trow[l], -taus[k] * G[l][k], acc) : acc;
+    trow[k] = (k < np) ? ((lane < k) ? acc : ((lane == k) ? taus[k] : 0.0)) : 0.0;
+  }
+  if (lane < np) {
+    double* T = Tb + (long long)blockIdx.x * sT + (size_t)lane * ldT;
+#pragma unroll
+    for (int k = 0; k < kNB; ++k)
+      if (k < np) T[k] = trow[k];
+  }
+}
+
+constexpr int kTB = 16;        // rows per block of the triangular solve
+constexpr int kTrsmMaxN = 256;  // order of the triangular matrix at most
+
+// op(L) X = B in place for item blockIdx.y: L lower n x n, B n x m, op = L
+// (forward substitution) or L^T (backward).  One thread per right-hand side
+// column; blocks of kTB rows: the block's rows (or columns, for L^T) of L are
+// staged in LDS, the earlier blocks' solutions are subtracted, then the
+// diagonal block is solved in registers.
+__global__ __launch_bounds__(256) void k_trsm(const double* L, long long sL, int ldL, double* B, long long sB,
+                                              int ldB, int n, int m, int trans) {
+  __shared__ double Ls[kTB][kTrsmMaxN + 1];
+  const double* l = L + (long long)blockIdx.y * sL;
+  double* b = B + (long long)blockIdx.y * sB;
+  const int tid = threadIdx.x;
+  const int c = blockIdx.x * 256 + tid;
+  const int nb = (n + kTB - 1) / kTB;
+  for (int kb = 0; kb < nb; ++kb) {
+    const int K = trans ? (nb - 1 - kb) : kb;
+    const int r0 = K * kTB;
+    const int rn = (n - r0) < kTB ? (n - r0) : kTB;
+    __syncthreads();
+    for (int e = tid; e < rn * n; e += 256) {
+      const int r = e / n, t = e % n;
+      Ls[r][t] = trans ? l[(size_t)t * ldL + r0 + r] : l[(size_t)(r0 + r) * ldL + t];
+    }
+    __syncthreads();
+    if (c < m) {
+      double acc[kTB];
+#pragma unroll
+      for (int r = 0; r < kTB; ++r) acc[r] = (r < rn) ? b[(size_t)(r0 + r) * ldB + c] : 0.0;
+      if (!trans) {
+        for (int t = 0; t < r0; ++t) {
+          const double x = b[(size_t)t * ldB + c];
+#pragma unroll
+          for (int r = 0; r < kTB; ++r) acc[r] = fma(-Ls[r][t], x, acc[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < kTB; ++r) {
+#pragma unroll
+          for (int t = 0; t < kTB; ++t)
+            if (t < r) acc[r] = fma(-Ls[r][r0 + t], acc[t], acc[r]);
+          acc[r] = (r < rn) ? acc[r] / Ls[r][r0 + r] : 0.0;
+        }
+      } else {
+        for (int t = r0 + rn; t < n; ++t) {
+          const double x = b[(size_t)t * ldB + c];
+#pragma unroll
+          for (int r = 0; r < kTB; ++r) acc[r] = fma(-Ls[r][t], x, acc[r]);
+        }
+#pragma unroll
+        for (int r = kTB - 1; r >= 0; --r) {
+#pragma unroll
+          for (int t = 0; t < kTB; ++t)
+            if (t > r && t < rn) acc[r] = fma(-Ls[r][r0 + t], acc[t], acc[r]);
+          acc[r] = (r < rn) ? acc[r] / Ls[r][r0 + r] : 0.0;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < kTB; ++r)
+        if (r < rn) b[(size_t)(r0 + r) * ldB + c] = acc[r];
+    }
   }
 }
 
@@ -258,9 +448,15 @@ __global__ void k_project(const double* m, const double* L, const double* P, int
 }
 
 // ------------------------------------------------------------- host helpers
+struct Arena;
+double* arena_take(Arena* ar, size_t n);
+size_t arena_mark(Arena* ar);
+void arena_release(Arena* ar, size_t mark);
+
 struct Ops {
   rocblas_handle h;
   hipStream_t st;
+  Arena* ar;
   int err = 0;
   std::string msg;
 
@@ -281,32 +477,68 @@ struct Ops {
                                       C.p, C.ld, C.s, cnt);
     if (s != rocblas_status_success) fail("rocblas_dgemm_strided_batched", (int)s);
   }
-  // op(L) X = X (X: M x N, L lower M x M), in place
-  void trsm_left(bool tL, int M, int N, V L, V X, int cnt) {
-    if (err || cnt <= 0) return;
-    const double one = 1.0;
-    const rocblas_status s =
-        rocblas_dtrsm_strided_batched(h, rocblas_side_right, rocblas_fill_upper, op(tL), rocblas_diagonal_non_unit,
-                                      N, M, &one, L.p, L.ld, L.s, X.p, X.ld, X.s, cnt);
-    if (s != rocblas_status_success) fail("rocblas_dtrsm_strided_batched", (int)s);
-  }
-  // X op(L) = X (X: M x N, L lower N x N), in place
-  void trsm_right(bool tL, int M, int N, V L, V X, int cnt) {
-    if (err || cnt <= 0) return;
-    const double one = 1.0;
-    const rocblas_status s =
-        rocblas_dtrsm_strided_batched(h, rocblas_side_left, rocblas_fill_upper, op(tL), rocblas_diagonal_non_unit,
-                                      N, M, &one, L.p, L.ld, L.s, X.p, X.ld, X.s, cnt);
-    if (s != rocblas_status_success) fail("rocblas_dtrsm_strided_batched", (int)s);
-  }
+  // tria in place: the blocked LQ (panels of kNB rows in LDS, trailing rows updated
+  // by three GEMMs per panel), or k_lq for matrices of at most one panel
   void lq(V A, int m, int n, int cnt) {
     if (err || cnt <= 0) return;
-    if (n > kLqMaxN) {
-      fail("lq: too many columns", -1);
+    if (n > 256) {
+      fail("lq: more than 256 columns", -1);
       return;
     }
-    hipLaunchKernelGGL(k_lq, dim3(cnt), dim3(kLqThreads), 0, st, A.p, A.s, m, n, A.ld);
-    check("k_lq");
+    const int K = m < n ? m : n;
+    const int kmax = K < n - 1 ? K : n - 1;
+    if (m <= kNB || kmax <= 0) {
+      hipLaunchKernelGGL(k_lq, dim3(cnt), dim3(kLqThreads), 0, st, A.p, A.s, m, n, A.ld);
+      check("k_lq");
+      return;
+    }
+    const size_t mark = arena_mark(ar);
+    V Vb{arena_take(ar, (size_t)cnt * kNB * n), (long long)kNB * n, n};
+    V Tb{arena_take(ar, (size_t)cnt * kNB * kNB), (long long)kNB * kNB, kNB};
+    V Y{arena_take(ar, (size_t)cnt * m * kNB), (long long)m * kNB, kNB};
+    V Y2{arena_take(ar, (size_t)cnt * m * kNB), (long long)m * kNB, kNB};
+    if (!Vb.p || !Tb.p || !Y.p || !Y2.p) {
+      fail("workspace (lq)", -2);
+      return;
+    }
+    for (int i0 = 0; i0 < kmax && !err; i0 += kNB) {
+      const int np = (kmax - i0) < kNB ? (kmax - i0) : kNB;
+      const int nc = n - i0;
+      const int nq = (n + 63) / 64;
+      if (nq <= 1) {
+        hipLaunchKernelGGL(k_lq_panel<1>, dim3(cnt), dim3(64), 0, st, A.p, A.s, n, A.ld, i0, np, Vb.p, Vb.s, Vb.ld,
+                           Tb.p, Tb.s, Tb.ld);
+      } else if (nq == 2) {
+        hipLaunchKernelGGL(k_lq_panel<2>, dim3(cnt), dim3(64), 0, st, A.p, A.s, n, A.ld, i0, np, Vb.p, Vb.s, Vb.ld,
+                           Tb.p, Tb.s, Tb.ld);
+      } else if (nq == 3) {
+        hipLaunchKernelGGL(k_lq_panel<3>, dim3(cnt), dim3(64), 0, st, A.p, A.s, n, A.ld, i0, np, Vb.p, Vb.s, Vb.ld,
+                           Tb.p, Tb.s, Tb.ld);
+      } else {
+        hipLaunchKernelGGL(k_lq_panel<4>, dim3(cnt), dim3(64), 0, st, A.p, A.s, n, A.ld, i0, np, Vb.p, Vb.s, Vb.ld,
+                           Tb.p, Tb.s, Tb.ld);
+      }
+      check("k_lq_panel");
+      const int mr = m - (i0 + np);
+      if (mr <= 0) continue;
+      // A_trail (I - V^T T V): Y = A_trail V^T, Y2 = Y T, A_trail -= Y2 V
+      const V At = A.at(i0 + np, i0);
+      gemm(false, true, mr, np, nc, 1.0, At, Vb, 0.0, Y, cnt);
+      gemm(false, false, mr, np, np, 1.0, Y, Tb, 0.0, Y2, cnt);
+      gemm(false, false, mr, nc, np, -1.0, Y2, Vb, 1.0, At, cnt);
+    }
+    arena_release(ar, mark);
+  }
+  // op(L) X = X in place (L lower n x n, X n x m)
+  void trsm(bool trans, int n, int m, V L, V X, int cnt) {
+    if (err || cnt <= 0) return;
+    if (n > kTrsmMaxN) {
+      fail("trsm: order too large", -1);
+      return;
+    }
+    hipLaunchKernelGGL(k_trsm, dim3((m + 255) / 256, cnt), dim3(256), 0, st, L.p, L.s, L.ld, X.p, X.s, X.ld, n, m,
+                       trans ? 1 : 0);
+    check("k_trsm");
   }
   void copy(V dst, V src, int rows, int cols, int cnt, bool trans = false, bool rev = false, double alpha = 1.0) {
     if (err || cnt <= 0) return;
@@ -340,6 +572,9 @@ struct Arena {
     return p;
   }
 };
+double* arena_take(Arena* ar, size_t n) { return ar->take(n); }
+size_t arena_mark(Arena* ar) { return ar->off; }
+void arena_release(Arena* ar, size_t mark) { ar->off = mark; }
 
 // filter elements (A, b, U, eta, Z) as strided views
 struct FE {
@@ -397,15 +632,15 @@ void filter_fn(Ops& o, Arena& ar, int D, const FE& e1, const FE& e2, const FE& o
   o.fill(X.at(D, D), D, D, cnt, 0.0);
   o.lq(X, 2 * D, 2 * D, cnt);
   const V Xi11 = X.at(0, 0), Xi21 = X.at(D, 0), Xi22 = X.at(D, D);
-  // M = Xi11^-1 (U1^T A2^T); A = A2 A1 - M^T (Xi21^T A1)
-  o.gemm(true, true, D, D, D, 1.0, e1.U, e2.A, 0.0, T1, cnt);
-  o.trsm_left(false, D, D, Xi11, T1, cnt);
+  // m_ = Xi11^-1 U1^T -> T3; M = Xi11^-1 (U1^T A2^T) = m_ A2^T -> T1
+  o.copy(T3, e1.U, D, D, cnt, true);
+  o.trsm(false, D, D, Xi11, T3, cnt);
+  o.gemm(false, true, D, D, D, 1.0, T3, e2.A, 0.0, T1, cnt);
+  // A = A2 A1 - M^T (Xi21^T A1)
   o.gemm(true, false, D, D, D, 1.0, Xi21, e1.A, 0.0, T2, cnt);
   o.gemm(false, false, D, D, D, 1.0, e2.A, e1.A, 0.0, out.A, cnt);
   o.gemm(true, false, D, D, D, -1.0, T1, T2, 1.0, out.A, cnt);
-  // m_ = Xi11^-1 U1^T; b = A2 (v2 - m_^T Xi21^T v2) + b2, v2 = b1 + U1 (U1^T eta2)
-  o.copy(T3, e1.U, D, D, cnt, true);
-  o.trsm_left(false, D, D, Xi11, T3, cnt);
+  // b = A2 (v2 - m_^T Xi21^T v2) + b2, v2 = b1 + U1 (U1^T eta2)
   o.gemm(true, false, D, 1, D, 1.0, e1.U, e2.eta, 0.0, v1, cnt);
   o.copy(v2, e1.b, D, 1, cnt);
   o.gemm(false, false, D, 1, D, 1.0, e1.U, v1, 1.0, v2, cnt);
@@ -418,9 +653,9 @@ void filter_fn(Ops& o, Arena& ar, int D, const FE& e1, const FE& e2, const FE& o
   o.copy(Y.at(0, D), e2.U, D, D, cnt);
   o.lq(Y, D, 2 * D, cnt);
   o.copy(out.U, Y.at(0, 0), D, D, cnt);
-  // _e = Xi11^-T Xi21^T; eta = A1^T (w2 - _e^T U1^T w2) + eta1, w2 = eta2 - Z2 (Z2^T b1)
+  // _e = Xi11^-T Xi21^T -> T4; eta = A1^T (w2 - _e^T U1^T w2) + eta1, w2 = eta2 - Z2 (Z2^T b1)
   o.copy(T4, Xi21, D, D, cnt, true);
-  o.trsm_left(true, D, D, Xi11, T4, cnt);
+  o.trsm(true, D, D, Xi11, T4, cnt);
   o.gemm(true, false, D, 1, D, 1.0, e2.Z, e1.b, 0.0, v1, cnt);
   o.copy(v2, e2.eta, D, 1, cnt);
   o.gemm(false, false, D, 1, D, -1.0, e2.Z, v1, 1.0, v2, cnt);
@@ -527,7 +762,8 @@ void engine_destroy(Engine* e) {
 }
 
 bool supports(int kind, int D, int d) {
-  return kind == kLorenz96 && d >= 4 && D % d == 0 && D / d >= 2 && 2 * D <= kLqMaxN;
+  // the LQ panels hold kNB rows of up to 2 D columns in LDS: D <= 128 keeps them within 32 KiB
+  return kind == kLorenz96 && d >= 4 && D % d == 0 && D / d >= 2 && D <= 128;
 }
 
 namespace {
@@ -546,8 +782,9 @@ size_t workspace_doubles(int D, int d, long long Ns) {
   w += n1 * (D + DD);                       // smoothed natural order
   // per-call temporaries at the widest level (the filter combine), and the post step
   const size_t h = (size_t)(Ns / 2 + 1);
-  w += h * (4 * DD + 2 * DD + 4 * DD + 3 * D) + 4096;
-  w += (size_t)Ns * (4 * DD + (size_t)d * D + 2 * D + 3 * d + 4) + 4096;
+  w += h * (4 * DD + 2 * DD + 5 * DD + 3 * D) + 4096;
+  w += (size_t)(Ns + 1) * (2 * D * kNB + 2 * 2 * D * kNB + kNB * kNB) + 4096;  // blocked-LQ panels
+  w += (size_t)Ns * (5 * DD + (size_t)d * D + 2 * D + 3 * d + 4) + 4096;
   w += 8 * n1 + 64 * 1024;
   return w + w / 10;
 }
@@ -567,8 +804,8 @@ int solve(Engine* e, const Problem& p, const pof_opts& opts, pof_out* out, const
     err = "large-state engine: no device linearisation for this field and (D, d)";
     return POF_E_UNSUPPORTED;
   }
-  if (2 * D > kLqMaxN) {
-    err = "large-state engine: D too large";
+  if (D > 128) {
+    err = "large-state engine: D > 128";
     return POF_E_UNSUPPORTED;
   }
   // workspace
@@ -584,10 +821,10 @@ int solve(Engine* e, const Problem& p, const pof_opts& opts, pof_out* out, const
     e->cap = need;
   }
   Arena ar{e->buf, e->cap, 0};
-  Ops o{e->h, st};
+  Ops o{e->h, st, &ar};
   rocblas_set_stream(e->h, st);
   // constants: F, QL, N1z, P, x0 (one block)
-  double* cst = ar.take(4 * DD + 2 * D);
+  double* cst = ar.take(5 * DD + 2 * D);
   double* mA = ar.take((size_t)(Ns + 1) * D);
   double* mB = ar.take((size_t)(Ns + 1) * D);
   double* H = ar.take((size_t)Ns * d * D);
@@ -605,7 +842,8 @@ int solve(Engine* e, const Problem& p, const pof_opts& opts, pof_out* out, const
   double* dQL = cst + DD;
   double* dN1z = cst + 2 * DD;
   double* dL0 = cst + 3 * DD;
-  double* dP = cst + 4 * DD;
+  double* dQLi = cst + 4 * DD;
+  double* dP = cst + 5 * DD;
   double* dm0 = dP + D;
   double* t_ssq = terms;
   double* t_nll = t_ssq + (Ns + 1);
@@ -622,13 +860,22 @@ int solve(Engine* e, const Problem& p, const pof_opts& opts, pof_out* out, const
   up(dN1z, p.N1z, DD);
   up(dL0, p.x0L, DD);
   up(dP, p.P, D);
+  // QL^-1 by forward substitution (the objective's solve with the constant QL)
+  std::vector<double> qli((size_t)DD, 0.0);
+  for (int j = 0; j < D; ++j)
+    for (int i = j; i < D; ++i) {
+      double v = (i == j) ? 1.0 : 0.0;
+      for (int t = j; t < i; ++t) v -= p.QL[(size_t)i * D + t] * qli[(size_t)t * D + j];
+      qli[(size_t)i * D + j] = v / p.QL[(size_t)i * D + i];
+    }
+  up(dQLi, qli.data(), DD);
   up(dm0, p.x0m, D);
   up(mA, p.m0, (size_t)(Ns + 1) * D);
   if (he != hipSuccess) {
     err = std::string("large-state engine upload: ") + hipGetErrorString(he);
     return POF_E_HIP;
   }
-  const V Fv{dF, 0, D}, QLv{dQL, 0, D}, N1zv{dN1z, 0, D};
+  const V Fv{dF, 0, D}, QLv{dQL, 0, D}, N1zv{dN1z, 0, D}, QLiv{dQLi, 0, D};
   const size_t base_mark = ar.off;
 
   double* mcur = mA;
@@ -662,14 +909,14 @@ int solve(Engine* e, const Problem& p, const pof_opts& opts, pof_out* out, const
       const size_t mark = ar.off;
       const int n = (int)Ns;
       V Pb{ar.take((size_t)n * (d + D) * D), (long long)(d + D) * D, D};
-      V Kb{ar.take((size_t)n * D * d), (long long)D * d, d};
       V HF{ar.take((size_t)n * d * D), (long long)d * D, D};
       V Zt{ar.take((size_t)n * d * D), (long long)d * D, D};
       V X2{ar.take((size_t)n * d * D), (long long)d * D, D};
+      V KT{ar.take((size_t)n * d * D), (long long)d * D, D};
       V t0{ar.take((size_t)d + 8), 0, 1};
       V m1{ar.take((size_t)D + 8), 0, 1};
       V N10{ar.take((size_t)2 * DD), 2 * DD, 2 * D};
-      if (!Pb.p || !Kb.p || !HF.p || !Zt.p || !X2.p || !t0.p || !m1.p || !N10.p) o.fail("workspace", -2);
+      if (!Pb.p || !HF.p || !Zt.p || !X2.p || !KT.p || !t0.p || !m1.p || !N10.p) o.fail("workspace", -2);
       // element 0: m1 = F m0, N1 = tria([F L0 | QL]); elements k >= 1: m1 = 0, N1 = tria([0 | QL])
       o.gemm(false, false, D, 1, D, 1.0, Fv, V{dm0, 0, 1}, 0.0, m1, 1);
       o.gemm(false, false, D, D, D, 1.0, Fv, V{dL0, 0, D}, 0.0, N10.at(0, 0), 1);
@@ -682,7 +929,8 @@ int solve(Engine* e, const Problem& p, const pof_opts& opts, pof_out* out, const
         const V N1 = part == 0 ? V{N10.p, 0, 2 * D} : N1zv;
         const V Hk = Hv.item(k0), ck = cv.item(k0);
         const FE ok = leaf.item(k0);
-        const V P0 = Pb.item(k0), K0 = Kb.item(k0), HF0 = HF.item(k0), Z0 = Zt.item(k0), X0 = X2.item(k0);
+        const V P0 = Pb.item(k0), HF0 = HF.item(k0), Z0 = Zt.item(k0), X0 = X2.item(k0);
+        const V KT0 = KT.item(k0);
         // Psi = tria([[H N1], [N1]]) (the zero column block of filter.py:63-66 changes no pivot)
         o.gemm(false, false, d, D, D, 1.0, Hk, N1, 0.0, P0.at(0, 0), cnt);
         o.copy(P0.at(d, 0), N1, D, D, cnt);
@@ -690,29 +938,29 @@ int solve(Engine* e, const Problem& p, const pof_opts& opts, pof_out* out, const
         // U = Psi22 = [Psi[d:, d:D] | 0]
         o.copy(ok.U, P0.at(d, d), D, D - d, cnt);
         o.fill(ok.U.at(0, D - d), D, d, cnt, 0.0);
-        // K = Psi21 Psi11^-1
-        o.copy(K0, P0.at(d, 0), D, d, cnt);
-        o.trsm_right(false, D, d, P0.at(0, 0), K0, cnt);
+        // K^T = Psi11^-T Psi21^T (K = Psi21 Psi11^-1)
+        o.copy(KT0, P0.at(d, 0), d, D, cnt, true);
+        o.trsm(true, d, D, P0.at(0, 0), KT0, cnt);
         // A = F - K (H F)
         o.gemm(false, false, d, D, D, 1.0, Hk, Fv, 0.0, HF0, cnt);
         o.copy(ok.A, Fv, D, D, cnt);
-        o.gemm(false, false, D, D, d, -1.0, K0, HF0, 1.0, ok.A, cnt);
+        o.gemm(true, false, D, D, d, -1.0, KT0, HF0, 1.0, ok.A, cnt);
         // b = m1 - K (H m1 + c)
         if (part == 0) {
           o.copy(t0, ck, d, 1, 1);
           o.gemm(false, false, d, 1, D, 1.0, Hk, m1, 1.0, t0, 1);
           o.copy(ok.b, m1, D, 1, 1);
-          o.gemm(false, false, D, 1, d, -1.0, K0, t0, 1.0, ok.b, 1);
+          o.gemm(true, false, D, 1, d, -1.0, KT0, t0, 1.0, ok.b, 1);
         } else {
-          o.gemm(false, false, D, 1, d, -1.0, K0, ck, 0.0, ok.b, cnt);
+          o.gemm(true, false, D, 1, d, -1.0, KT0, ck, 0.0, ok.b, cnt);
         }
         // Z = [(Psi11^-1 H F)^T | 0]; eta = -(Psi11^-T Psi11^-1 H F)^T c
         o.copy(Z0, HF0, d, D, cnt);
-        o.trsm_left(false, d, D, P0.at(0, 0), Z0, cnt);
+        o.trsm(false, d, D, P0.at(0, 0), Z0, cnt);
         o.copy(ok.Z, Z0, D, d, cnt, true);
         o.fill(ok.Z.at(0, d), D, D - d, cnt, 0.0);
         o.copy(X0, Z0, d, D, cnt);
-        o.trsm_left(true, d, D, P0.at(0, 0), X0, cnt);
+        o.trsm(true, d, D, P0.at(0, 0), X0, cnt);
         o.gemm(true, false, D, 1, d, -1.0, X0, ck, 0.0, ok.eta, cnt);
       }
       ar.off = mark;
@@ -740,7 +988,8 @@ int solve(Engine* e, const Problem& p, const pof_opts& opts, pof_out* out, const
       V Fm{ar.take((size_t)n * D), D, 1};
       V Sb{ar.take((size_t)n * d * D), (long long)d * D, D};
       V yv{ar.take((size_t)n * d), d, 1};
-      if (!Phi.p || !Fm.p || !Sb.p || !yv.p) o.fail("workspace", -2);
+      V Phin{ar.take((size_t)n * DD), DD, D};
+      if (!Phi.p || !Fm.p || !Sb.p || !yv.p || !Phin.p) o.fail("workspace", -2);
       const V fmv{fm, D, 1}, fLv{fL, DD, D};
       o.gemm(false, false, D, D, D, 1.0, Fv, fLv, 0.0, Phi.at(0, 0), n);
       o.copy(Phi.at(0, D), QLv, D, D, n);
@@ -748,8 +997,10 @@ int solve(Engine* e, const Problem& p, const pof_opts& opts, pof_out* out, const
       o.fill(Phi.at(D, D), D, D, n, 0.0);
       o.lq(Phi, 2 * D, 2 * D, n);
       // E = Phi21 Phi11^-1, D = Phi22, g = m - E F m
-      o.copy(snat.E, Phi.at(D, 0), D, D, n);
-      o.trsm_right(false, D, D, Phi.at(0, 0), snat.E, n);
+      // E^T = Phi11^-T Phi21^T
+      o.copy(Phin, Phi.at(D, 0), D, D, n, true);
+      o.trsm(true, D, D, Phi.at(0, 0), Phin, n);
+      o.copy(snat.E, Phin, D, D, n, true);
       o.copy(snat.Dm, Phi.at(D, D), D, D, n);
       o.gemm(false, false, D, 1, D, 1.0, Fv, fmv, 0.0, Fm, n);
       o.copy(snat.g, fmv, D, 1, n);
@@ -790,13 +1041,13 @@ int solve(Engine* e, const Problem& p, const pof_opts& opts, pof_out* out, const
     // ---- objective (utils.py:97-101) and the convergence partials
     {
       const size_t mark = ar.off;
-      V r{ar.take((size_t)Ns * D), D, 1};
-      if (!r.p) o.fail("workspace", -2);
+      V r{ar.take((size_t)Ns * D), D, 1}, r2{ar.take((size_t)Ns * D), D, 1};
+      if (!r.p || !r2.p) o.fail("workspace", -2);
       o.copy(r, V{sm, D, 1}, D, 1, (int)Ns);
       o.gemm(false, false, D, 1, D, -1.0, Fv, V{sm + D, D, 1}, 1.0, r, (int)Ns);
-      o.trsm_left(false, D, 1, QLv, r, (int)Ns);
+      o.gemm(false, false, D, 1, D, 1.0, QLiv, r, 0.0, r2, (int)Ns);
       if (!o.err) {
-        hipLaunchKernelGGL(k_sqnorm, dim3((unsigned)((Ns + 127) / 128)), dim3(128), 0, st, r.p, D, (int)Ns, t_obj);
+        hipLaunchKernelGGL(k_sqnorm, dim3((unsigned)((Ns + 127) / 128)), dim3(128), 0, st, r2.p, D, (int)Ns, t_obj);
         hipLaunchKernelGGL(k_viol, dim3((unsigned)((Ns + 128) / 128)), dim3(128), 0, st, sm, mcur, D, (int)(Ns + 1),
                            t_viol);
         hipLaunchKernelGGL(k_reduce4, dim3(1), dim3(256), 0, st, t_ssq, t_nll, t_obj, (int)Ns, t_viol, (int)(Ns + 1),

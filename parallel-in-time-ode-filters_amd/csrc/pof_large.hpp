@@ -2,7 +2,7 @@
 // dimensions whose elements do not fit one worker's registers (Lorenz-96 with
 // d = 32, q = 3: D = 128).  Every scan level runs as batched primitives over all
 // its combines: a hand-written Householder LQ kernel (one workgroup per matrix),
-// rocBLAS strided-batched GEMM / TRSM for the plain products and solves, and
+// rocBLAS strided-batched GEMM for the plain products, a triangular-solve kernel, and
 // small element-wise kernels.  The host drives JAX's odd/even recursion.
 #pragma once
 
