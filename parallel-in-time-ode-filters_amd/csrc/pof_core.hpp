@@ -1035,15 +1035,21 @@ __device__ void filter_combine(const Ctx<D, SS>& cx, const FiltArrays& x1, const
   // the rows of A2, U2 (and Z1) are needed only after this factorisation: their
   // loads are issued half way through it, where fewer registers are live and
   // the remaining pivot steps hide the memory latency
-  lq<S, 2 * D, D, D, 1, 0, D, false>(xr, r, s, cx.bc());
-  load_dist<S, D>(a2, x2.A + (i2 * D + rr) * D, s);
-  load_dist<S, D>(u2, x2.U + (i2 * D + rr) * D, s);
-  if constexpr (!state_only) {
-    load_dist<S, D>(z1, x1.Z + (i1 * D + rz) * D, s);  // Z1 rows feed the rows D..2D-1
-  } else {
+  if constexpr (state_only) {
+    // Xi11 and Xi21 are final after the first D pivots; the last D pivots only
+    // form Xi22, which enters Z alone
+    lq<S, 2 * D, D, D, 1, 0, D / 2, false>(xr, r, s, cx.bc());
+    load_dist<S, D>(a2, x2.A + (i2 * D + rr) * D, s);
+    load_dist<S, D>(u2, x2.U + (i2 * D + rr) * D, s);
     zero(z1);
+    lq<S, 2 * D, D, D, 1, D / 2, D>(xr, r, s, cx.bc());
+  } else {
+    lq<S, 2 * D, D, D, 1, 0, D, false>(xr, r, s, cx.bc());
+    load_dist<S, D>(a2, x2.A + (i2 * D + rr) * D, s);
+    load_dist<S, D>(u2, x2.U + (i2 * D + rr) * D, s);
+    load_dist<S, D>(z1, x1.Z + (i1 * D + rz) * D, s);  // Z1 rows feed the rows D..2D-1
+    lq<S, 2 * D, D, D, 1, D>(xr, r, s, cx.bc());
   }
-  lq<S, 2 * D, D, D, 1, D>(xr, r, s, cx.bc());
   if (act) {
 #pragma unroll
     for (int t = 0; t < NL; ++t) TX11[r * LD + t * S + s] = xr[t];

@@ -179,8 +179,12 @@ __global__ __launch_bounds__((Geo<D, kLdsState>::BLOCK)) void k_filter_down(Grou
 // s < Ns the predicted observation statistics of filter.py:84-114:
 //   S = tria([H_s tria([F L_s | LQ]) | 0]), yhat = H_s F m_s + c_s,
 //   ssq_s = |S^-T yhat|^2 (utils.py:110-112), nll_s = -loglik(yhat, S) (utils.py:22-30).
+// The smoother factor D = Phi22 comes from the last D pivots of the Phi
+// factorisation and is formed only when the IVP's smoothed factors are wanted
+// this iteration (wchol); dm_only recomputes just D, for the factor pass of an
+// IVP whose last iteration carried means only.
 template <int D, int d, bool EK1>
-__global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK)) void k_filter_post(GroupArgs g, LaunchMap lm) {
+__global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK)) void k_filter_post(GroupArgs g, LaunchMap lm, int dm_only) {
   extern __shared__ double smem[];
   auto cx = make_ctx<D, kLdsPost>(g.F, g.LQ, g.N1z, smem);
   using L = Lay<D, decltype(cx)::S>;
@@ -201,6 +205,7 @@ __global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK)) void k_filter_post(Group
   const double* mp = (s == 0) ? a.x0m : pre0.m + (size_t)(s - 1) * D;
   const double* Lp = (s == 0) ? a.x0L : pre0.L + (size_t)(s - 1) * D * D;
   const size_t ro = (size_t)(Ns - s);
+  const bool wc = dm_only || g.wchol[j] != 0;
   if (s == Ns) {
     if (act) {
       double lr[NL], zr[NL];
@@ -221,7 +226,7 @@ __global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK)) void k_filter_post(Group
   double* Vfm = cx.vec(1);
   // global reads first: the filtered state and the linearisation point of obs model s
   Obs<D, d, EK1> ob;
-  ob.load(a, s);
+  if (!dm_only) ob.load(a, s);
   {
     double lr[NL];
     load_dist<S, D>(lr, Lp + rr * D, sb);
@@ -254,7 +259,21 @@ __global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK)) void k_filter_post(Group
       ph[NL + t] = act ? lq_r[t] : 0.0;
     }
   }
-  lq<S, 2 * D, D, D>(ph, r, sb, cx.bc());
+  lq<S, 2 * D, D, D, 1, 0, D, false>(ph, r, sb, cx.bc());
+  if (wc) {
+    lq<S, 2 * D, D, D, 1, D>(ph, r, sb, cx.bc());  // Phi22
+  } else {
+    lq<S, 2 * D, D, D, 1, D, D>(ph, r, sb, cx.bc());  // no pivot: clear the strict upper part
+  }
+  if (dm_only) {
+    if (r >= D && r < 2 * D) {
+      double dd[NL];
+#pragma unroll
+      for (int t = 0; t < NL; ++t) dd[t] = ph[NL + t];
+      store_dist<S, D>(sel.Dm + (ro * D + (r - D)) * D, dd, sb);
+    }
+    return;
+  }
   if (act) {
 #pragma unroll
     for (int t = 0; t < NL; ++t) TP11[r * LD + t * S + sb] = ph[t];
@@ -321,7 +340,7 @@ __global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK)) void k_filter_post(Group
     for (int k = 0; k < D; ++k) gg = fma(-TE[r * LD + k], Vfm[k], gg);
     if (sb == 0) sel.g[ro * D + r] = gg;
     store_dist<S, D>(sel.E + (ro * D + r) * D, e, sb);
-    store_dist<S, D>(sel.Dm + (ro * D + r) * D, dd, sb);
+    if (wc) store_dist<S, D>(sel.Dm + (ro * D + r) * D, dd, sb);
   }
   wsync();
 }
@@ -873,6 +892,13 @@ void smooth_scan(const GroupArgs& g, const LaunchPlan& p, hipStream_t st, Rec& r
 template <int D, int d>
 hipError_t run_smooth_chol(const GroupArgs& g, const LaunchPlan& p, hipStream_t st, KernelTimer* t) {
   Rec rec(t, st);
+  // the smoother elements' D of the last (means-only) iteration, then the factor scan
+  if (g.affine) {
+    POF_LAUNCH((k_filter_post<D, d, false>), kLdsPost, p.pts_total, st, g, p.map(p.pts_off, p.pts_total), 1);
+  } else {
+    POF_LAUNCH((k_filter_post<D, d, true>), kLdsPost, p.pts_total, st, g, p.map(p.pts_off, p.pts_total), 1);
+  }
+  rec(kKFilterPost, p.leaf_total);
   smooth_scan<D>(g, p, st, rec);
   return hipGetLastError();
 }
@@ -907,9 +933,9 @@ hipError_t run_iteration(const GroupArgs& g, const LaunchPlan& p, double* scal, 
   }
   // ---- smoother elements + observation statistics
   if (g.affine) {
-    POF_LAUNCH((k_filter_post<D, d, false>), kLdsPost, p.pts_total, st, g, p.map(p.pts_off, p.pts_total));
+    POF_LAUNCH((k_filter_post<D, d, false>), kLdsPost, p.pts_total, st, g, p.map(p.pts_off, p.pts_total), 0);
   } else {
-    POF_LAUNCH((k_filter_post<D, d, true>), kLdsPost, p.pts_total, st, g, p.map(p.pts_off, p.pts_total));
+    POF_LAUNCH((k_filter_post<D, d, true>), kLdsPost, p.pts_total, st, g, p.map(p.pts_off, p.pts_total), 0);
   }
   rec(kKFilterPost, p.leaf_total);
   // ---- smoother scan (reversed order)
