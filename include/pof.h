@@ -206,6 +206,24 @@ int pof_linear_filtsmooth(pof_ctx* ctx, int D, int d, int64_t Ns, const double* 
                           double* smooth_chol_dev, double* filt_mean_dev, double* filt_chol_dev, double* nll,
                           double* obj, double* ssq, void* stream);
 
+/* RTS smoother alone over given filtered states [Ns+1, D] / [Ns+1, D, D] (device),
+ * constant transition (F, QL) on the host.  Replaces
+ * pof.parallel_filtsmooth.smoothing (smoother.py:8-22; sequential = 0) and
+ * pof.sequential_filtsmooth.smoothing (sequential_filtsmooth/smoother.py:8-28;
+ * sequential = 1).  Outputs the smoothed states and obj (utils.py:97-101). */
+int pof_smoothing(pof_ctx* ctx, int D, int64_t Ns, const double* F_host, const double* QL_host,
+                  const double* filt_mean_dev, const double* filt_chol_dev, int sequential, double* smooth_mean_dev,
+                  double* smooth_chol_dev, double* obj, void* stream);
+
+/* Square-root EKF of the problem's residual model, linearised at each predicted
+ * mean, with the caller's prior x0 and constant transition (F, QL) (host), for
+ * n_times - 1 steps.  Replaces pof.sequential_filtsmooth.extended_kalman_filter
+ * (sequential_filtsmooth/filter.py:9-30): filtered states [Ns+1, ...] on the
+ * device (may be NULL), nll = summed log-likelihood, ssq. */
+int pof_ekf(pof_ctx* ctx, const pof_problem* prob, const double* x0_mean_host, const double* x0_chol_host,
+            const double* F_host, const double* QL_host, double* filt_mean_dev, double* filt_chol_dev, double* nll,
+            double* ssq, void* stream);
+
 /* ------------------------------------------------------------------ batches
  * Many IVPs of one state dimension D = d (q + 1) solved together: every scan
  * level of every IVP in the group runs in one launch, each IVP keeps its own

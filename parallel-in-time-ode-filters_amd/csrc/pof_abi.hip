@@ -379,7 +379,9 @@ struct pof_batch {
   int D = 0, d = 0, q = 0;
   bool affine = false;
   bool cached = false;
-  int mode = -1;  // 0 parallel IEKS, 1 sequential IEKS, 2 sequential EKS single pass (set by the first admission)
+  // 0 parallel IEKS, 1 sequential IEKS, 2 sequential EKS single pass, 3 / 4 parallel /
+  // sequential smoother alone over given filtered states (set by the first admission)
+  int mode = -1;
   const pof::EngineFns* e = nullptr;
   long long entries = 0;
   int slots = 0;
@@ -872,7 +874,9 @@ int batch_launch(pof_batch* b) {
     HIP_TRY(hipMemcpyAsync(b->wchol, b->hwchol.data(), (size_t)b->slots * sizeof(int), hipMemcpyHostToDevice, b->st));
   }
   KernelTimer* tm = ctx->profiling ? &ctx->timer : nullptr;
-  if (b->mode >= 1) {
+  if (b->mode >= 3) {
+    HIP_TRY(b->e->smooth_only(b->g, b->plan, b->scal_dev, b->mode == 4 ? 1 : 0, b->st, tm));
+  } else if (b->mode >= 1) {
     HIP_TRY(b->e->seq_iteration(b->g, b->plan, b->scal_dev, b->mode == 2 ? 1 : 0, b->st, tm));
   } else {
     HIP_TRY(b->e->iteration(b->g, b->plan, b->scal_dev, b->st, tm));
@@ -978,7 +982,7 @@ int batch_finish(pof_batch* b, int max_done, long long* done, int* n_done) {
   HIP_TRY(hipMemcpyAsync(b->scale, b->hscale.data(), S * sizeof(double), hipMemcpyHostToDevice, b->st));
   KernelTimer* tm = ctx->profiling ? &ctx->timer : nullptr;
   if (!need.empty()) {
-    if (b->mode >= 1) {
+    if (b->mode == 1 || b->mode == 2 || b->mode == 4) {
       HIP_TRY(b->e->seq_smooth_chol(b->g, pc, b->st, tm));
     } else {
       HIP_TRY(b->e->smooth_chol(b->g, pc, b->st, tm));
@@ -1439,6 +1443,106 @@ int pof_ieks_step(pof_ctx* ctx, const pof_problem* prob, const double* states_me
   if (rc != POF_OK) return rc;
   if (nll) *nll = po.nll;
   if (obj) *obj = po.obj;
+  if (ssq) *ssq = po.sigma_squared;
+  return POF_OK;
+}
+
+int pof_smoothing(pof_ctx* ctx, int D, int64_t Ns, const double* F_host, const double* QL_host,
+                  const double* filt_mean_dev, const double* filt_chol_dev, int sequential, double* smooth_mean_dev,
+                  double* smooth_chol_dev, double* obj, void* stream) {
+  if (!ctx) return POF_E_ARG;
+  if (D < 1 || Ns < 1 || Ns > 0x3fffffff || !F_host || !QL_host || !filt_mean_dev || !filt_chol_dev)
+    return set_err(ctx, POF_E_ARG, "invalid smoothing arguments");
+  // the smoother does not depend on the observation dimension: any kernel table entry of this D
+  const pof::EngineFns* e = nullptr;
+  int d = 0;
+  for (int dd = 1; dd <= 4 && !e; ++dd)
+    if (D % dd == 0 && (e = pof::find_engine(D, dd)) != nullptr) d = dd;
+  if (!e) return set_err(ctx, POF_E_UNSUPPORTED, "no kernels for state dimension %d", D);
+  HIP_TRY(hipSetDevice(ctx->device));
+  pof_batch* b = nullptr;
+  int rc = batch_new(ctx, D, d, entries_for(Ns + 1), 1, true, &b);
+  if (rc != POF_OK) return rc;
+  struct Guard {
+    pof_batch* b;
+    ~Guard() { batch_free(b); }
+  } guard{b};
+  if (stream) b->st = (hipStream_t)stream;
+  const size_t DD = (size_t)D * D;
+  std::vector<double> M(DD * 2, 0.0), N1z;
+  for (int r = 0; r < D; ++r)
+    for (int c = 0; c < D; ++c) M[(size_t)r * 2 * D + D + c] = QL_host[(size_t)r * D + c];
+  pof::tria_host(M, D, 2 * D, N1z);
+  HIP_TRY(hipMemcpy(b->F, F_host, DD * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(b->LQ, QL_host, DD * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(b->N1z, N1z.data(), DD * 8, hipMemcpyHostToDevice));
+  // state 0 enters as the prior of the slot, states 1..Ns as the level-0 filter prefixes
+  std::vector<double> m0(D), L0(DD);
+  HIP_TRY(hipMemcpy(m0.data(), filt_mean_dev, D * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(L0.data(), filt_chol_dev, DD * 8, hipMemcpyDeviceToHost));
+  std::vector<double> P(D, 1.0);
+  pof_opts o{};
+  o.maxiters = 1;
+  o.fixed_iters = 1;
+  pof_out po{};
+  po.full_means_dev = smooth_mean_dev;
+  po.full_chols_dev = smooth_chol_dev;
+  rc = admit_raw(b, (int)Ns, P.data(), m0.data(), L0.data(), 0, nullptr, nullptr, nullptr, &o, &po, 1,
+                 sequential ? 4 : 3, false, nullptr);
+  if (rc != POF_OK) return rc;
+  const long long base = b->sl[0].start;
+  HIP_TRY(hipMemcpyAsync(b->g.fpre.m + (size_t)base * D, filt_mean_dev + D, (size_t)Ns * D * 8,
+                         hipMemcpyDeviceToDevice, b->st));
+  HIP_TRY(hipMemcpyAsync(b->g.fpre.L + (size_t)base * DD, filt_chol_dev + DD, (size_t)Ns * DD * 8,
+                         hipMemcpyDeviceToDevice, b->st));
+  rc = run_to_completion(b);
+  if (rc != POF_OK) return rc;
+  if (obj) *obj = po.obj;
+  return POF_OK;
+}
+
+int pof_ekf(pof_ctx* ctx, const pof_problem* prob, const double* x0_mean_host, const double* x0_chol_host,
+            const double* F_host, const double* QL_host, double* filt_mean_dev, double* filt_chol_dev, double* nll,
+            double* ssq, void* stream) {
+  if (!ctx || !prob || !x0_mean_host || !x0_chol_host || !F_host || !QL_host)
+    return set_err(ctx, POF_E_ARG, "null argument");
+  pof::Setup s;
+  int rc = pof::build_setup(prob, s, ctx->err);
+  if (rc != POF_OK) return rc;
+  if (!engine_or_err(ctx, s.D, s.d)) return POF_E_UNSUPPORTED;
+  HIP_TRY(hipSetDevice(ctx->device));
+  pof_batch* b = nullptr;
+  rc = batch_new(ctx, s.D, s.d, entries_for(prob->n_times), 1, false, &b);
+  if (rc != POF_OK) return rc;
+  struct Guard {
+    pof_batch* b;
+    ~Guard() { batch_free(b); }
+  } guard{b};
+  if (stream) b->st = (hipStream_t)stream;
+  const int D = s.D;
+  const size_t DD = (size_t)D * D;
+  std::vector<double> M(DD * 2, 0.0), N1z;
+  for (int r = 0; r < D; ++r)
+    for (int c = 0; c < D; ++c) M[(size_t)r * 2 * D + D + c] = QL_host[(size_t)r * D + c];
+  pof::tria_host(M, D, 2 * D, N1z);
+  HIP_TRY(hipMemcpy(b->F, F_host, DD * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(b->LQ, QL_host, DD * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(b->N1z, N1z.data(), DD * 8, hipMemcpyHostToDevice));
+  pof_opts o{};
+  o.maxiters = 1;
+  o.fixed_iters = 1;
+  o.sequential = 1;
+  pof_out po{};
+  rc = admit_raw(b, s.Ns, s.P.data(), x0_mean_host, x0_chol_host, prob->kind, prob->params, nullptr, nullptr, &o,
+                 &po, 1, 2, false, nullptr);
+  if (rc != POF_OK) return rc;
+  rc = batch_launch(b);
+  if (rc != POF_OK) return rc;
+  HIP_TRY(hipEventSynchronize(b->ev));
+  if (filt_mean_dev || filt_chol_dev) HIP_TRY(b->e->filtered(b->g, b->plan, filt_mean_dev, filt_chol_dev, b->st));
+  rc = batch_finish(b, 0, nullptr, nullptr);
+  if (rc != POF_OK) return rc;
+  if (nll) *nll = po.nll;
   if (ssq) *ssq = po.sigma_squared;
   return POF_OK;
 }

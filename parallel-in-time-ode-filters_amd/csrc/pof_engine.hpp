@@ -71,6 +71,9 @@ struct EngineFns {
   hipError_t (*seq_iteration)(const GroupArgs& g, const LaunchPlan& p, double* scal, int at_pred, hipStream_t s,
                               KernelTimer* t);
   hipError_t (*seq_smooth_chol)(const GroupArgs& g, const LaunchPlan& p, hipStream_t s, KernelTimer* t);
+  // the smoother alone over filtered states in the filter prefix arrays (parallel or sequential)
+  hipError_t (*smooth_only)(const GroupArgs& g, const LaunchPlan& p, double* scal, int sequential, hipStream_t s,
+                            KernelTimer* t);
   // EK1 linearisation at n points: H [n, d, D], c [n, d]
   hipError_t (*linearize)(const ModelArgs& a, const double* m, int n, double* H, double* c, hipStream_t s);
   // linearisation of the given LinMethod at n states (m [n, D], L [n, D, D]):

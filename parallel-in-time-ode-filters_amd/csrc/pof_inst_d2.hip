@@ -7,23 +7,28 @@ const EngineFns* engines_d2(int D) {
   static const EngineFns table[] = {
       EngineFns{4, 2, &run_iteration<4, 2>, &run_smooth_chol<4, 2>,
                 &run_output<4, 2>, &run_filtered<4, 2>,
-                &run_seq_iteration<4, 2>, &run_seq_smooth_chol<4, 2>, &run_linearize<4, 2>,
+                &run_seq_iteration<4, 2>, &run_seq_smooth_chol<4, 2>, &run_smooth_only<4, 2>,
+                &run_linearize<4, 2>,
                 &run_linearize_alt<4, 2>},
       EngineFns{6, 2, &run_iteration<6, 2>, &run_smooth_chol<6, 2>,
                 &run_output<6, 2>, &run_filtered<6, 2>,
-                &run_seq_iteration<6, 2>, &run_seq_smooth_chol<6, 2>, &run_linearize<6, 2>,
+                &run_seq_iteration<6, 2>, &run_seq_smooth_chol<6, 2>, &run_smooth_only<6, 2>,
+                &run_linearize<6, 2>,
                 &run_linearize_alt<6, 2>},
       EngineFns{8, 2, &run_iteration<8, 2>, &run_smooth_chol<8, 2>,
                 &run_output<8, 2>, &run_filtered<8, 2>,
-                &run_seq_iteration<8, 2>, &run_seq_smooth_chol<8, 2>, &run_linearize<8, 2>,
+                &run_seq_iteration<8, 2>, &run_seq_smooth_chol<8, 2>, &run_smooth_only<8, 2>,
+                &run_linearize<8, 2>,
                 &run_linearize_alt<8, 2>},
       EngineFns{10, 2, &run_iteration<10, 2>, &run_smooth_chol<10, 2>,
                 &run_output<10, 2>, &run_filtered<10, 2>,
-                &run_seq_iteration<10, 2>, &run_seq_smooth_chol<10, 2>, &run_linearize<10, 2>,
+                &run_seq_iteration<10, 2>, &run_seq_smooth_chol<10, 2>, &run_smooth_only<10, 2>,
+                &run_linearize<10, 2>,
                 &run_linearize_alt<10, 2>},
       EngineFns{12, 2, &run_iteration<12, 2>, &run_smooth_chol<12, 2>,
                 &run_output<12, 2>, &run_filtered<12, 2>,
-                &run_seq_iteration<12, 2>, &run_seq_smooth_chol<12, 2>, &run_linearize<12, 2>,
+                &run_seq_iteration<12, 2>, &run_seq_smooth_chol<12, 2>, &run_smooth_only<12, 2>,
+                &run_linearize<12, 2>,
                 &run_linearize_alt<12, 2>},
   };
   for (const auto& e : table)

@@ -181,10 +181,12 @@ __global__ __launch_bounds__((Geo<D, kLdsState>::BLOCK)) void k_filter_down(Grou
 //   ssq_s = |S^-T yhat|^2 (utils.py:110-112), nll_s = -loglik(yhat, S) (utils.py:22-30).
 // The smoother factor D = Phi22 comes from the last D pivots of the Phi
 // factorisation and is formed only when the IVP's smoothed factors are wanted
-// this iteration (wchol); dm_only recomputes just D, for the factor pass of an
-// IVP whose last iteration carried means only.
+// this iteration (wchol).  mode kPostDm recomputes just D, for the factor pass
+// of an IVP whose last iteration carried means only; kPostNoObs forms the
+// smoother elements without observation statistics (smoother on its own).
+enum PostMode : int { kPostFull = 0, kPostDm = 1, kPostNoObs = 2 };
 template <int D, int d, bool EK1>
-__global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK)) void k_filter_post(GroupArgs g, LaunchMap lm, int dm_only) {
+__global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK)) void k_filter_post(GroupArgs g, LaunchMap lm, int mode) {
   extern __shared__ double smem[];
   auto cx = make_ctx<D, kLdsPost>(g.F, g.LQ, g.N1z, smem);
   using L = Lay<D, decltype(cx)::S>;
@@ -205,6 +207,7 @@ __global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK)) void k_filter_post(Group
   const double* mp = (s == 0) ? a.x0m : pre0.m + (size_t)(s - 1) * D;
   const double* Lp = (s == 0) ? a.x0L : pre0.L + (size_t)(s - 1) * D * D;
   const size_t ro = (size_t)(Ns - s);
+  const bool dm_only = mode == kPostDm;
   const bool wc = dm_only || g.wchol[j] != 0;
   if (s == Ns) {
     if (act) {
@@ -226,7 +229,7 @@ __global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK)) void k_filter_post(Group
   double* Vfm = cx.vec(1);
   // global reads first: the filtered state and the linearisation point of obs model s
   Obs<D, d, EK1> ob;
-  if (!dm_only) ob.load(a, s);
+  if (mode == kPostFull) ob.load(a, s);
   {
     double lr[NL];
     load_dist<S, D>(lr, Lp + rr * D, sb);
@@ -296,6 +299,12 @@ __global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK)) void k_filter_post(Group
   wsync();
   // observation statistics with obs model s (the smoother element stays in
   // LDS until the end so that no store is outstanding during the LQ below)
+  if (mode == kPostNoObs) {
+    if (r == 0 && sb == 0) {
+      t_ssq[s] = 0.0;
+      t_nll[s] = 0.0;
+    }
+  } else {
   double sr[NL + NLd];
   {
     double t1[NL];
@@ -330,6 +339,7 @@ __global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK)) void k_filter_post(Group
     t_ssq[s] = ss;
     // -loglik = 0.5 |S^-1 y|^2 + sum log|S_ii| + d log(2 pi) / 2
     t_nll[s] = 0.5 * nn + (ld + d * log(2.0 * M_PI) / 2.0);
+  }
   }
   if (act) {
     double e[NL], dd[NL];
@@ -894,9 +904,9 @@ hipError_t run_smooth_chol(const GroupArgs& g, const LaunchPlan& p, hipStream_t 
   Rec rec(t, st);
   // the smoother elements' D of the last (means-only) iteration, then the factor scan
   if (g.affine) {
-    POF_LAUNCH((k_filter_post<D, d, false>), kLdsPost, p.pts_total, st, g, p.map(p.pts_off, p.pts_total), 1);
+    POF_LAUNCH((k_filter_post<D, d, false>), kLdsPost, p.pts_total, st, g, p.map(p.pts_off, p.pts_total), kPostDm);
   } else {
-    POF_LAUNCH((k_filter_post<D, d, true>), kLdsPost, p.pts_total, st, g, p.map(p.pts_off, p.pts_total), 1);
+    POF_LAUNCH((k_filter_post<D, d, true>), kLdsPost, p.pts_total, st, g, p.map(p.pts_off, p.pts_total), kPostDm);
   }
   rec(kKFilterPost, p.leaf_total);
   smooth_scan<D>(g, p, st, rec);
@@ -933,9 +943,9 @@ hipError_t run_iteration(const GroupArgs& g, const LaunchPlan& p, double* scal, 
   }
   // ---- smoother elements + observation statistics
   if (g.affine) {
-    POF_LAUNCH((k_filter_post<D, d, false>), kLdsPost, p.pts_total, st, g, p.map(p.pts_off, p.pts_total), 0);
+    POF_LAUNCH((k_filter_post<D, d, false>), kLdsPost, p.pts_total, st, g, p.map(p.pts_off, p.pts_total), kPostFull);
   } else {
-    POF_LAUNCH((k_filter_post<D, d, true>), kLdsPost, p.pts_total, st, g, p.map(p.pts_off, p.pts_total), 0);
+    POF_LAUNCH((k_filter_post<D, d, true>), kLdsPost, p.pts_total, st, g, p.map(p.pts_off, p.pts_total), kPostFull);
   }
   rec(kKFilterPost, p.leaf_total);
   // ---- smoother scan (reversed order)
@@ -969,6 +979,31 @@ hipError_t run_seq_iteration(const GroupArgs& g, const LaunchPlan& p, double* sc
   rec(kKFilterLeaf, p.leaf_total);
   POF_LAUNCH((k_seq_smooth<D>), kLdsSeq, p.one_total, st, g, one);
   rec(kKSmoothUp, p.leaf_total);
+  hipLaunchKernelGGL((k_smooth_post<D>), dim3((p.pts_total + 127) / 128), dim3(128), 0, st, g,
+                     p.map(p.pts_off, p.pts_total));
+  rec(kKSmoothPost, p.pts_total);
+  hipLaunchKernelGGL(k_reduce_part, dim3(p.chunk_total), dim3(256), 0, st, g, p.map(p.chunk_off, p.chunk_total));
+  hipLaunchKernelGGL(k_reduce_final, dim3(p.m), dim3(64), 0, st, g, p.map(p.chunk_off, p.chunk_total), d, scal);
+  rec(kKReduce, p.leaf_total);
+  return hipGetLastError();
+}
+
+// The smoother alone over filtered states already in the filter prefix arrays
+// (parallel: smoother elements, reverse scan; sequential: the RTS recursion),
+// then the smoothed means, objective and reduction of run_iteration.
+template <int D, int d>
+hipError_t run_smooth_only(const GroupArgs& g, const LaunchPlan& p, double* scal, int sequential, hipStream_t st,
+                           KernelTimer* t) {
+  Rec rec(t, st);
+  if (sequential) {
+    POF_LAUNCH((k_seq_smooth<D>), kLdsSeq, p.one_total, st, g, p.map(p.one_off, p.one_total));
+    rec(kKSmoothUp, p.leaf_total);
+  } else {
+    POF_LAUNCH((k_filter_post<D, d, false>), kLdsPost, p.pts_total, st, g, p.map(p.pts_off, p.pts_total),
+               kPostNoObs);
+    rec(kKFilterPost, p.leaf_total);
+    smooth_scan<D>(g, p, st, rec);
+  }
   hipLaunchKernelGGL((k_smooth_post<D>), dim3((p.pts_total + 127) / 128), dim3(128), 0, st, g,
                      p.map(p.pts_off, p.pts_total));
   rec(kKSmoothPost, p.pts_total);

@@ -109,6 +109,9 @@ SIGNATURES = {
         [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int64, _dp, _dp, _dp, _dp, _vp, _vp, _vp, ctypes.c_int, _vp,
          _vp, _vp, _vp, _dp, _dp, _dp, _vp],
     ),
+    "pof_smoothing": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int64, _dp, _dp, _vp, _vp, ctypes.c_int, _vp, _vp, _dp,
+                                     _vp]),
+    "pof_ekf": (ctypes.c_int, [_vp, _P(Problem), _dp, _dp, _dp, _dp, _vp, _vp, _dp, _dp, _vp]),
     "pof_batch_entries_for": (ctypes.c_int64, [ctypes.c_int64]),
     "pof_batch_bytes": (ctypes.c_int64, [ctypes.c_int, ctypes.c_int64, ctypes.c_int32]),
     "pof_batch_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int32, _P(_vp)]),

@@ -57,7 +57,7 @@ def set_up_solver(*, f, y0, ts, order):
     """(``convenience.py:13-45``): transitions, preconditioner, projections, residual model, x0.
 
     Returns the reference's dictionary keys; matrices are host float64 arrays,
-    ``dtm`` is the (constant) preconditioned transition pair, ``om`` the
+    ``dtm`` is the preconditioned transition pair repeated for every step, ``om`` the
     residual model ``E1 x - f(E0 x)`` and ``x0`` the preconditioned Taylor-mode
     initial state.
     """
@@ -82,7 +82,8 @@ def set_up_solver(*, f, y0, ts, order):
         "f": f,
         "y0": spec.y0,
         "ts": spec.ts,
-        "dtm": TransitionModel(F, QL),
+        # one (F, QL) per step, the same pair repeated (convenience.py:19), as read-only views
+        "dtm": TransitionModel(np.broadcast_to(F, (spec.Ns, D, D)), np.broadcast_to(QL, (spec.Ns, D, D))),
         "om": NonlinearModel(f, E0, E1),
         "x0": MVNSqrt(m0, L0),
         "E0": E0,

@@ -79,7 +79,30 @@ def linear_noiseless_filtering(x0, transition_models, observation_models):
     return filt, nll, obj, ssq
 
 
-def smoothing(transition_models, filter_trajectory):
-    raise NotImplementedError(
-        "the smoother runs fused with the filter on the GPU; call linear_filtsmooth(x0, dtm, dom)"
+def _smooth(transition_models, filter_trajectory, sequential):
+    import torch
+
+    F, QL = _constant_transition(transition_models)
+    fm = as_f64_device(filter_trajectory.mean)
+    fL = as_f64_device(filter_trajectory.chol)
+    n1, D = fm.shape
+    if fL.shape != (n1, D, D):
+        raise ValueError(f"filter_trajectory.chol has shape {tuple(fL.shape)}, expected {(n1, D, D)}")
+    sm = torch.empty_like(fm)
+    sL = torch.empty_like(fL)
+    obj = ctypes.c_double()
+    ctx = _lib.context(fm.device.index)
+    torch.cuda.synchronize(fm.device)
+    _lib.check(
+        _lib.load().pof_smoothing(ctx, D, n1 - 1, _lib.hptr(F), _lib.hptr(QL), _lib.dptr(fm), _lib.dptr(fL),
+                                  1 if sequential else 0, _lib.dptr(sm), _lib.dptr(sL), ctypes.byref(obj),
+                                  _lib.stream_ptr(fm.device)),
+        ctx, "pof_smoothing",
     )
+    return MVNSqrt(sm, sL), obj.value
+
+
+def smoothing(transition_models, filter_trajectory):
+    """Parallel square-root RTS smoother over given filtered states (``smoother.py:8-22``):
+    ``(MVNSqrt(means[N+1, D], chols[N+1, D, D]), obj)``."""
+    return _smooth(transition_models, filter_trajectory, False)

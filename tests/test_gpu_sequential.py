@@ -304,3 +304,90 @@ def test_sequential_linear_filtsmooth_random_model(gpu):
     assert rel(cov(filt.chol.cpu().numpy()), cov(rfilt.chol)) < 1e-9
     assert close(ssq, rssq, 1e-9) and close(nll, rnll, 1e-9) and close(obj, robj, 1e-9)
     assert fnll == nll and fssq == ssq
+
+
+@pytest.mark.parametrize("sequential", [False, True])
+def test_smoothing_on_given_filtered_states(gpu, sequential):
+    """parallel_filtsmooth.smoothing / sequential_filtsmooth.smoothing over the oracle's filtered
+    states of a random linear model (smoother.py:8-22, sequential_filtsmooth/smoother.py:8-28)."""
+    import torch
+
+    import pof.parallel_filtsmooth as pfs
+    import pof.sequential_filtsmooth as sfs
+    from oracle import model as om
+    from oracle import parallel as opar
+    from oracle import sequential as oseq
+    from pof.transitions import TransitionModel
+    from pof.utils import MVNSqrt
+
+    rng = np.random.default_rng(31)
+    d, q, Ns = 2, 2, 29
+    D = d * (q + 1)
+    F, QL = om.preconditioned_discretize(om.IWP(d, q))
+    dom = om.AffineModel(rng.standard_normal((Ns, d, D)), rng.standard_normal((Ns, d)), np.zeros((Ns, d, d)))
+    x0 = om.MVNSqrt(rng.standard_normal(D), np.tril(rng.standard_normal((D, D))))
+    filt, _, _, _ = opar.linear_noiseless_filtering(x0, F, QL, dom)
+    ref, robj = (oseq.smoothing if sequential else opar.smoothing)(F, QL, filt)
+    fn = sfs.smoothing if sequential else pfs.smoothing
+    dtm = TransitionModel(np.broadcast_to(F, (Ns, D, D)), np.broadcast_to(QL, (Ns, D, D)))
+    out, obj = fn(dtm, MVNSqrt(torch.as_tensor(filt.mean, device="cuda"), torch.as_tensor(filt.chol, device="cuda")))
+    torch.cuda.synchronize()
+    assert rel(out.mean.cpu().numpy(), ref.mean) < 1e-10
+    assert rel(cov(out.chol.cpu().numpy()), cov(ref.chol)) < 1e-9
+    assert close(obj, robj, 1e-9)
+
+
+@pytest.mark.parametrize("name,N,q", [("fitzhughnagumo", 200, 3), ("rigid_body", 64, 2)])
+def test_extended_kalman_filter_and_filtsmooth(gpu, name, N, q):
+    """sequential_filtsmooth.extended_kalman_filter / filtsmooth on the residual model of set_up_solver
+    (filter.py:9-30, __init__.py:5-10) against the oracle."""
+    import torch
+
+    import pof.convenience as pc
+    import pof.sequential_filtsmooth as sfs
+    from oracle import model as om
+    from oracle import sequential as oseq
+
+    ivp, oivp = _pair(name)
+    ts = np.linspace(oivp.t0, oivp.tmax, N)
+    setup = pc.set_up_solver(f=ivp.f, y0=ivp.y0, ts=ts, order=q)
+    os_ = om.set_up_solver(oivp, ts, q)
+    filt, nll, ssq = sfs.extended_kalman_filter(setup["x0"], setup["dtm"], setup["om"])
+    out, nll2, obj, ssq2 = sfs.filtsmooth(setup["x0"], setup["dtm"], setup["om"])
+    torch.cuda.synchronize()
+    rfilt, rnll, rssq = oseq.extended_kalman_filter(os_.x0, os_.F, os_.QL, os_, os_.Ns)
+    rout, rnll2, robj, rssq2 = oseq.filtsmooth(os_.x0, os_.F, os_.QL, os_, os_.Ns)
+    assert rel(filt.mean.cpu().numpy(), rfilt.mean) < 1e-9
+    assert rel(cov(filt.chol.cpu().numpy()), cov(rfilt.chol)) < 1e-8
+    assert close(nll, rnll, 1e-8)
+    assert rel(out.mean.cpu().numpy(), rout.mean) < 1e-9
+    assert rel(cov(out.chol.cpu().numpy()), cov(rout.chol)) < 1e-8
+    assert close(obj, robj, 1e-6)
+    assert nll2 == nll and ssq2 == ssq
+
+
+def test_reference_equality_parallel_vs_sequential(gpu):
+    """tests/test_filtsmooth.py:41-62 of the reference: q = 0, d = 1, all-zero model; the parallel and
+    sequential filters and smoothers agree exactly."""
+    import torch
+
+    import pof.parallel_filtsmooth as pfs
+    import pof.sequential_filtsmooth as sfs
+    from pof.observations import AffineModel
+    from pof.transitions import IWP, discretize_transitions
+    from pof.utils import MVNSqrt
+
+    iwp = IWP(num_derivatives=0, wiener_process_dimension=1)
+    dtm = discretize_transitions(iwp, np.arange(10))
+    Ns = dtm.F.shape[0]
+    dom = AffineModel(torch.ones((Ns, 1, 1), dtype=torch.float64, device="cuda"),
+                      torch.zeros((Ns, 1), dtype=torch.float64, device="cuda"), None)
+    x0 = MVNSqrt(np.zeros(1), np.zeros((1, 1)))
+    pf, *_ = pfs.linear_noiseless_filtering(x0, dtm, dom)
+    ps, _ = pfs.smoothing(dtm, pf)
+    sf, *_ = sfs.linear_noiseless_filter(x0, dtm, dom)
+    ss, _ = sfs.smoothing(dtm, sf)
+    torch.cuda.synchronize()
+    assert pf.mean.shape == ps.mean.shape == sf.mean.shape == (10, 1)
+    assert torch.equal(pf.mean, sf.mean) and torch.equal(pf.chol, sf.chol)
+    assert torch.equal(ps.mean, ss.mean) and torch.equal(ps.chol, ss.chol)
