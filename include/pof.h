@@ -245,6 +245,9 @@ int pof_batch_admit(pof_batch* batch, const pof_problem* prob, const pof_opts* o
                     int64_t* ticket);
 /* Enqueue one IEKS iteration for every admitted, unfinished IVP (asynchronous). */
 int pof_batch_launch(pof_batch* batch);
+/* 1 if pof_batch_finish would not wait (the launched iteration has completed, or
+ * nothing is launched), 0 while it runs. */
+int pof_batch_ready(const pof_batch* batch);
 /* Wait for the launched iteration, apply each IVP's convergence test
  * (pof/solver.py:36-57) and retire finished IVPs: factor pass, output
  * projection into their pof_out buffers, info fields.  Writes up to max_done

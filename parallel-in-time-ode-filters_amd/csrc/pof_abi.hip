@@ -1626,6 +1626,14 @@ int pof_batch_admit(pof_batch* b, const pof_problem* prob, const pof_opts* opts,
 
 int pof_batch_launch(pof_batch* b) { return b ? batch_launch(b) : POF_E_ARG; }
 
+int pof_batch_ready(const pof_batch* b) {
+  if (!b) return POF_E_ARG;
+  if (!b->launched || !b->need_sync) return 1;
+  const hipError_t q = hipEventQuery(b->ev);
+  if (q == hipErrorNotReady) return 0;
+  return q == hipSuccess ? 1 : POF_E_HIP;
+}
+
 int pof_batch_finish(pof_batch* b, int32_t max_done, int64_t* done, int32_t* n_done) {
   if (!b) return POF_E_ARG;
   std::vector<long long> tmp(std::max(max_done, 0));
@@ -1686,20 +1694,32 @@ int pof_solve_batch(pof_ctx* ctx, const pof_problem* probs, const pof_opts* opts
     int rc = admit_problem(groups[gid[i]].b, &probs[i], &opts[i], &outs[i], nullptr);
     if (rc != POF_OK) return rc;
   }
-  // iterate all groups concurrently (one stream each) until every IVP has finished
+  // iterate every group on its own stream, each at its own pace: a group's next
+  // iteration is enqueued as soon as its previous one has been finished, so a
+  // group with short iterations never waits for one with long ones
+  for (auto& g : groups) {
+    if (pof_batch_active(g.b) == 0) continue;
+    int rc = batch_launch(g.b);
+    if (rc != POF_OK) return rc;
+  }
   for (;;) {
     bool any = false;
     for (auto& g : groups) {
-      if (pof_batch_active(g.b) == 0) continue;
+      if (!g.b->launched) continue;
       any = true;
-      int rc = batch_launch(g.b);
-      if (rc != POF_OK) return rc;
-    }
-    if (!any) break;
-    for (auto& g : groups) {
+      if (g.b->need_sync) {
+        const hipError_t q = hipEventQuery(g.b->ev);
+        if (q == hipErrorNotReady) continue;
+        if (q != hipSuccess) return set_err(ctx, POF_E_HIP, "hipEventQuery: %s", hipGetErrorString(q));
+      }
       int rc = batch_finish(g.b, 0, nullptr, nullptr);
       if (rc != POF_OK) return rc;
+      if (pof_batch_active(g.b) > 0) {
+        rc = batch_launch(g.b);
+        if (rc != POF_OK) return rc;
+      }
     }
+    if (!any) break;
   }
   return POF_OK;
 }

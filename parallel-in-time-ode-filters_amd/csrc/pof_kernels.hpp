@@ -123,9 +123,18 @@ __global__ __launch_bounds__((Geo<D, kLdsLeaf>::BLOCK)) void k_filter_leaf(Group
   }
 }
 
+// Waves per SIMD the up-sweep combine is compiled for: at D = 12 the register
+// budget of 4 waves (128 VGPRs) costs a 12-byte spill and lifts the occupancy
+// from 3 waves; elsewhere the natural allocation is kept.
+template <int D>
+constexpr int up_min_waves() {
+  return D == 12 ? 4 : 1;
+}
+
 // Up-sweep: out[i] = in[2i] o in[2i+1]
 template <int D>
-__global__ __launch_bounds__((Geo<D, kLdsFull>::BLOCK)) void k_filter_up(GroupArgs g, LaunchMap lm, int l) {
+__global__ __launch_bounds__((Geo<D, kLdsFull>::BLOCK), up_min_waves<D>()) void k_filter_up(GroupArgs g, LaunchMap lm,
+                                                                                             int l) {
   extern __shared__ double smem[];
   auto cx = make_ctx<D, kLdsFull>(g.F, g.LQ, g.N1z, smem);
   int j, i;

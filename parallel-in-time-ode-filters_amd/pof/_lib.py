@@ -118,6 +118,7 @@ SIGNATURES = {
     "pof_batch_destroy": (None, [_vp]),
     "pof_batch_admit": (ctypes.c_int, [_vp, _P(Problem), _P(Opts), _P(Out), _P(ctypes.c_int64)]),
     "pof_batch_launch": (ctypes.c_int, [_vp]),
+    "pof_batch_ready": (ctypes.c_int, [_vp]),
     "pof_batch_finish": (ctypes.c_int, [_vp, ctypes.c_int32, _P(ctypes.c_int64), _P(ctypes.c_int32)]),
     "pof_batch_active": (ctypes.c_int, [_vp]),
     "pof_batch_free_entries": (ctypes.c_int64, [_vp]),

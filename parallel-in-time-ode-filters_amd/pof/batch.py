@@ -173,7 +173,9 @@ class Batch:
         _fill_out(out, tensors)
         import torch
 
-        torch.cuda.synchronize(self.device)
+        # the output buffers come from torch's allocator on torch's stream; the batch writes them
+        # on its own stream (waiting for the whole device would stall the other groups)
+        torch.cuda.current_stream(self.device).synchronize()
         ticket = ctypes.c_int64()
         rc = self._lib.pof_batch_admit(self._h, ctypes.byref(prob), ctypes.byref(opts), ctypes.byref(out),
                                        ctypes.byref(ticket))
@@ -188,6 +190,17 @@ class Batch:
         """Enqueue one iteration for every admitted, unfinished IVP (returns at once)."""
         _lib.check(self._lib.pof_batch_launch(self._h), self._ctx, "pof_batch_launch")
         self._launched = True
+
+    @property
+    def launched(self):
+        return self._launched
+
+    def ready(self):
+        """True if :meth:`finish` would not wait (the launched iteration has completed)."""
+        rc = self._lib.pof_batch_ready(self._h)
+        if rc < 0:
+            _lib.check(rc, self._ctx, "pof_batch_ready")
+        return rc == 1
 
     def finish(self):
         """Wait for the launched iteration; returns ``[(ticket, (MVNSqrt, info)), ...]`` of finished IVPs."""

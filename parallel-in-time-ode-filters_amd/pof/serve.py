@@ -6,9 +6,12 @@ loop works as follows:
 
 * It admits waiting requests in arrival order as soon as their group has a
   free slot and a free workspace range (admission by device memory).
-* It iterates every group that has IVPs in flight (one IEKS iteration per
-  loop turn; the groups' launches overlap on their own streams). Iterates
-  stay in HBM between turns.
+* It iterates every group that has IVPs in flight, each at its own pace on
+  its own stream: as soon as a group's iteration has completed (an event
+  query, no wait) its finished IVPs are retired, waiting requests are
+  admitted into it and its next iteration is enqueued, so a group with short
+  iterations never waits for one with long ones. Iterates stay in HBM
+  between iterations.
 * It retires each IVP the iteration it converges, copies its result to the
   host, and frees its range for the next request.
 
@@ -84,7 +87,8 @@ class Server:
             while nxt < n and recs[order[nxt]].arrival <= now:
                 waiting.append(int(order[nxt]))
                 nxt += 1
-            # admission in arrival order; a request that does not fit waits (its group is full)
+            # admission in arrival order; a request whose group is full or has an iteration in flight
+            # waits (admission happens between a group's iterations)
             still = collections.deque()
             blocked = set()
             while waiting:
@@ -95,6 +99,10 @@ class Server:
                     still.append(rid)
                     continue
                 g = self._group(f.d, q)
+                if g.launched:
+                    blocked.add(key)
+                    still.append(rid)
+                    continue
                 try:
                     t = g.admit(f, y0, ts, init=init, calibrate=self.calibrate, maxiters=self.maxiters)
                 except _batch.BatchFull:
@@ -108,10 +116,14 @@ class Server:
                 if nxt < n:
                     time.sleep(max(0.0, recs[order[nxt]].arrival - (clock() - t0)))
                 continue
-            live = [(k, g) for k, g in self.groups.items() if g.active]
-            for _, g in live:
-                g.launch()
-            for k, g in live:
+            for g in self.groups.values():
+                if not g.launched and g.active:
+                    g.launch()
+            progressed = False
+            for k, g in self.groups.items():
+                if not g.launched or not g.ready():
+                    continue
+                progressed = True
                 for t, (ys, info) in g.finish():
                     rid = inflight.pop((k, t))
                     r = recs[rid]
@@ -121,6 +133,8 @@ class Server:
                     r.status = info["status"]
                     if self.keep_means:
                         r.means = m
+            if not progressed:
+                time.sleep(20e-6)
         return recs
 
     def close(self):
