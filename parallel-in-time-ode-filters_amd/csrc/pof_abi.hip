@@ -816,19 +816,27 @@ int batch_launch(pof_batch* b) {
   if (!b->dirty_slots.empty() || !b->init_host.empty() || !b->init_dev.empty() || !b->init_zero.empty()) {
     const int D = b->D;
     const size_t DD = (size_t)D * D, S = (size_t)b->slots;
-    for (int j : b->dirty_slots) {
-      HIP_TRY(hipMemcpyAsync(b->Pd + (size_t)j * D, &b->hPd[(size_t)j * D], D * 8, hipMemcpyHostToDevice, b->st));
-      HIP_TRY(hipMemcpyAsync(b->x0m + (size_t)j * D, &b->hx0m[(size_t)j * D], D * 8, hipMemcpyHostToDevice, b->st));
-      HIP_TRY(hipMemcpyAsync(b->x0L + (size_t)j * DD, &b->hx0L[(size_t)j * DD], DD * 8, hipMemcpyHostToDevice,
-                             b->st));
-      HIP_TRY(hipMemcpyAsync(b->params + (size_t)j * kMaxParams, &b->hparams[(size_t)j * kMaxParams],
-                             kMaxParams * 8, hipMemcpyHostToDevice, b->st));
+    if (!b->dirty_slots.empty()) {
+      // one copy per table over the slot range of the new IVPs: the host
+      // tables mirror the device ones for every slot (cur through the flips
+      // of each launch), so the slots in between are rewritten unchanged
+      int lo = b->slots, hi = -1;
+      for (int j : b->dirty_slots) {
+        lo = std::min(lo, j);
+        hi = std::max(hi, j);
+      }
+      const size_t n = (size_t)(hi - lo + 1), l = (size_t)lo;
+      HIP_TRY(hipMemcpyAsync(b->Pd + l * D, &b->hPd[l * D], n * D * 8, hipMemcpyHostToDevice, b->st));
+      HIP_TRY(hipMemcpyAsync(b->x0m + l * D, &b->hx0m[l * D], n * D * 8, hipMemcpyHostToDevice, b->st));
+      HIP_TRY(hipMemcpyAsync(b->x0L + l * DD, &b->hx0L[l * DD], n * DD * 8, hipMemcpyHostToDevice, b->st));
+      HIP_TRY(hipMemcpyAsync(b->params + l * kMaxParams, &b->hparams[l * kMaxParams], n * kMaxParams * 8,
+                             hipMemcpyHostToDevice, b->st));
       for (int t = 0; t < 4; ++t)
-        HIP_TRY(hipMemcpyAsync(b->outp + t * S + j, &b->hout[t * S + j], sizeof(double*), hipMemcpyHostToDevice,
+        HIP_TRY(hipMemcpyAsync(b->outp + t * S + l, &b->hout[t * S + l], n * sizeof(double*), hipMemcpyHostToDevice,
                                b->st));
-      HIP_TRY(hipMemcpyAsync(b->cur + j, &b->hcur[j], sizeof(int), hipMemcpyHostToDevice, b->st));
-      HIP_TRY(hipMemcpyAsync(b->wchol + j, &b->hwchol[j], sizeof(int), hipMemcpyHostToDevice, b->st));
-      HIP_TRY(hipMemcpyAsync(b->pfull + j, &b->hpfull[j], sizeof(int), hipMemcpyHostToDevice, b->st));
+      HIP_TRY(hipMemcpyAsync(b->cur + l, &b->hcur[l], n * sizeof(int), hipMemcpyHostToDevice, b->st));
+      HIP_TRY(hipMemcpyAsync(b->wchol + l, &b->hwchol[l], n * sizeof(int), hipMemcpyHostToDevice, b->st));
+      HIP_TRY(hipMemcpyAsync(b->pfull + l, &b->hpfull[l], n * sizeof(int), hipMemcpyHostToDevice, b->st));
     }
     b->dirty_slots.clear();
     HIP_TRY(hipMemcpyAsync(b->kind, b->hkind.data(), S * sizeof(int), hipMemcpyHostToDevice, b->st));
@@ -883,6 +891,7 @@ int batch_launch(pof_batch* b) {
   }
   hipLaunchKernelGGL(k_flip_cur, dim3((b->plan.m + 255) / 256), dim3(256), 0, b->st, b->cur, b->plan.dev, b->plan.m);
   HIP_TRY(hipGetLastError());
+  for (int j : b->active) b->hcur[j] ^= 1;  // host mirror of the flip
   HIP_TRY(collect_timing(ctx, b->st));
   if (b->need_sync) {
     HIP_TRY(hipMemcpyAsync(b->scal_host, b->scal_dev, (size_t)b->plan.m * 8 * sizeof(double),

@@ -49,6 +49,13 @@ constexpr int split_of() {
   return (8 * D <= 64) ? 4 : (4 * D <= 64) ? 2 : 1;
 }
 
+// Largest observation (ODE) dimension d <= 4 with d | D: the widest leaf
+// element Z = [Zr | 0] (nonzero in its first d columns) an engine of state
+// dimension D combines.
+constexpr int max_obs_dim(int D) {
+  return (D % 4 == 0) ? 4 : (D % 3 == 0) ? 3 : (D % 2 == 0) ? 2 : 1;
+}
+
 // Lanes per worker and the padded row length for state dimension D with
 // rows split over S lanes.
 template <int D_, int S_>
@@ -67,10 +74,11 @@ enum LdsKind : int { kLdsFull = 0, kLdsState, kLdsSmooth, kLdsLeaf, kLdsPost, kL
 
 template <int K>
 struct LdsNeeds {
-  static constexpr int NT = (K == kLdsFull) ? 6
+  static constexpr int NT = (K == kLdsFull) ? 5
                             : (K == kLdsState || K == kLdsLeaf) ? 4
-                            : (K == kLdsPost || K == kLdsSeq) ? 5
-                                                              : 2;
+                            : (K == kLdsSeq) ? 5
+                            : (K == kLdsPost) ? 3
+                                              : 2;
   static constexpr int NV = (K == kLdsFull || K == kLdsState || K == kLdsSeq) ? ((K == kLdsFull) ? 8 : 4)
                             : (K == kLdsPost) ? 2
                                               : 1;
@@ -414,7 +422,15 @@ __device__ __forceinline__ void tile_t_times_tile(int r, const double* T1, const
 //
 // The pivots [I0, I1) are applied; FINISH clears the strict upper part at the
 // end (so one factorisation can be run in pieces, with other work between).
-template <int S, int NR, int B1, int B2, int NG = 1, int I0 = 0, int I1 = 1 << 20, bool FINISH = true>
+//
+// W2 < B2 declares a banded second block: row r's entries there are zero
+// beyond compact column max(r, W2) (W2 = 0: lower triangular).  Each pivot row
+// i < B1 then keeps that band (the reflectors before it span columns <= i +
+// the band), so the reflector of pivot i is zero beyond column max(i, W2) of
+// the second block and those columns are left out of its sums and updates.
+// They would only add exact zeros: the factor is the same.
+template <int S, int NR, int B1, int B2, int NG = 1, int I0 = 0, int I1 = 1 << 20, bool FINISH = true,
+          int W2 = 1 << 20>
 __device__ __forceinline__ void lq(double (&a)[(padded<B1, S>() + padded<B2, S>()) / S], int row, int sub,
                                    double* bc) {
   static_assert(NG == 1 || NG == 2, "one or two matrices");
@@ -432,15 +448,18 @@ __device__ __forceinline__ void lq(double (&a)[(padded<B1, S>() + padded<B2, S>(
     if (i == NC - 1) break;  // a length-1 tail: H = I
     const int pc = (i < B1) ? i : P1 + (i - B1);  // padded column of pivot i
     const int t0 = pc / S, k0 = pc % S;
+    // end (exclusive) of the register columns the reflector can touch
+    const int band = (i > W2) ? i : W2;
+    const int te = (i < B1 && W2 < B2 && P1 + band < NP) ? (P1 + band) / S + 1 : NX;
     if (valid && li == i) {
 #pragma unroll
-      for (int t = t0; t < NX; ++t) brow[t * S + sub] = a[t];
+      for (int t = t0; t < te; ++t) brow[t * S + sub] = a[t];
     }
     wsync();
     const double alpha = brow[pc];
     double p[NX];
 #pragma unroll
-    for (int t = t0; t < NX; ++t) p[t] = brow[t * S + sub];
+    for (int t = t0; t < te; ++t) p[t] = brow[t * S + sub];
     // the pivot's own group of S columns: only columns after pc are tail
     const bool tl = sub > k0;
     const double p0 = tl ? p[t0] : 0.0;
@@ -452,7 +471,7 @@ __device__ __forceinline__ void lq(double (&a)[(padded<B1, S>() + padded<B2, S>(
     sq[0] = p0 * p0;
     dt[0] = a[t0] * p0;
 #pragma unroll
-    for (int t = t0 + 1; t < NX; ++t) {
+    for (int t = t0 + 1; t < te; ++t) {
       sq[(t - t0) % CH] = fma(p[t], p[t], sq[(t - t0) % CH]);
       dt[(t - t0) % CH] = fma(a[t], p[t], dt[(t - t0) % CH]);
     }
@@ -484,7 +503,7 @@ __device__ __forceinline__ void lq(double (&a)[(padded<B1, S>() + padded<B2, S>(
       const double tws = tw * scal;
       a[t0] = owner ? a[t0] - tw : fma(-tws, p0, a[t0]);
 #pragma unroll
-      for (int t = t0 + 1; t < NX; ++t) a[t] = fma(-tws, p[t], a[t]);
+      for (int t = t0 + 1; t < te; ++t) a[t] = fma(-tws, p[t], a[t]);
     }
   }
   if constexpr (FINISH) {
@@ -981,9 +1000,10 @@ __device__ void filter_combine(const Ctx<D, SS>& cx, const FiltArrays& x1, const
   double* TZ2 = cx.tile(1);
   double* TX11 = cx.tile(2);
   double* TX21 = cx.tile(3);
-  // tiles 4-5 and vectors 4-7 exist only in the full-combine LDS layout
-  double* TX22 = state_only ? nullptr : cx.tile(4);
-  double* TA1 = state_only ? nullptr : cx.tile(5);
+  // tile 4 and vectors 4-7 exist only in the full-combine LDS layout; Xi22
+  // takes the tile of Z2, whose last use precedes the Xi factorisation
+  double* TX22 = state_only ? nullptr : TZ2;
+  double* TA1 = state_only ? nullptr : cx.tile(4);
   double* Veta2 = cx.vec(0);
   double* Vb1 = cx.vec(1);
   double* Vu = cx.vec(2);
@@ -1018,6 +1038,25 @@ __device__ void filter_combine(const Ctx<D, SS>& cx, const FiltArrays& x1, const
     }
   }
   wsync();
+  // the terms of eta that need Z2: s2 = eta2 - Z2 Z2^T b1 and U1^T s2 (the
+  // rest of eta follows the factorisation)
+  if constexpr (!state_only) {
+    double* Vt = cx.vec(4);
+    double* Vs2 = cx.vec(5);
+    double* Vu2 = cx.vec(6);
+    {
+      const double t = lds_dot<S, D, LD>(TZ2 + rr, Vb1, s);
+      if (act) Vt[r] = t;
+    }
+    wsync();
+    const double s2 = eta2r - lds_dot<S, D, 1>(TZ2 + rr * LD, Vt, s);
+    if (act) Vs2[r] = s2;
+    wsync();
+    {
+      const double uu = lds_dot<S, D, LD>(TU1 + rr, Vs2, s);
+      if (act) Vu2[r] = uu;
+    }
+  }
   // Xi_ = [[U1^T Z2, I], [Z2, 0]]: rows 0..2D-1, column blocks (D, D)
   double xr[2 * NL];
   {
@@ -1038,13 +1077,13 @@ __device__ void filter_combine(const Ctx<D, SS>& cx, const FiltArrays& x1, const
   if constexpr (state_only) {
     // Xi11 and Xi21 are final after the first D pivots; the last D pivots only
     // form Xi22, which enters Z alone
-    lq<S, 2 * D, D, D, 1, 0, D / 2, false>(xr, r, s, cx.bc());
+    lq<S, 2 * D, D, D, 1, 0, D / 2, false, 0>(xr, r, s, cx.bc());
     load_dist<S, D>(a2, x2.A + (i2 * D + rr) * D, s);
     load_dist<S, D>(u2, x2.U + (i2 * D + rr) * D, s);
     zero(z1);
-    lq<S, 2 * D, D, D, 1, D / 2, D>(xr, r, s, cx.bc());
+    lq<S, 2 * D, D, D, 1, D / 2, D, true, 0>(xr, r, s, cx.bc());
   } else {
-    lq<S, 2 * D, D, D, 1, 0, D, false>(xr, r, s, cx.bc());
+    lq<S, 2 * D, D, D, 1, 0, D, false, 0>(xr, r, s, cx.bc());
     load_dist<S, D>(a2, x2.A + (i2 * D + rr) * D, s);
     load_dist<S, D>(u2, x2.U + (i2 * D + rr) * D, s);
     load_dist<S, D>(z1, x1.Z + (i1 * D + rz) * D, s);  // Z1 rows feed the rows D..2D-1
@@ -1102,23 +1141,10 @@ __device__ void filter_combine(const Ctx<D, SS>& cx, const FiltArrays& x1, const
   // held in registers through the factorisation below)
   double eo = 0.0;
   if constexpr (!state_only) {
-    double* Vt = cx.vec(4);
-    double* Vs2 = cx.vec(5);
-    double* Vu2 = cx.vec(6);
+    const double* Vs2 = cx.vec(5);
+    const double* Vu2 = cx.vec(6);
     double* Vs3 = cx.vec(7);
-    {
-      const double t = lds_dot<S, D, LD>(TZ2 + rr, Vb1, s);
-      if (act) Vt[r] = t;
-    }
-    wsync();
-    const double s2 = eta2r - lds_dot<S, D, 1>(TZ2 + rr * LD, Vt, s);
-    if (act) Vs2[r] = s2;
-    wsync();
-    {
-      const double uu = lds_dot<S, D, LD>(TU1 + rr, Vs2, s);
-      if (act) Vu2[r] = uu;
-    }
-    wsync();
+    const double s2 = Vs2[rr];
     double w2[NL];
     load_dist<S, D>(w2, Vu2, s);
     rsolve_lowerT_s<S, D, LD>(w2, TX11, s);  // Xi11^-1 (U1^T s2)
@@ -1161,7 +1187,9 @@ __device__ void filter_combine(const Ctx<D, SS>& cx, const FiltArrays& x1, const
       jr[NL + t] = act ? u2[t] : (zl ? z1[t] : 0.0);
     }
   }
-  lq<S, D, D, D, state_only ? 1 : 2>(jr, r, s, cx.bc());
+  // bands: U2 is lower triangular; Z1 too, except a leaf element's [Zr | 0]
+  // (nonzero in its first d columns only)
+  lq<S, D, D, D, state_only ? 1 : 2, 0, 1 << 20, true, state_only ? 0 : max_obs_dim(D) - 1>(jr, r, s, cx.bc());
   if (act) {
     if (s == 0) (state_only ? so.m : y.b)[o * D + r] = bb;
     store_dist<S, D>((state_only ? so.L : y.U) + (o * D + r) * D, jr, s);

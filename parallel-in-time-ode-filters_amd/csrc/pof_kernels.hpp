@@ -229,11 +229,13 @@ __global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK)) void k_filter_post(Group
     }
     return;
   }
+  // three tiles: L (dead once Phi is formed) then E; Phi11 (dead once S's
+  // rows are formed) then S; D
   double* TL = cx.tile(0);
   double* TP11 = cx.tile(1);
-  double* TE = cx.tile(2);
-  double* TD = cx.tile(3);
-  double* TS = cx.tile(4);
+  double* TE = TL;
+  double* TD = cx.tile(2);
+  double* TS = TP11;
   double* Vm = cx.vec(0);
   double* Vfm = cx.vec(1);
   // global reads first: the filtered state and the linearisation point of obs model s
