@@ -6,7 +6,10 @@ IWP order q = 3 (state dimension D = 12), float64 throughout, constant
 initial trajectory, exactly 10 IEKS iterations per solve.  Inputs are tiny
 (y0, ts); everything else lives in HBM.
 
-Also measured (same line, objects "batched", "served" and "precision"): a
+Also measured (same line): "converged", the same solve run to the
+convergence criterion (at most 200 iterations); "accuracy", the error of the
+10-iteration solve against the FP64 oracle on the first 512 steps of the
+grid; and (objects "batched", "served" and "precision") a
 batch of 2048 mixed IVPs solved together vs the plain one-IVP-per-launch
 scan; a Poisson request stream served with continuous batching vs a FIFO
 plain per-request scan (p50/p99 latency, solves/s); and Lorenz-96 with
@@ -127,6 +130,54 @@ def gpu_run(steps, warmup):
         "instrumented_iteration_ms": total_ms / ITERS,
         "iteration_tflops_algorithmic": iter_flops / (total_ms / ITERS * 1e-3) / 1e12,
     }
+
+
+def plain_checks(prefix=512, maxiters=200):
+    """The plain workload's other figures: run to the convergence criterion
+    (solver.py:36-57; capped at `maxiters`) on the full grid, and the error of
+    the 10-iteration solve against the FP64 oracle on the first `prefix` steps
+    of the same grid (same dt), where the oracle finishes in a fraction of a second."""
+    import numpy as np
+    import torch
+
+    import pof.ivp
+    import pof.solver
+    from oracle import problems as op
+    from oracle import solver as osol
+    from pof import workloads
+
+    ivp = pof.ivp.lorenz63()
+    ts = np.linspace(0.0, 10.0, NS + 1)
+    pof.solver.solve(f=ivp.f, y0=ivp.y0, ts=ts, order=Q, init="constant", maxiters=2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ys, info = pof.solver.solve(f=ivp.f, y0=ivp.y0, ts=ts, order=Q, init="constant", maxiters=maxiters)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    conv = {
+        "iterations": info["iterations"],
+        "status": info.get("status"),
+        "seconds": t,
+        "ms_per_iteration": 1e3 * t / max(info["iterations"], 1.0),
+        "maxiters": maxiters,
+        "finite": bool(torch.isfinite(ys.mean).all()),
+    }
+    tp = ts[: prefix + 1]
+    yp, ip = pof.solver.solve(f=ivp.f, y0=ivp.y0, ts=tp, order=Q, init="constant", fixed_iters=ITERS)
+    torch.cuda.synchronize()
+    ref, rinfo = osol.solve(op.lorenz63(), tp, Q, init="constant", fixed_iters=ITERS)
+    elem, norm = workloads.max_rel_err(yp.mean.cpu().numpy(), ref.mean)
+    c = yp.chol.cpu().numpy() / ip["sigma_squared"]
+    cr = ref.chol / rinfo["sigma_squared"]
+    cc = np.einsum("nij,nkj->nik", c, c)
+    ccr = np.einsum("nij,nkj->nik", cr, cr)
+    acc = {
+        "check": f"first {prefix} steps of the grid (dt = 10/{NS}), {ITERS} fixed iterations, vs the FP64 oracle",
+        "max_rel_err_means": elem,
+        "normwise_err_means": norm,
+        "cov_rel_err": float(np.linalg.norm(cc - ccr) / np.linalg.norm(ccr)),
+    }
+    return conv, acc
 
 
 def cpu_baseline():
@@ -358,6 +409,8 @@ def main():
     ap.add_argument("--copy-probe", action="store_true", help="only run the HBM counter calibration copy")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched and served measurements")
     ap.add_argument("--no-precision", action="store_true", help="skip the Lorenz-96 D = 128 measurement")
+    ap.add_argument("--no-checks", action="store_true",
+                    help="skip the run-to-convergence solve and the oracle prefix check of the plain workload")
     ap.add_argument("--precision-only", action="store_true",
                     help="only the Lorenz-96 D = 128 measurement (printed as its own JSON object)")
     ap.add_argument("--precision-ns", type=int, default=8192)
@@ -382,6 +435,11 @@ def main():
     g = gpu_run(args.steps, args.warmup)
     if args.verbose:
         print(json.dumps(g["per_class"], indent=1), file=sys.stderr)
+    converged = accuracy = None
+    if not args.no_checks:
+        converged, accuracy = plain_checks()
+        if args.verbose:
+            print(json.dumps({"converged": converged, "accuracy": accuracy}), file=sys.stderr, flush=True)
     batched = served = None
     if not args.no_batch:
         batched = batched_run(args.batch_ivps, maxiters=100, plain_sample=32, oracle_sample=4)
@@ -423,6 +481,8 @@ def main():
         "cpu_baseline": cpu,
         "kernel_classes_ms_per_solve": {k: round(v["ms"], 4) for k, v in g["per_class"].items()},
         "finite": g["finite"],
+        "converged": converged,
+        "accuracy": accuracy,
         "batched": batched,
         "served": served,
         "precision": precision,

@@ -70,19 +70,26 @@ struct Lay {
 
 // LDS layout kinds: how many tiles / broadcast vectors a worker of each
 // kernel needs and whether the block stages the model constants F, LQ, N1z.
-enum LdsKind : int { kLdsFull = 0, kLdsState, kLdsSmooth, kLdsLeaf, kLdsPost, kLdsSeq };
+enum LdsKind : int { kLdsFull = 0, kLdsState, kLdsSmooth, kLdsLeaf, kLdsPost, kLdsSeq, kLdsLeaf1 };
+
+// Widest observation dimension of the per-worker engines.
+constexpr int kMaxObs = 4;
 
 template <int K>
 struct LdsNeeds {
   static constexpr int NT = (K == kLdsFull) ? 5
-                            : (K == kLdsState || K == kLdsLeaf) ? 4
+                            : (K == kLdsState) ? 4
                             : (K == kLdsSeq) ? 5
                             : (K == kLdsPost) ? 3
-                                              : 2;
+                            : (K == kLdsLeaf) ? 1
+                                              : 2;  // kLdsSmooth, kLdsLeaf1
   static constexpr int NV = (K == kLdsFull || K == kLdsState || K == kLdsSeq) ? ((K == kLdsFull) ? 8 : 4)
                             : (K == kLdsPost) ? 2
                                               : 1;
-  static constexpr bool CONSTS = (K == kLdsLeaf || K == kLdsPost || K == kLdsSeq);
+  static constexpr bool CONSTS = (K == kLdsLeaf || K == kLdsLeaf1 || K == kLdsPost || K == kLdsSeq);
+  // the element kernels' small blocks after the tiles: Psi11 (at most kMaxObs
+  // rows of stride LD) and the gain K (D rows of stride kMaxObs)
+  static constexpr bool SMALL = (K == kLdsLeaf || K == kLdsLeaf1);
 };
 
 // Row split per kernel kind: the scan combines use the widest split (their
@@ -90,7 +97,7 @@ struct LdsNeeds {
 // of a row repeats, keep one lane per row and two workers per wavefront.
 template <int D, int K>
 constexpr int kind_split() {
-  return (K == kLdsLeaf || K == kLdsPost || K == kLdsSeq) ? 1 : split_of<D>();
+  return (K == kLdsLeaf || K == kLdsLeaf1 || K == kLdsPost || K == kLdsSeq) ? 1 : split_of<D>();
 }
 
 template <int D_, int K = kLdsFull>
@@ -100,7 +107,8 @@ struct Geo : Lay<D_, kind_split<D_, K>()> {
   static constexpr int NT = LdsNeeds<K>::NT;        // tiles per worker
   static constexpr int NV = LdsNeeds<K>::NV;        // broadcast vectors per worker (2D entries each)
   static constexpr int BC = 4 * L::DP + 2;          // pivot-row broadcast of two matrices of 2 DP columns
-  static constexpr int WORKER = BC + NV * 2 * D_ + NT * TILE;
+  static constexpr int SMALL = LdsNeeds<K>::SMALL ? kMaxObs * L::LD + kMaxObs * D_ : 0;
+  static constexpr int WORKER = BC + NV * 2 * D_ + NT * TILE + SMALL;
   static constexpr int CONSTS = LdsNeeds<K>::CONSTS ? 3 * TILE : 0;  // F, LQ, N1z
   static constexpr size_t bytes_for(int wpb) { return sizeof(double) * (CONSTS + wpb * WORKER); }
   // workers per block: as many as fit in 64 KiB of LDS, at most 256 threads
@@ -137,6 +145,10 @@ struct Ctx {
 // tests of one function (which otherwise duplicates whole regions per branch
 // and multiplies register pressure).
 __device__ __forceinline__ int opaque(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+__device__ __forceinline__ size_t opaque_sz(size_t v) {
   asm volatile("" : "+v"(v));
   return v;
 }
@@ -841,10 +853,13 @@ __device__ void filter_element(const Ctx<D, SS>& cx, const ModelArgs& a, int k,
   Obs<D, d, EK1> ob;
   ob.load(a, k);
 
-  double* T0 = cx.tile(0);
-  double* T1 = cx.tile(1);
-  double* T2 = cx.tile(2);
-  double* T3 = cx.tile(3);
+  // LDS (kLdsLeaf / kLdsLeaf1 layouts): tile 0 U rows; tile 1 N1 (first
+  // element only); then Psi11 (d rows, stride LD) and K (D rows, stride kMaxObs)
+  static_assert(d <= kMaxObs, "observation dimension beyond the element layout");
+  double* T2 = cx.tile(0);
+  double* T0 = cx.tile(1);
+  double* T1 = cx.tile(first ? 2 : 1);
+  double* T3 = T1 + kMaxObs * LD;
   double* V0 = cx.vec(0);
   const double* N1 = cx.sN1;
   double m1r = 0.0;
@@ -911,7 +926,7 @@ __device__ void filter_element(const Ctx<D, SS>& cx, const ModelArgs& a, int k,
       rsolve_lower<d, LD>(kk, T1);
       if (s == 0) {
 #pragma unroll
-        for (int c = 0; c < d; ++c) T3[(r - d) * LD + c] = kk[c];
+        for (int c = 0; c < d; ++c) T3[(r - d) * kMaxObs + c] = kk[c];
       }
       // compact columns d .. D + d - 1 of the row are U's columns 0 .. D - 1
       constexpr int P1 = L::DP;
@@ -939,7 +954,7 @@ __device__ void filter_element(const Ctx<D, SS>& cx, const ModelArgs& a, int k,
   {
     double kr[d];
 #pragma unroll
-    for (int i = 0; i < d; ++i) kr[i] = T3[rr * LD + i];
+    for (int i = 0; i < d; ++i) kr[i] = T3[rr * kMaxObs + i];
     load_dist<S, D>(U, T2 + rr * LD, s);
     // A = F - (K H) F ; b = m1 - K (H m1 + c)
     double kh[kh_len<D, d, EK1>()];
@@ -1015,16 +1030,14 @@ __device__ void filter_combine(const Ctx<D, SS>& cx, const FiltArrays& x1, const
   // the memory latency is paid once rather than once per phase
   const double b1r = b1p[i1 * D + rr];
   const double eta2r = x2.eta[i2 * D + rr];
-  const double b2r = x2.b[i2 * D + rr];
   double a2[NL], u2[NL], z1[NL];
-  double eta1r = 0.0;
+  double b2r, eta1r = 0.0;
   {
     double u1[NL], z2[NL], a1[NL];
     load_dist<S, D>(u1, U1p + (i1 * D + rr) * D, s);
     load_dist<S, D>(z2, x2.Z + (i2 * D + rr) * D, s);
     if constexpr (!state_only) {
       load_dist<S, D>(a1, x1.A + (i1 * D + rr) * D, s);
-      eta1r = x1.eta[i1 * D + rr];
     }
     if (act) {
 #pragma unroll
@@ -1080,6 +1093,7 @@ __device__ void filter_combine(const Ctx<D, SS>& cx, const FiltArrays& x1, const
     lq<S, 2 * D, D, D, 1, 0, D / 2, false, 0>(xr, r, s, cx.bc());
     load_dist<S, D>(a2, x2.A + (i2 * D + rr) * D, s);
     load_dist<S, D>(u2, x2.U + (i2 * D + rr) * D, s);
+    b2r = x2.b[i2 * D + rr];
     zero(z1);
     lq<S, 2 * D, D, D, 1, D / 2, D, true, 0>(xr, r, s, cx.bc());
   } else {
@@ -1087,6 +1101,8 @@ __device__ void filter_combine(const Ctx<D, SS>& cx, const FiltArrays& x1, const
     load_dist<S, D>(a2, x2.A + (i2 * D + rr) * D, s);
     load_dist<S, D>(u2, x2.U + (i2 * D + rr) * D, s);
     load_dist<S, D>(z1, x1.Z + (i1 * D + rz) * D, s);  // Z1 rows feed the rows D..2D-1
+    b2r = x2.b[i2 * D + rr];
+    eta1r = x1.eta[i1 * D + rr];
     lq<S, 2 * D, D, D, 1, D>(xr, r, s, cx.bc());
   }
   if (act) {
@@ -1131,10 +1147,11 @@ __device__ void filter_combine(const Ctx<D, SS>& cx, const FiltArrays& x1, const
     const double u = lds_dot<S, D, LD>(TU1 + rr, Veta2, s);
     if (act) Vu[r] = u;
     wsync();
-    const double sv = b1r + lds_dot<S, D, 1>(TU1 + rr * LD, Vu, s);
+    const double sv = Vb1[rr] + lds_dot<S, D, 1>(TU1 + rr * LD, Vu, s);
     if (act) Vs[r] = sv;
     wsync();
     bb = b2r + row_dot<S, D>(wr, Vs, s);
+    pin1(bb);  // formed here: sunk to its store, it holds b2 and partial sums through the last factorisation
   }
   // eta = A1^T (I - Xi21 Xi11^-1 U1^T)(eta2 - Z2 Z2^T b1) + eta1 and A = W A1
   // (A's rows go to the LDS of U1, unused from here on, so that they are not
@@ -1159,6 +1176,7 @@ __device__ void filter_combine(const Ctx<D, SS>& cx, const FiltArrays& x1, const
     }
     wsync();
     eo = eta1r + lds_dot<S, D, LD>(TA1 + rr, Vs3, s);
+    pin1(eo);
     {
       double wf[D];
       gather<S, D>(wr, wf);
@@ -1190,6 +1208,9 @@ __device__ void filter_combine(const Ctx<D, SS>& cx, const FiltArrays& x1, const
   // bands: U2 is lower triangular; Z1 too, except a leaf element's [Zr | 0]
   // (nonzero in its first d columns only)
   lq<S, D, D, D, state_only ? 1 : 2, 0, 1 << 20, true, state_only ? 0 : max_obs_dim(D) - 1>(jr, r, s, cx.bc());
+  // output offsets formed here, not hoisted above the factorisation (where
+  // they would hold registers through it)
+  o = opaque_sz(o);
   if (act) {
     if (s == 0) (state_only ? so.m : y.b)[o * D + r] = bb;
     store_dist<S, D>((state_only ? so.L : y.U) + (o * D + r) * D, jr, s);

@@ -98,9 +98,10 @@ __device__ __forceinline__ ModelArgs view(const GroupArgs& g, int j) {
 // Leaf elements.  FIRST: element 0 of each IVP (it carries the prior x0;
 // one work item per IVP); otherwise work item i builds element i + 1.
 template <int D, int d, bool EK1, bool FIRST>
-__global__ __launch_bounds__((Geo<D, kLdsLeaf>::BLOCK)) void k_filter_leaf(GroupArgs g, LaunchMap lm) {
+__global__ __launch_bounds__((Geo<D, FIRST ? kLdsLeaf1 : kLdsLeaf>::BLOCK)) void k_filter_leaf(GroupArgs g,
+                                                                                            LaunchMap lm) {
   extern __shared__ double smem[];
-  auto cx = make_ctx<D, kLdsLeaf>(g.F, g.LQ, g.N1z, smem);
+  auto cx = make_ctx<D, FIRST ? kLdsLeaf1 : kLdsLeaf>(g.F, g.LQ, g.N1z, smem);
   int j, k;
   if (!locate(lm, cx.wid, j, k)) return;
   k = FIRST ? 0 : k + 1;
@@ -929,12 +930,12 @@ hipError_t run_iteration(const GroupArgs& g, const LaunchPlan& p, double* scal, 
   Rec rec(t, st);
   // ---- filter: leaf elements, up-sweep, down-sweep
   if (g.affine) {
-    POF_LAUNCH((k_filter_leaf<D, d, false, true>), kLdsLeaf, p.one_total, st, g, p.map(p.one_off, p.one_total));
+    POF_LAUNCH((k_filter_leaf<D, d, false, true>), kLdsLeaf1, p.one_total, st, g, p.map(p.one_off, p.one_total));
     if (p.leaf1_total > 0)
       POF_LAUNCH((k_filter_leaf<D, d, false, false>), kLdsLeaf, p.leaf1_total, st, g,
                  p.map(p.leaf1_off, p.leaf1_total));
   } else {
-    POF_LAUNCH((k_filter_leaf<D, d, true, true>), kLdsLeaf, p.one_total, st, g, p.map(p.one_off, p.one_total));
+    POF_LAUNCH((k_filter_leaf<D, d, true, true>), kLdsLeaf1, p.one_total, st, g, p.map(p.one_off, p.one_total));
     if (p.leaf1_total > 0)
       POF_LAUNCH((k_filter_leaf<D, d, true, false>), kLdsLeaf, p.leaf1_total, st, g,
                  p.map(p.leaf1_off, p.leaf1_total));
