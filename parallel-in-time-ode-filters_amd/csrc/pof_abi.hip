@@ -144,6 +144,14 @@ static bool field_eval(int kind, int d, const double* p, const double* y, double
 
 // ------------------------------------------------------------- host algebra
 // Householder LQ with LAPACK's sign convention (the device lq() on the host).
+// 1 if the strict upper part of the n x n row-major matrix A is exactly zero
+static bool strictly_upper_zero(const double* A, int n) {
+  for (int r = 0; r < n; ++r)
+    for (int c = r + 1; c < n; ++c)
+      if (A[(size_t)r * n + c] != 0.0) return false;
+  return true;
+}
+
 static void tria_host(const std::vector<double>& M, int nr, int nc, std::vector<double>& L) {
   std::vector<double> a(M);
   const int K = std::min(nr, nc);
@@ -598,6 +606,7 @@ int batch_new(pof_ctx* ctx, int D, int d, long long entries, int slots, bool aff
   g.cx = nullptr;
   g.q = b->q;
   g.affine = affine ? 1 : 0;
+  g.lq_lower = 0;
   b->hPd.assign(S * D, 0.0);
   b->hx0m.assign(S * D, 0.0);
   b->hx0L.assign(S * DD, 0.0);
@@ -623,6 +632,7 @@ int batch_new(pof_ctx* ctx, int D, int d, long long entries, int slots, bool aff
     tria_host(M, D, 2 * D, N1);
     he = hipMemcpy(b->F, Fm.data(), DD * 8, hipMemcpyHostToDevice);
     if (he == hipSuccess) he = hipMemcpy(b->LQ, LQm.data(), DD * 8, hipMemcpyHostToDevice);
+    b->g.lq_lower = strictly_upper_zero(LQm.data(), D) ? 1 : 0;
     if (he == hipSuccess) he = hipMemcpy(b->N1z, N1.data(), DD * 8, hipMemcpyHostToDevice);
     if (he != hipSuccess) {
       batch_free(b);
@@ -1456,6 +1466,25 @@ int pof_ieks_step(pof_ctx* ctx, const pof_problem* prob, const double* states_me
   return POF_OK;
 }
 
+namespace {
+// A caller's transition model (F, QL) for an affine batch: F, QL, N1 = tria([0 | QL]),
+// and whether QL is lower triangular (its strict upper part exactly zero), which
+// lets the smoother-element factorisation skip QL's upper columns.
+hipError_t upload_transition(pof_batch* b, const double* F_host, const double* QL_host) {
+  const int D = b->D;
+  const size_t DD = (size_t)D * D;
+  std::vector<double> M(DD * 2, 0.0), N1z;
+  for (int r = 0; r < D; ++r)
+    for (int c = 0; c < D; ++c) M[(size_t)r * 2 * D + D + c] = QL_host[(size_t)r * D + c];
+  pof::tria_host(M, D, 2 * D, N1z);
+  b->g.lq_lower = pof::strictly_upper_zero(QL_host, D) ? 1 : 0;
+  hipError_t e = hipMemcpy(b->F, F_host, DD * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(b->LQ, QL_host, DD * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(b->N1z, N1z.data(), DD * 8, hipMemcpyHostToDevice);
+  return e;
+}
+}  // namespace
+
 int pof_smoothing(pof_ctx* ctx, int D, int64_t Ns, const double* F_host, const double* QL_host,
                   const double* filt_mean_dev, const double* filt_chol_dev, int sequential, double* smooth_mean_dev,
                   double* smooth_chol_dev, double* obj, void* stream) {
@@ -1478,13 +1507,7 @@ int pof_smoothing(pof_ctx* ctx, int D, int64_t Ns, const double* F_host, const d
   } guard{b};
   if (stream) b->st = (hipStream_t)stream;
   const size_t DD = (size_t)D * D;
-  std::vector<double> M(DD * 2, 0.0), N1z;
-  for (int r = 0; r < D; ++r)
-    for (int c = 0; c < D; ++c) M[(size_t)r * 2 * D + D + c] = QL_host[(size_t)r * D + c];
-  pof::tria_host(M, D, 2 * D, N1z);
-  HIP_TRY(hipMemcpy(b->F, F_host, DD * 8, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(b->LQ, QL_host, DD * 8, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(b->N1z, N1z.data(), DD * 8, hipMemcpyHostToDevice));
+  HIP_TRY(upload_transition(b, F_host, QL_host));
   // state 0 enters as the prior of the slot, states 1..Ns as the level-0 filter prefixes
   std::vector<double> m0(D), L0(DD);
   HIP_TRY(hipMemcpy(m0.data(), filt_mean_dev, D * 8, hipMemcpyDeviceToHost));
@@ -1530,13 +1553,7 @@ int pof_ekf(pof_ctx* ctx, const pof_problem* prob, const double* x0_mean_host, c
   if (stream) b->st = (hipStream_t)stream;
   const int D = s.D;
   const size_t DD = (size_t)D * D;
-  std::vector<double> M(DD * 2, 0.0), N1z;
-  for (int r = 0; r < D; ++r)
-    for (int c = 0; c < D; ++c) M[(size_t)r * 2 * D + D + c] = QL_host[(size_t)r * D + c];
-  pof::tria_host(M, D, 2 * D, N1z);
-  HIP_TRY(hipMemcpy(b->F, F_host, DD * 8, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(b->LQ, QL_host, DD * 8, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(b->N1z, N1z.data(), DD * 8, hipMemcpyHostToDevice));
+  HIP_TRY(upload_transition(b, F_host, QL_host));
   pof_opts o{};
   o.maxiters = 1;
   o.fixed_iters = 1;
@@ -1577,13 +1594,7 @@ int pof_linear_filtsmooth(pof_ctx* ctx, int D, int d, int64_t Ns, const double* 
   if (stream) b->st = (hipStream_t)stream;
   // the caller's model: F, QL, N1 = tria([0 | QL]), x0; no preconditioning (P = 1)
   const size_t DD = (size_t)D * D;
-  std::vector<double> M(DD * 2, 0.0), N1z;
-  for (int r = 0; r < D; ++r)
-    for (int c = 0; c < D; ++c) M[(size_t)r * 2 * D + D + c] = QL_host[(size_t)r * D + c];
-  pof::tria_host(M, D, 2 * D, N1z);
-  HIP_TRY(hipMemcpy(b->F, F_host, DD * 8, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(b->LQ, QL_host, DD * 8, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(b->N1z, N1z.data(), DD * 8, hipMemcpyHostToDevice));
+  HIP_TRY(upload_transition(b, F_host, QL_host));
   b->g.Hx = H_dev;
   b->g.cx = c_dev;
   b->g.Rx = R_dev;

@@ -274,7 +274,13 @@ __global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK)) void k_filter_post(Group
       ph[NL + t] = act ? lq_r[t] : 0.0;
     }
   }
-  lq<S, 2 * D, D, D, 1, 0, D, false>(ph, r, sb, cx.bc());
+  // second block [LQ; 0]: banded when LQ is lower triangular (the IWP prior's
+  // Cholesky factor; a caller's model is checked on the host)
+  if (g.lq_lower) {
+    lq<S, 2 * D, D, D, 1, 0, D, false, 0>(ph, r, sb, cx.bc());
+  } else {
+    lq<S, 2 * D, D, D, 1, 0, D, false>(ph, r, sb, cx.bc());
+  }
   if (wc) {
     lq<S, 2 * D, D, D, 1, D>(ph, r, sb, cx.bc());  // Phi22
   } else {

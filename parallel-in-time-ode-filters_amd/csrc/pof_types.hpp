@@ -101,6 +101,7 @@ struct GroupArgs {
   const double* Rx;           // [Ns, d, d] or null (noiseless)
   int q;
   int affine;
+  int lq_lower;  // 1: LQ's strict upper part is zero (the smoother-element factorisation skips it)
 };
 
 // Work list of one launch: active IVP a (id ivp[a]) owns the work items

@@ -106,8 +106,10 @@ def test_unscented(gpu, name, q, kappa):
         assert rel(gram(out.cholR[k].cpu().numpy()), gram(r.cholR)) < 1e-8
 
 
-@pytest.mark.parametrize("sequential", [False, True])
-def test_linear_filtsmooth_with_noise(gpu, sequential):
+@pytest.mark.parametrize("sequential,rotate_ql", [(False, False), (True, False), (False, True)])
+def test_linear_filtsmooth_with_noise(gpu, sequential, rotate_ql):
+    """rotate_ql: QL times an orthogonal matrix (same Q, no longer lower triangular), which the
+    smoother-element factorisation must take without its banded shortcut."""
     import torch
 
     import pof.parallel_filtsmooth as pfs
@@ -123,6 +125,10 @@ def test_linear_filtsmooth_with_noise(gpu, sequential):
     d, q, Ns = 2, 2, 41
     D = d * (q + 1)
     F, QL = om.preconditioned_discretize(om.IWP(d, q))
+    if rotate_ql:
+        rot, _ = np.linalg.qr(rng.standard_normal((D, D)))
+        QL = QL @ rot
+        assert np.abs(np.triu(QL, 1)).max() > 1e-3
     H = rng.standard_normal((Ns, d, D))
     c = rng.standard_normal((Ns, d))
     R = np.tril(rng.standard_normal((Ns, d, d))) * 0.5
