@@ -283,7 +283,17 @@ def batched_run(n_ivps, maxiters, plain_sample, oracle_sample, seed=4):
     }
 
 
-def served_run(n_req, rate, maxiters, seed=5):
+def _progress(label, every):
+    """on_done hook: one stderr line per `every` completed requests."""
+
+    def hook(rec, n_done):
+        if n_done % every == 0:
+            print(f"{label}: {n_done} requests done at {rec.done:.1f} s", file=sys.stderr, flush=True)
+
+    return hook
+
+
+def served_run(n_req, rate, maxiters, seed=5, verbose=False):
     """Poisson request stream served with continuous batching vs a FIFO plain per-request scan."""
     import numpy as np
     import torch
@@ -297,7 +307,7 @@ def served_run(n_req, rate, maxiters, seed=5):
     srv = pof.serve.Server(budget_bytes=96 << 30, maxiters=maxiters)
     srv.run(reqs[: min(24, n_req)], np.zeros(min(24, n_req)))  # warm-up: allocate the groups
     t0 = time.perf_counter()
-    recs = srv.run(reqs, arr)
+    recs = srv.run(reqs, arr, on_done=_progress("served", 500) if verbose else None)
     wall = time.perf_counter() - t0
     st = pof.serve.latency_stats(recs)
 
@@ -307,7 +317,7 @@ def served_run(n_req, rate, maxiters, seed=5):
         return ys.mean.cpu().numpy()
 
     t0 = time.perf_counter()
-    precs = pof.serve.fifo_plain(reqs, arr, solve_one)
+    precs = pof.serve.fifo_plain(reqs, arr, solve_one, on_done=_progress("fifo plain", 500) if verbose else None)
     pwall = time.perf_counter() - t0
     pst = pof.serve.latency_stats(precs)
     same = all(np.array_equal(a.means, b.means) for a, b in zip(recs, precs))
@@ -447,7 +457,8 @@ def main():
             print(json.dumps(batched), file=sys.stderr, flush=True)
         # mean Ns of log-uniform [256, 65536] over [256, 16384]: 3.04x more work per request
         rate = args.serve_rate or 0.7 * batched["solves_per_s"] / 3.04
-        served = served_run(args.serve_requests, rate, maxiters=100) if args.serve_requests > 0 else None
+        served = (served_run(args.serve_requests, rate, maxiters=100, verbose=args.verbose)
+                  if args.serve_requests > 0 else None)
         if args.verbose:
             print(json.dumps(served), file=sys.stderr, flush=True)
     precision = None if args.no_precision else precision_run(args.precision_ns)

@@ -1,6 +1,8 @@
 """Summaries of rocprofv3 output for the IEKS kernels.
 
 * ``stats``: the ``--kernel-trace --stats`` per-kernel table as Markdown.
+* ``mfma``: MFMA busy fraction and FP64 MFMA flops per kernel class from one
+  ``--pmc MfmaUtil MfmaFlopsF64 GRBM_GUI_ACTIVE`` run.
 * ``traffic``: HBM bytes per launch of every kernel from two counter runs
   (``--pmc FETCH_SIZE`` and ``--pmc WRITE_SIZE``, KiB per dispatch), each
   scaled by what the same counter reads for a plain device copy of known
@@ -103,6 +105,65 @@ def stats_markdown(path, command=""):
     return "\n".join(out) + "\n"
 
 
+def kernel_class(name):
+    """rocBLAS / Tensile GEMM kernels (``Cijk_*``) form one class; other kernels by short name."""
+    k = short_name(name)
+    return "rocBLAS DGEMM (Cijk_*, FP64 MFMA)" if k.startswith("Cijk_") else k
+
+
+def mfma_rows(pmc_csv, stats_csv=None):
+    """Per kernel class: dispatches, duration-weighted MFMA busy fraction and FP64 MFMA flops.
+
+    ``pmc_csv`` is a counter_collection.csv of ``--pmc MfmaUtil MfmaFlopsF64 GRBM_GUI_ACTIVE``
+    (rocprofv3's derived counters: MFMA busy cycles over active cycles x SIMDs, in percent;
+    512 flops per F64 MFMA op).  Each dispatch's MfmaUtil is weighted by its GRBM_GUI_ACTIVE
+    cycles.  With ``stats_csv`` (kernel stats of the same command) the flops are also given as
+    TFLOP/s over the class's kernel time.
+    """
+    per = collections.defaultdict(dict)  # dispatch -> counters
+    names = {}
+    with open(pmc_csv) as fh:
+        for r in csv.DictReader(fh):
+            did = r.get("Dispatch_Id") or r.get("Correlation_Id")
+            per[did][r["Counter_Name"]] = float(r["Counter_Value"])
+            names[did] = r["Kernel_Name"]
+    agg = collections.defaultdict(lambda: [0, 0.0, 0.0, 0.0])  # n, sum util*gui, sum gui, flops
+    for did, c in per.items():
+        a = agg[kernel_class(names[did])]
+        g = c.get("GRBM_GUI_ACTIVE", 0.0)
+        a[0] += 1
+        a[1] += c.get("MfmaUtil", 0.0) * g
+        a[2] += g
+        a[3] += c.get("MfmaFlopsF64", 0.0)
+    ms = {}
+    if stats_csv:
+        with open(stats_csv) as fh:
+            for r in csv.DictReader(fh):
+                k = kernel_class(r["Name"])
+                ms[k] = ms.get(k, 0.0) + float(r["TotalDurationNs"]) / 1e6
+    rows = []
+    for k, (n, uw, g, fl) in sorted(agg.items(), key=lambda x: -x[1][2]):
+        rows.append((k, n, uw / g if g else 0.0, g, fl, ms.get(k)))
+    tot_g = sum(r[3] for r in rows)
+    overall = sum(r[2] * r[3] for r in rows) / tot_g if tot_g else 0.0
+    return rows, overall
+
+
+def mfma_markdown(pmc_csv, stats_csv=None, command=""):
+    rows, overall = mfma_rows(pmc_csv, stats_csv)
+    out = ["# MFMA utilisation (rocprofv3 --pmc MfmaUtil MfmaFlopsF64 GRBM_GUI_ACTIVE)", ""]
+    if command:
+        out += ["Commands:", "", "```", command, "```", ""]
+    out += [f"Whole run, weighted by active cycles: MFMA busy {overall:.2f}% of SIMD cycles.", ""]
+    out += ["| kernel class | dispatches | MFMA busy % | active cycles share % | FP64 MFMA GFLOP | TFLOP/s over kernel time |",
+            "|---|---|---|---|---|---|"]
+    tot_g = sum(r[3] for r in rows) or 1.0
+    for k, n, u, g, fl, ms in rows[:20]:
+        tf = f"{fl / (ms * 1e-3) / 1e12:.2f}" if ms else ""
+        out.append(f"| `{k[:80]}` | {n} | {u:.2f} | {100 * g / tot_g:.2f} | {fl / 1e9:.2f} | {tf} |")
+    return "\n".join(out) + "\n"
+
+
 def read_traffic(md_path, kernel):
     """Bytes per launch of the first kernel whose name contains ``kernel``, or None."""
     try:
@@ -129,9 +190,15 @@ def main(argv=None):
     t.add_argument("--copy-write", required=True)
     t.add_argument("--copy-bytes", type=float, default=float(8 << 25))
     t.add_argument("--command", default="")
+    m = sub.add_parser("mfma")
+    m.add_argument("csv")
+    m.add_argument("--stats", default=None)
+    m.add_argument("--command", default="")
     a = ap.parse_args(argv)
     if a.cmd == "stats":
         sys.stdout.write(stats_markdown(a.csv, a.command))
+    elif a.cmd == "mfma":
+        sys.stdout.write(mfma_markdown(a.csv, a.stats, a.command))
     else:
         sys.stdout.write(traffic_markdown(a.fetch, a.write, a.copy_fetch, a.copy_write, a.copy_bytes, a.command))
 

@@ -70,10 +70,11 @@ class Server:
             self.groups[key] = b
         return b
 
-    def run(self, requests, arrivals, clock=time.perf_counter):
+    def run(self, requests, arrivals, clock=time.perf_counter, on_done=None):
         """Serve ``requests[i]`` (``(f, y0, ts, order, init)``) arriving at ``arrivals[i]`` seconds.
 
         Returns one :class:`Record` per request, with times in seconds from the start.
+        ``on_done(record, n_done)`` is called as each request completes.
         """
         n = len(requests)
         recs = [Record(i, float(arrivals[i])) for i in range(n)]
@@ -81,6 +82,7 @@ class Server:
         nxt = 0
         waiting = collections.deque()
         inflight = {}  # (group key, ticket) -> rid
+        ndone = 0
         t0 = clock()
         while nxt < n or waiting or inflight:
             now = clock() - t0
@@ -133,6 +135,9 @@ class Server:
                     r.status = info["status"]
                     if self.keep_means:
                         r.means = m
+                    ndone += 1
+                    if on_done is not None:
+                        on_done(r, ndone)
             if not progressed:
                 time.sleep(20e-6)
         return recs
@@ -154,17 +159,18 @@ def latency_stats(recs):
     }
 
 
-def fifo_plain(requests, arrivals, solve_one, clock=time.perf_counter):
+def fifo_plain(requests, arrivals, solve_one, clock=time.perf_counter, on_done=None):
     """The plain baseline: one request at a time, in arrival order, each solved alone.
 
     ``solve_one(request)`` runs one solve and returns its host means.  Returns
     the records (arrival, done) of a single-server FIFO queue run in real time.
+    ``on_done(record, n_done)`` is called as each request completes.
     """
     n = len(requests)
     recs = [Record(i, float(arrivals[i])) for i in range(n)]
     order = np.argsort(arrivals, kind="stable")
     t0 = clock()
-    for rid in order:
+    for k, rid in enumerate(order):
         r = recs[int(rid)]
         now = clock() - t0
         if r.arrival > now:
@@ -172,4 +178,6 @@ def fifo_plain(requests, arrivals, solve_one, clock=time.perf_counter):
         r.admitted = clock() - t0
         r.means = solve_one(requests[int(rid)])
         r.done = clock() - t0
+        if on_done is not None:
+            on_done(r, k + 1)
     return recs
