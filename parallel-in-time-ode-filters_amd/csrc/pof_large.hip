@@ -109,9 +109,16 @@ constexpr int kNB = 16;  // rows per panel of the blocked LQ
 template <int NQ>
 __global__ __launch_bounds__(64) void k_lq_panel(double* A, long long sA, int n, int ld, int i0, int np, double* Vb,
                                                  long long sV, int ldV, double* Tb, long long sT, int ldT) {
+  // Cross-lane sums go through LDS in batches: each lane writes its partial sum
+  // of every value of a step (red[value][lane]), then lanes 4v .. 4v + 3 add up
+  // value v (16 partials each, then two butterfly steps) and publish it in tot.
+  __shared__ double red[kNB][65];
+  __shared__ double tot[kNB];
+  __shared__ double colk[kNB];  // column k of the panel rows (held by lane k, slot 0)
   __shared__ double G[kNB][kNB];
   __shared__ double taus[kNB];
   const int lane = threadIdx.x;
+  const int rv = lane >> 2, rs = lane & 3;  // value and quarter this lane reduces
   double* a = A + (long long)blockIdx.x * sA + (size_t)i0 * ld + i0;
   const int nc = n - i0;
   double P[kNB][NQ];
@@ -123,19 +130,49 @@ __global__ __launch_bounds__(64) void k_lq_panel(double* A, long long sA, int n,
       P[r][q] = (r < np && c < nc) ? a[(size_t)r * ld + c] : 0.0;
     }
   }
+  auto reduce = [&]() {
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int t = 0; t < 16; ++t) s += red[rv][rs * 16 + t];
+    s += __shfl_xor(s, 1, 64);
+    s += __shfl_xor(s, 2, 64);
+    if (rs == 0) tot[rv] = s;
+    __syncthreads();
+  };
   double betas[kNB];
 #pragma unroll
   for (int k = 0; k < kNB; ++k) {
     betas[k] = 0.0;
     if (k < np) {
-      double sum = 0.0;
+      __syncthreads();  // the previous step has read red, tot and colk
+      // value 0: the squared tail of pivot row k; value r > k: row r . tail of row k
+      {
+        double sq = 0.0;
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) {
-        const int c = lane + 64 * q;
-        sum = (c > k && c < nc) ? fma(P[k][q], P[k][q], sum) : sum;
+        for (int q = 0; q < NQ; ++q) {
+          const int c = lane + 64 * q;
+          sq = (c > k && c < nc) ? fma(P[k][q], P[k][q], sq) : sq;
+        }
+        red[0][lane] = sq;
       }
-      const double sig = wave_sum(sum);
-      const double alpha = __shfl(P[k][0], k, 64);
+#pragma unroll
+      for (int r = k + 1; r < kNB; ++r) {
+        double w = 0.0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          const int c = lane + 64 * q;
+          w = (c > k && c < nc) ? fma(P[r][q], P[k][q], w) : w;
+        }
+        red[r][lane] = w;
+      }
+      if (lane == k) {
+#pragma unroll
+        for (int r = 0; r < kNB; ++r) colk[r] = P[r][0];
+      }
+      reduce();
+      const double sig = tot[0];
+      const double alpha = colk[k];
       double tau = 0.0, beta = alpha, scal = 0.0;
       if (sig != 0.0) {
         const double nrm = sqrt(fma(alpha, alpha, sig));
@@ -154,14 +191,8 @@ __global__ __launch_bounds__(64) void k_lq_panel(double* A, long long sA, int n,
 #pragma unroll
         for (int r = k + 1; r < kNB; ++r) {
           if (r < np) {
-            double w = 0.0;
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-              const int c = lane + 64 * q;
-              w = (c > k && c < nc) ? fma(P[r][q], P[k][q], w) : w;
-            }
-            w = wave_sum(w) + __shfl(P[r][0], k, 64);
-            const double tw = tau * w;
+            // row r . v with v = [1, scal * tail]
+            const double tw = tau * fma(scal, tot[r], colk[r]);
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
               const int c = lane + 64 * q;
@@ -188,21 +219,27 @@ __global__ __launch_bounds__(64) void k_lq_panel(double* A, long long sA, int n,
       }
     }
   }
-  // G[l][k] = v_l . v_k (l < k) = v_l[k] + sum_{c > k} v_l[c] v_k[c]
+  // G[l][k] = v_l . v_k (l < k) = v_l[k] + sum_{c > k} v_l[c] v_k[c], one batch per k
 #pragma unroll
   for (int k = 1; k < kNB; ++k) {
+    if (k < np) {
+      __syncthreads();
 #pragma unroll
-    for (int l = 0; l < k; ++l) {
-      if (k < np) {
+      for (int l = 0; l < k; ++l) {
         double w = 0.0;
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
           const int c = lane + 64 * q;
           w = (c > k && c < nc) ? fma(P[l][q], P[k][q], w) : w;
         }
-        w = wave_sum(w) + __shfl(P[l][0], k, 64);
-        if (lane == 0) G[l][k] = w;
+        red[l][lane] = w;
       }
+      if (lane == k) {
+#pragma unroll
+        for (int l = 0; l < kNB; ++l) colk[l] = P[l][0];
+      }
+      reduce();
+      if (lane < k) G[lane][k] = tot[lane] + colk[lane];
     }
   }
   __syncthreads();
