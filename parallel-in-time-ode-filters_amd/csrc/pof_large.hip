@@ -337,6 +337,29 @@ __global__ void k_copy(double* dst, long long ds, int dld, const double* src, lo
   dst[(long long)b * ds + (size_t)r * dld + c] = alpha * x;
 }
 
+// Transposed copy through a 32 x 32 LDS tile: dst item b (rows x cols) =
+// alpha * src item b' (c, r), both sides read and written along rows.
+// grid (ceil(cols / 32), ceil(rows / 32), cnt), 256 threads (32 x 8).
+__global__ __launch_bounds__(256) void k_copy_t(double* dst, long long ds, int dld, const double* src, long long ss,
+                                                int sld, int rows, int cols, int cnt, int rev, double alpha) {
+  __shared__ double t[32][33];
+  const int b = blockIdx.z;
+  if (b >= cnt) return;
+  const long long bs = rev ? (long long)(cnt - 1 - b) : (long long)b;
+  const int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  // t[k][x] = src (c0 + k, r0 + x) = dst (r0 + x, c0 + k)
+  for (int k = ty; k < 32; k += 8) {
+    const int sr = c0 + k, sc = r0 + tx;
+    t[k][tx] = (sr < cols && sc < rows) ? src[bs * ss + (size_t)sr * sld + sc] : 0.0;
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const int dr = r0 + k, dc = c0 + tx;
+    if (dr < rows && dc < cols) dst[(long long)b * ds + (size_t)dr * dld + dc] = alpha * t[tx][k];
+  }
+}
+
 // dst item (rows x cols) = dv on the diagonal, 0 elsewhere
 __global__ void k_fill(double* dst, long long ds, int dld, int rows, int cols, int cnt, double dv) {
   const int b = blockIdx.y;
@@ -579,6 +602,12 @@ struct Ops {
   }
   void copy(V dst, V src, int rows, int cols, int cnt, bool trans = false, bool rev = false, double alpha = 1.0) {
     if (err || cnt <= 0) return;
+    if (trans && rows > 1 && cols > 1) {
+      hipLaunchKernelGGL(k_copy_t, dim3((cols + 31) / 32, (rows + 31) / 32, cnt), dim3(256), 0, st, dst.p, dst.s,
+                         dst.ld, src.p, src.s, src.ld, rows, cols, cnt, rev ? 1 : 0, alpha);
+      check("k_copy_t");
+      return;
+    }
     const int e = rows * cols;
     hipLaunchKernelGGL(k_copy, dim3((e + 255) / 256, cnt), dim3(256), 0, st, dst.p, dst.s, dst.ld, src.p, src.s,
                        src.ld, rows, cols, cnt, trans ? 1 : 0, rev ? 1 : 0, alpha);
@@ -655,10 +684,10 @@ void filter_fn(Ops& o, Arena& ar, int D, const FE& e1, const FE& e2, const FE& o
   const long long DD = (long long)D * D;
   V X{ar.take((size_t)cnt * 4 * DD), 4 * DD, 2 * D};
   V Y{ar.take((size_t)cnt * 2 * DD), 2 * DD, 2 * D};
-  V T1{ar.take((size_t)cnt * DD), DD, D}, T2{ar.take((size_t)cnt * DD), DD, D};
+  V T2{ar.take((size_t)cnt * DD), DD, D};
   V T3{ar.take((size_t)cnt * DD), DD, D}, T4{ar.take((size_t)cnt * DD), DD, D};
   V v1{ar.take((size_t)cnt * D), D, 1}, v2{ar.take((size_t)cnt * D), D, 1}, v3{ar.take((size_t)cnt * D), D, 1};
-  if (!X.p || !Y.p || !T1.p || !T2.p || !T3.p || !T4.p || !v1.p || !v2.p || !v3.p) {
+  if (!X.p || !Y.p || !T2.p || !T3.p || !T4.p || !v1.p || !v2.p || !v3.p) {
     o.fail("workspace", -2);
     return;
   }
@@ -669,14 +698,15 @@ void filter_fn(Ops& o, Arena& ar, int D, const FE& e1, const FE& e2, const FE& o
   o.fill(X.at(D, D), D, D, cnt, 0.0);
   o.lq(X, 2 * D, 2 * D, cnt);
   const V Xi11 = X.at(0, 0), Xi21 = X.at(D, 0), Xi22 = X.at(D, D);
-  // m_ = Xi11^-1 U1^T -> T3; M = Xi11^-1 (U1^T A2^T) = m_ A2^T -> T1
+  // m_ = Xi11^-1 U1^T -> T3; M^T = (m_ A2^T)^T = A2 m_^T, formed in place as the left
+  // block of [M^T | U2] (Y)
   o.copy(T3, e1.U, D, D, cnt, true);
   o.trsm(false, D, D, Xi11, T3, cnt);
-  o.gemm(false, true, D, D, D, 1.0, T3, e2.A, 0.0, T1, cnt);
+  o.gemm(false, true, D, D, D, 1.0, e2.A, T3, 0.0, Y.at(0, 0), cnt);
   // A = A2 A1 - M^T (Xi21^T A1)
   o.gemm(true, false, D, D, D, 1.0, Xi21, e1.A, 0.0, T2, cnt);
   o.gemm(false, false, D, D, D, 1.0, e2.A, e1.A, 0.0, out.A, cnt);
-  o.gemm(true, false, D, D, D, -1.0, T1, T2, 1.0, out.A, cnt);
+  o.gemm(false, false, D, D, D, -1.0, Y.at(0, 0), T2, 1.0, out.A, cnt);
   // b = A2 (v2 - m_^T Xi21^T v2) + b2, v2 = b1 + U1 (U1^T eta2)
   o.gemm(true, false, D, 1, D, 1.0, e1.U, e2.eta, 0.0, v1, cnt);
   o.copy(v2, e1.b, D, 1, cnt);
@@ -686,7 +716,6 @@ void filter_fn(Ops& o, Arena& ar, int D, const FE& e1, const FE& e2, const FE& o
   o.copy(out.b, e2.b, D, 1, cnt);
   o.gemm(false, false, D, 1, D, 1.0, e2.A, v2, 1.0, out.b, cnt);
   // U = tria([M^T | U2])
-  o.copy(Y.at(0, 0), T1, D, D, cnt, true);
   o.copy(Y.at(0, D), e2.U, D, D, cnt);
   o.lq(Y, D, 2 * D, cnt);
   o.copy(out.U, Y.at(0, 0), D, D, cnt);
@@ -833,6 +862,10 @@ int solve(Engine* e, const Problem& p, const pof_opts& opts, pof_out* out, const
   const int D = p.D, d = p.d, q1 = p.q + 1;
   const long long Ns = p.Ns;
   const long long DD = (long long)D * D;
+  if (Ns > 65534) {  // batched launches put the item count in grid y / z
+    err = "large engine supports at most 65534 steps";
+    return POF_E_ARG;
+  }
   if (Ns < 1) {
     err = "large engine needs Ns >= 1";
     return POF_E_ARG;
