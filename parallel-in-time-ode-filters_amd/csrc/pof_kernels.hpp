@@ -131,6 +131,10 @@ template <int D>
 constexpr int up_min_waves() {
   return D == 12 ? 4 : 1;
 }
+template <int D>
+constexpr int down_min_waves() {
+  return D == 15 ? 4 : 1;
+}
 
 // Up-sweep: out[i] = in[2i] o in[2i+1]
 template <int D>
@@ -151,7 +155,8 @@ __global__ __launch_bounds__((Geo<D, kLdsFull>::BLOCK), up_min_waves<D>()) void 
 //   p[0] = state(e[0]); p[2i+1] = up[i]; p[2i] = up[i-1] o e[2i]  (i >= 1)
 // With top != 0 (a level of one element) only p[0] = state(e[0]) is written.
 template <int D>
-__global__ __launch_bounds__((Geo<D, kLdsState>::BLOCK)) void k_filter_down(GroupArgs g, LaunchMap lm, int l) {
+__global__ __launch_bounds__((Geo<D, kLdsState>::BLOCK), down_min_waves<D>()) void k_filter_down(GroupArgs g,
+                                                                                               LaunchMap lm, int l) {
   extern __shared__ double smem[];
   auto cx = make_ctx<D, kLdsState>(g.F, g.LQ, g.N1z, smem);
   int j, i;
