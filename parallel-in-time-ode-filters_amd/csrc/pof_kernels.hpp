@@ -95,10 +95,24 @@ __device__ __forceinline__ ModelArgs view(const GroupArgs& g, int j) {
 }
 
 // ------------------------------------------------------------------ kernels
+// Launch bounds of the element kernels on the IEKS path (EK1), where LDS
+// allows more waves per SIMD than their natural register allocation: the
+// bound trades a small spill for occupancy.  Measured to pay at D = 12 only
+// (leaf 3 -> 4, post 2 -> 3 waves, +0.9% on Lorenz-63 Ns = 65536); the same
+// bounds on leaf<15> and post<10> ran slower.
+template <int D, bool EK1>
+constexpr int leaf_min_waves(bool first) {
+  return (first || !EK1) ? 1 : (D == 12) ? 4 : 1;
+}
+template <int D, bool EK1>
+constexpr int post_min_waves() {
+  return !EK1 ? 1 : (D == 12) ? 3 : 1;
+}
+
 // Leaf elements.  FIRST: element 0 of each IVP (it carries the prior x0;
 // one work item per IVP); otherwise work item i builds element i + 1.
 template <int D, int d, bool EK1, bool FIRST>
-__global__ __launch_bounds__((Geo<D, FIRST ? kLdsLeaf1 : kLdsLeaf>::BLOCK)) void k_filter_leaf(GroupArgs g,
+__global__ __launch_bounds__((Geo<D, FIRST ? kLdsLeaf1 : kLdsLeaf>::BLOCK), (leaf_min_waves<D, EK1>(FIRST))) void k_filter_leaf(GroupArgs g,
                                                                                             LaunchMap lm) {
   extern __shared__ double smem[];
   auto cx = make_ctx<D, FIRST ? kLdsLeaf1 : kLdsLeaf>(g.F, g.LQ, g.N1z, smem);
@@ -201,7 +215,7 @@ __global__ __launch_bounds__((Geo<D, kLdsState>::BLOCK), down_min_waves<D>()) vo
 // smoother elements without observation statistics (smoother on its own).
 enum PostMode : int { kPostFull = 0, kPostDm = 1, kPostNoObs = 2 };
 template <int D, int d, bool EK1>
-__global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK)) void k_filter_post(GroupArgs g, LaunchMap lm, int mode) {
+__global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK), (post_min_waves<D, EK1>())) void k_filter_post(GroupArgs g, LaunchMap lm, int mode) {
   extern __shared__ double smem[];
   auto cx = make_ctx<D, kLdsPost>(g.F, g.LQ, g.N1z, smem);
   using L = Lay<D, decltype(cx)::S>;
