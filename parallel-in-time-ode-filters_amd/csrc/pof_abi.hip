@@ -1161,7 +1161,8 @@ int admit_problem(pof_batch* b, const pof_problem* prob, const pof_opts* opts, p
   Setup s;
   int rc = build_setup(prob, s, ctx->err);
   if (rc != POF_OK) return rc;
-  if (s.D != b->D || s.d != b->d) return set_err(ctx, POF_E_ARG, "problem (D=%d, d=%d) does not match the batch", s.D, s.d);
+  if (s.D != b->D || s.d != b->d)
+    return set_err(ctx, POF_E_ARG, "problem (D=%d, d=%d) does not match the batch", s.D, s.d);
   std::vector<double> m0;
   rc = (prob->init == POF_INIT_COARSE) ? coarse_means(ctx, prob, s, m0) : initial_means(prob, s, m0, ctx->err);
   if (rc != POF_OK) return rc;
@@ -1651,7 +1652,8 @@ int pof_batch_ready(const pof_batch* b) {
   if (!b->launched || !b->need_sync) return 1;
   const hipError_t q = hipEventQuery(b->ev);
   if (q == hipErrorNotReady) return 0;
-  return q == hipSuccess ? 1 : POF_E_HIP;
+  if (q != hipSuccess) return set_err(b->ctx, POF_E_HIP, "hipEventQuery: %s", hipGetErrorString(q));
+  return 1;
 }
 
 int pof_batch_finish(pof_batch* b, int32_t max_done, int64_t* done, int32_t* n_done) {

@@ -112,8 +112,8 @@ constexpr int post_min_waves() {
 // Leaf elements.  FIRST: element 0 of each IVP (it carries the prior x0;
 // one work item per IVP); otherwise work item i builds element i + 1.
 template <int D, int d, bool EK1, bool FIRST>
-__global__ __launch_bounds__((Geo<D, FIRST ? kLdsLeaf1 : kLdsLeaf>::BLOCK), (leaf_min_waves<D, EK1>(FIRST))) void k_filter_leaf(GroupArgs g,
-                                                                                            LaunchMap lm) {
+__global__ __launch_bounds__((Geo<D, FIRST ? kLdsLeaf1 : kLdsLeaf>::BLOCK), (leaf_min_waves<D, EK1>(FIRST))) void
+k_filter_leaf(GroupArgs g, LaunchMap lm) {
   extern __shared__ double smem[];
   auto cx = make_ctx<D, FIRST ? kLdsLeaf1 : kLdsLeaf>(g.F, g.LQ, g.N1z, smem);
   int j, k;
@@ -215,7 +215,9 @@ __global__ __launch_bounds__((Geo<D, kLdsState>::BLOCK), down_min_waves<D>()) vo
 // smoother elements without observation statistics (smoother on its own).
 enum PostMode : int { kPostFull = 0, kPostDm = 1, kPostNoObs = 2 };
 template <int D, int d, bool EK1>
-__global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK), (post_min_waves<D, EK1>())) void k_filter_post(GroupArgs g, LaunchMap lm, int mode) {
+__global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK), (post_min_waves<D, EK1>())) void k_filter_post(GroupArgs g,
+                                                                                                      LaunchMap lm,
+                                                                                                      int mode) {
   extern __shared__ double smem[];
   auto cx = make_ctx<D, kLdsPost>(g.F, g.LQ, g.N1z, smem);
   using L = Lay<D, decltype(cx)::S>;

@@ -155,7 +155,8 @@ def mfma_markdown(pmc_csv, stats_csv=None, command=""):
     if command:
         out += ["Commands:", "", "```", command, "```", ""]
     out += [f"Whole run, weighted by active cycles: MFMA busy {overall:.2f}% of SIMD cycles.", ""]
-    out += ["| kernel class | dispatches | MFMA busy % | active cycles share % | FP64 MFMA GFLOP | TFLOP/s over kernel time |",
+    out += ["| kernel class | dispatches | MFMA busy % | active cycles share % | FP64 MFMA GFLOP "
+            "| TFLOP/s over kernel time |",
             "|---|---|---|---|---|---|"]
     tot_g = sum(r[3] for r in rows) or 1.0
     for k, n, u, g, fl, ms in rows[:20]:
