@@ -152,3 +152,63 @@ def test_batch_full_is_reported(gpu):
     while b.active:
         b.step()
     b.close()
+
+
+def test_ready_polling_matches_step(gpu):
+    """launch(), polling ready() (the event query of pof_batch_ready) and finish() give the
+    results of step(), which waits."""
+    import pof.batch
+    import pof.ivp
+
+    rng = np.random.default_rng(21)
+    d, q = 2, 3
+    probs = []
+    for _ in range(4):
+        ivp = pof.ivp.fhn_classic(a=float(rng.uniform(0.1, 0.3)), b=float(rng.uniform(0.1, 0.3)),
+                                  c=float(rng.uniform(2, 4)))
+        ts = np.linspace(0.0, 20.0, int(rng.integers(50, 400)))
+        probs.append((ivp.f, ivp.y0 + 0.1 * rng.standard_normal(2), ts, q, "constant"))
+    res = []
+    for poll in (False, True):
+        b = pof.batch.Batch(d=d, order=q, entries=20000, slots=8)
+        tick = {b.admit(f, y0, ts, init=init, maxiters=60): i for i, (f, y0, ts, _, init) in enumerate(probs)}
+        out = {}
+        while b.active:
+            if poll:
+                b.launch()
+                spins = 0
+                while not b.ready():
+                    spins += 1
+                    assert spins < 10_000_000
+                done = b.finish()
+            else:
+                done = b.step()
+            for t, r in done:
+                out[tick[t]] = r
+        b.close()
+        res.append(out)
+    assert set(res[0]) == set(res[1]) == set(range(len(probs)))
+    for i in range(len(probs)):
+        assert res[0][i][1]["iterations"] == res[1][i][1]["iterations"]
+        assert _equal(res[0][i][0].mean, res[1][i][0].mean), i
+
+
+def test_server_results_same_as_alone(gpu):
+    """The request loop (pof/serve.py): a stream of mixed requests, admitted while others
+    iterate; each result equals the same solve alone, on_done reports every completion."""
+    import pof.serve
+    import pof.solver
+
+    reqs = _mix(11, 16, nmax=200)
+    arrivals = np.linspace(0.0, 0.05, len(reqs))
+    srv = pof.serve.Server(budget_bytes=2 << 30, n_groups=4, slots=8, maxiters=60)
+    seen = []
+    recs = srv.run(reqs, arrivals, on_done=lambda r, n: seen.append((r.rid, n)))
+    srv.close()
+    assert sorted(n for _, n in seen) == list(range(1, len(reqs) + 1))
+    assert sorted(rid for rid, _ in seen) == list(range(len(reqs)))
+    for r, (f, y0, ts, q, init) in zip(recs, reqs):
+        ys, info = pof.solver.solve(f=f, y0=y0, ts=ts, order=q, init=init, maxiters=60)
+        assert r.iterations == info["iterations"], r.rid
+        assert np.array_equal(r.means, ys.mean.cpu().numpy()), r.rid
+        assert r.done >= r.admitted >= r.arrival
