@@ -143,11 +143,11 @@ k_filter_leaf(GroupArgs g, LaunchMap lm) {
 // from 3 waves; elsewhere the natural allocation is kept.
 template <int D>
 constexpr int up_min_waves() {
-  return D == 12 ? 4 : 1;
+  return D == 12 ? 4 : (D == 10) ? 5 : 1;
 }
 template <int D>
 constexpr int down_min_waves() {
-  return D == 15 ? 4 : 1;
+  return D == 15 ? 4 : (D == 10) ? 5 : 1;
 }
 
 // Up-sweep: out[i] = in[2i] o in[2i+1]
