@@ -79,7 +79,16 @@ typedef struct {
   int32_t fixed_iters; /* > 0: run exactly this many iterations, no convergence test */
   int32_t calibrate;   /* 1: returned factors scaled by sqrt(ssq) twice (solver.py:66-68 + step.py:42-44) */
   int32_t sequential;  /* 1: sequential filter + smoother per iteration (solve(sequential=True), step.py:39-40) */
+  int32_t products;    /* POF_PRODUCTS_F64 (default) or POF_PRODUCTS_F32 (large-state engine only) */
 } pof_opts;
+
+/* Precision of the scan combines' matrix products.  No counterpart in the
+ * reference (JAX runs one dtype); F32: the large-state engine forms the filter
+ * and smoother combines' matrix-matrix products with FP32 MFMA GEMMs (FP32
+ * accumulate), while every triangularisation, triangular solve, element and
+ * observation statistic stays FP64. */
+#define POF_PRODUCTS_F64 0
+#define POF_PRODUCTS_F32 1
 
 typedef struct {
   /* caller-owned device outputs (NULL = not wanted) */

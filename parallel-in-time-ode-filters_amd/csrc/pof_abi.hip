@@ -1158,6 +1158,8 @@ int coarse_means(pof_ctx* ctx, const pof_problem* prob, const Setup& fine, std::
 
 int admit_problem(pof_batch* b, const pof_problem* prob, const pof_opts* opts, pof_out* out, long long* ticket) {
   pof_ctx* ctx = b->ctx;
+  if (opts->products != POF_PRODUCTS_F64)
+    return set_err(ctx, POF_E_UNSUPPORTED, "FP32 products exist in the large-state engine only (pof_solve_large)");
   Setup s;
   int rc = build_setup(prob, s, ctx->err);
   if (rc != POF_OK) return rc;
@@ -1359,6 +1361,8 @@ static int solve_large(pof_ctx* ctx, const pof_problem* prob, const pof_opts* op
   int rc = pof::build_setup(prob, s, ctx->err);
   if (rc != POF_OK) return rc;
   if (opts->sequential) return set_err(ctx, POF_E_UNSUPPORTED, "the large-state engine runs the parallel IEKS only");
+  if (opts->products != POF_PRODUCTS_F64 && opts->products != POF_PRODUCTS_F32)
+    return set_err(ctx, POF_E_ARG, "products must be POF_PRODUCTS_F64 or POF_PRODUCTS_F32");
   if (s.Ns > 65534) return set_err(ctx, POF_E_ARG, "the large-state engine takes at most 65534 steps");
   std::vector<double> m0;
   rc = (prob->init == POF_INIT_COARSE) ? POF_E_UNSUPPORTED : pof::initial_means(prob, s, m0, ctx->err);
@@ -1399,7 +1403,8 @@ int pof_solve(pof_ctx* ctx, const pof_problem* prob, const pof_opts* opts, pof_o
   pof::Setup s;
   int rc = pof::build_setup(prob, s, ctx->err);
   if (rc != POF_OK) return rc;
-  if (!pof::find_engine(s.D, s.d) && pof::large::supports(prob->kind, s.D, s.d))
+  if (opts->products == POF_PRODUCTS_F32 ||
+      (!pof::find_engine(s.D, s.d) && pof::large::supports(prob->kind, s.D, s.d)))
     return solve_large(ctx, prob, opts, out, stream);
   if (!engine_or_err(ctx, s.D, s.d)) return POF_E_UNSUPPORTED;
   pof_batch* b = nullptr;

@@ -360,6 +360,28 @@ __global__ __launch_bounds__(256) void k_copy_t(double* dst, long long ds, int d
   }
 }
 
+// FP32 staging of a GEMM operand: dst item b (rows x cols, contiguous) = (float) src item b.
+__global__ void k_to_f32(float* dst, long long ds, const double* src, long long ss, int sld, int rows, int cols,
+                         int cnt) {
+  const int b = blockIdx.y;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= cnt || e >= rows * cols) return;
+  const int r = e / cols, c = e % cols;
+  dst[(long long)b * ds + e] = (float)src[(long long)b * ss + (size_t)r * sld + c];
+}
+
+// dst item b (rows x cols) = alpha * (double) src item b + beta * dst (beta = 0: dst is not read).
+__global__ void k_from_f32(double* dst, long long ds, int dld, const float* src, long long ss, int rows, int cols,
+                           int cnt, double alpha, double beta) {
+  const int b = blockIdx.y;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= cnt || e >= rows * cols) return;
+  const int r = e / cols, c = e % cols;
+  double* o = dst + (long long)b * ds + (size_t)r * dld + c;
+  const double p = alpha * (double)src[(long long)b * ss + e];
+  *o = (beta == 0.0) ? p : fma(beta, *o, p);
+}
+
 // dst item (rows x cols) = dv on the diagonal, 0 elsewhere
 __global__ void k_fill(double* dst, long long ds, int dld, int rows, int cols, int cnt, double dv) {
   const int b = blockIdx.y;
@@ -519,6 +541,8 @@ struct Ops {
   Arena* ar;
   int err = 0;
   std::string msg;
+  bool f32 = false;          // gemm() forms matrix-matrix products as FP32 GEMMs (set inside the combines)
+  bool f32_combines = false;  // the solve asked for FP32 combine products
 
   void fail(const char* what, int code) {
     if (!err) {
@@ -529,8 +553,51 @@ struct Ops {
     }
   }
   static rocblas_operation op(bool t) { return t ? rocblas_operation_transpose : rocblas_operation_none; }
-  // row-major C (M x N) = alpha op(A) op(B) + beta C
+  // row-major C (M x N) = alpha op(A) op(B) + beta C: FP64, or with f32 the product
+  // op(A) op(B) as an FP32 GEMM (FP32 operands, FP32 accumulate) for matrix-matrix
+  // shapes, scaled and added to C in FP64
   void gemm(bool tA, bool tB, int M, int N, int K, double alpha, V A, V B, double beta, V C, int cnt) {
+    if (f32 && M > 1 && N > 1 && K > 1)
+      gemm32(tA, tB, M, N, K, alpha, A, B, beta, C, cnt);
+    else
+      gemm64(tA, tB, M, N, K, alpha, A, B, beta, C, cnt);
+  }
+  void gemm32(bool tA, bool tB, int M, int N, int K, double alpha, V A, V B, double beta, V C, int cnt) {
+    if (err || cnt <= 0) return;
+    const int ra = tA ? K : M, ca = tA ? M : K, rb = tB ? N : K, cb = tB ? K : N;
+    const int na = A.s ? cnt : 1, nb = B.s ? cnt : 1;
+    const size_t mark = arena_mark(ar);
+    float* a32 = (float*)arena_take(ar, ((size_t)na * ra * ca + 1) / 2);
+    float* b32 = (float*)arena_take(ar, ((size_t)nb * rb * cb + 1) / 2);
+    float* p32 = (float*)arena_take(ar, ((size_t)cnt * M * N + 1) / 2);
+    if (!a32 || !b32 || !p32) {
+      fail("workspace (FP32 products)", -2);
+      arena_release(ar, mark);
+      return;
+    }
+    to_f32(a32, A, ra, ca, na);
+    to_f32(b32, B, rb, cb, nb);
+    const float one = 1.0f, zero = 0.0f;
+    const rocblas_status s = rocblas_sgemm_strided_batched(
+        h, op(tB), op(tA), N, M, K, &one, b32, cb, nb > 1 ? (long long)rb * cb : 0, a32, ca,
+        na > 1 ? (long long)ra * ca : 0, &zero, p32, N, (long long)M * N, cnt);
+    if (s != rocblas_status_success) fail("rocblas_sgemm_strided_batched", (int)s);
+    if (!err) {
+      const int e = M * N;
+      hipLaunchKernelGGL(k_from_f32, dim3((e + 255) / 256, cnt), dim3(256), 0, st, C.p, C.s, C.ld, p32,
+                         (long long)e, M, N, cnt, alpha, beta);
+      check("k_from_f32");
+    }
+    arena_release(ar, mark);
+  }
+  void to_f32(float* dst, V src, int rows, int cols, int cnt) {
+    if (err) return;
+    const int e = rows * cols;
+    hipLaunchKernelGGL(k_to_f32, dim3((e + 255) / 256, cnt), dim3(256), 0, st, dst, (long long)e, src.p, src.s,
+                       src.ld, rows, cols, cnt);
+    check("k_to_f32");
+  }
+  void gemm64(bool tA, bool tB, int M, int N, int K, double alpha, V A, V B, double beta, V C, int cnt) {
     if (err || cnt <= 0) return;
     const rocblas_status s =
         rocblas_dgemm_strided_batched(h, op(tB), op(tA), N, M, K, &alpha, B.p, B.ld, B.s, A.p, A.ld, A.s, &beta,
@@ -583,9 +650,10 @@ struct Ops {
       if (mr <= 0) continue;
       // A_trail (I - V^T T V): Y = A_trail V^T, Y2 = Y T, A_trail -= Y2 V
       const V At = A.at(i0 + np, i0);
-      gemm(false, true, mr, np, nc, 1.0, At, Vb, 0.0, Y, cnt);
-      gemm(false, false, mr, np, np, 1.0, Y, Tb, 0.0, Y2, cnt);
-      gemm(false, false, mr, nc, np, -1.0, Y2, Vb, 1.0, At, cnt);
+      // the factorisation's own products stay FP64
+      gemm64(false, true, mr, np, nc, 1.0, At, Vb, 0.0, Y, cnt);
+      gemm64(false, false, mr, np, np, 1.0, Y, Tb, 0.0, Y2, cnt);
+      gemm64(false, false, mr, nc, np, -1.0, Y2, Vb, 1.0, At, cnt);
     }
     arena_release(ar, mark);
   }
@@ -678,8 +746,17 @@ bool se_ok(const SE& e) { return e.g.p && e.E.p && e.Dm.p; }
 
 // Filter combine (earlier e1, later e2) -> out, filter.py:117-142 with the
 // operand order of the reference.
+// With FP32 combine products the GEMMs of this function run in FP32 (the
+// factorisations and triangular solves inside stay FP64).
+struct CombineProducts {
+  Ops& o;
+  explicit CombineProducts(Ops& o_) : o(o_) { o.f32 = o.f32_combines; }
+  ~CombineProducts() { o.f32 = false; }
+};
+
 void filter_fn(Ops& o, Arena& ar, int D, const FE& e1, const FE& e2, const FE& out, int cnt) {
   if (cnt <= 0 || o.err) return;
+  CombineProducts prec(o);
   const size_t mark = ar.off;
   const long long DD = (long long)D * D;
   V X{ar.take((size_t)cnt * 4 * DD), 4 * DD, 2 * D};
@@ -740,6 +817,7 @@ void filter_fn(Ops& o, Arena& ar, int D, const FE& e1, const FE& e2, const FE& o
 // Smoother combine (smoother.py:53-63; under reverse=True e1 is the later aggregate)
 void smoother_fn(Ops& o, Arena& ar, int D, const SE& e1, const SE& e2, const SE& out, int cnt, bool with_chol) {
   if (cnt <= 0 || o.err) return;
+  CombineProducts prec(o);
   const size_t mark = ar.off;
   o.copy(out.g, e2.g, D, 1, cnt);
   o.gemm(false, false, D, 1, D, 1.0, e2.E, e1.g, 1.0, out.g, cnt);
@@ -835,7 +913,7 @@ bool supports(int kind, int D, int d) {
 namespace {
 
 // workspace doubles for one solve
-size_t workspace_doubles(int D, int d, long long Ns) {
+size_t workspace_doubles(int D, int d, long long Ns, bool f32) {
   const size_t DD = (size_t)D * D;
   const size_t n1 = (size_t)Ns + 1;
   size_t w = 0;
@@ -852,6 +930,8 @@ size_t workspace_doubles(int D, int d, long long Ns) {
   w += (size_t)(Ns + 1) * (2 * D * kNB + 2 * 2 * D * kNB + kNB * kNB) + 4096;  // blocked-LQ panels
   w += (size_t)Ns * (5 * DD + (size_t)d * D + 2 * D + 3 * d + 4) + 4096;
   w += 8 * n1 + 64 * 1024;
+  // FP32 staging of one GEMM (both operands and the product; at most n1 items of D x D)
+  if (f32) w += n1 * 2 * DD + 3 * 4096;
   return w + w / 10;
 }
 
@@ -879,7 +959,8 @@ int solve(Engine* e, const Problem& p, const pof_opts& opts, pof_out* out, const
     return POF_E_UNSUPPORTED;
   }
   // workspace
-  const size_t need = workspace_doubles(D, d, Ns) * sizeof(double);
+  const bool f32 = opts.products == POF_PRODUCTS_F32;
+  const size_t need = workspace_doubles(D, d, Ns, f32) * sizeof(double);
   if (e->cap < need) {
     if (e->buf) (void)hipFree(e->buf);
     e->buf = nullptr;
@@ -892,6 +973,7 @@ int solve(Engine* e, const Problem& p, const pof_opts& opts, pof_out* out, const
   }
   Arena ar{e->buf, e->cap, 0};
   Ops o{e->h, st, &ar};
+  o.f32_combines = f32;
   rocblas_set_stream(e->h, st);
   // constants: F, QL, N1z, P, x0 (one block)
   double* cst = ar.take(5 * DD + 2 * D);

@@ -35,6 +35,8 @@ KINDS = {
 INIT_PRIOR = 0
 INIT_CONSTANT = 1
 INIT_COARSE = 2
+PRODUCTS_F64 = 0
+PRODUCTS_F32 = 1
 LIN_METHODS = {"ek1": 0, "ek0": 1, "uncertain": 2, "regularized": 3, "unscented": 4}
 MAX_PARAMS = 8
 STATUS = {0: "converged", 1: "maxiter", 2: "nan", 3: "fixed"}
@@ -61,6 +63,7 @@ class Opts(ctypes.Structure):
         ("fixed_iters", ctypes.c_int32),
         ("calibrate", ctypes.c_int32),
         ("sequential", ctypes.c_int32),
+        ("products", ctypes.c_int32),
     ]
 
 

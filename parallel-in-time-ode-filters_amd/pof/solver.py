@@ -7,7 +7,7 @@ from .utils import MVNSqrt
 
 
 def solve(*, f, y0, ts, order, init="prior", calibrate=True, maxiters=10_000, sequential=False,
-          fixed_iters=None, return_full_states=False, engine="auto"):
+          fixed_iters=None, return_full_states=False, engine="auto", products="float64"):
     """Parallel-in-time IEKS solve on the GPU (``pof/solver.py:11-73``).
 
     Same keyword arguments and result as the reference:
@@ -22,6 +22,10 @@ def solve(*, f, y0, ts, order, init="prior", calibrate=True, maxiters=10_000, se
     ``engine="large"`` forces the large-state engine (batched LQ / GEMM
     primitives per scan level), which ``"auto"`` picks by itself for state
     dimensions beyond the per-worker kernels (Lorenz-96, D = 128).
+    ``products="float32"`` (large-state engine only; implies it) forms the
+    scan combines' matrix-matrix products as FP32 GEMMs on MFMA with FP32
+    accumulate, keeping every triangularisation, triangular solve, element
+    and observation statistic in FP64.
     """
     import torch
 
@@ -37,6 +41,9 @@ def solve(*, f, y0, ts, order, init="prior", calibrate=True, maxiters=10_000, se
     opts.fixed_iters = int(fixed_iters or 0)
     opts.calibrate = 1 if calibrate else 0
     opts.sequential = 1 if sequential else 0
+    if products not in ("float64", "float32"):
+        raise ValueError(f"products={products!r}: expected 'float64' or 'float32'")
+    opts.products = _lib.PRODUCTS_F32 if products == "float32" else _lib.PRODUCTS_F64
     out = _lib.Out()
     out.means_dev = means.data_ptr()
     out.chols_dev = chols.data_ptr()

@@ -107,3 +107,29 @@ def test_lorenz96_matches_oracle(gpu, d, q, N, fixed):
     assert rel(cov(ys.chol.cpu().numpy()) / s_gpu**2, cov(ref.chol) / s_ref**2) < 1e-6
     assert abs(s_gpu - s_ref) <= _ssq_tol(oivp, ts, q, "constant", s_ref, fixed)
     assert abs(info["obj"] - rinfo["obj"]) <= 1e-6 * abs(rinfo["obj"]) + 1e-12
+
+
+def test_lorenz96_fp32_combine_products(gpu):
+    """products="float32" (the scan combines' matrix products as FP32 MFMA GEMMs, everything else
+    FP64) on the D = 128 configuration: means and L L^T / ssq^2 stay within 1e-5 of the FP64 solve.
+    sigma_squared is not compared: whitening the near-zero residual with S^-T amplifies the FP32
+    rounding (measured off by orders of magnitude), so the mode is an experiment, not a default."""
+    import torch
+
+    import pof.ivp
+    import pof.solver
+
+    ivp = pof.ivp.lorenz96(d=32)
+    ts = np.linspace(0.0, 16 * 5.0 / 8192, 17)
+    kw = dict(f=ivp.f, y0=ivp.y0, ts=ts, order=3, init="constant", fixed_iters=3)
+    y64, i64 = pof.solver.solve(engine="large", **kw)
+    y32, i32 = pof.solver.solve(products="float32", **kw)
+    torch.cuda.synchronize()
+    m64, m32 = y64.mean.cpu().numpy(), y32.mean.cpu().numpy()
+    print(f"fp32 combine products: means {rel(m32, m64):.2e}", flush=True)
+    assert rel(m32, m64) < 1e-5
+    c64 = cov(y64.chol.cpu().numpy()) / i64["sigma_squared"] ** 2
+    c32 = cov(y32.chol.cpu().numpy()) / i32["sigma_squared"] ** 2
+    assert rel(c32, c64) < 1e-5
+    with pytest.raises(ValueError):
+        pof.solver.solve(products="float16", **kw)
