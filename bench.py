@@ -344,7 +344,9 @@ def precision_run(ns=8192, iters=3, check_ns=32, check_iters=3):
     the oracle: max relative mean error (pof.workloads.max_rel_err), the
     covariance error ||c c^T - c_ref c_ref^T|| / ||c_ref c_ref^T|| after dividing
     out sigma_squared^2 (sign-free; sigma_squared itself depends on factor signs
-    for d >= 2), and the relative sigma_squared difference.
+    for d >= 2), and the relative sigma_squared difference.  The same figures
+    with the scan combines' products in FP32 (``products="float32"``) go into
+    ``fp32_products``.
     """
     import numpy as np
     import torch
