@@ -14,6 +14,8 @@ loop works as follows:
   between iterations.
 * It retires each IVP the iteration it converges, copies its result to the
   host, and frees its range for the next request.
+* A request larger than its group's whole workspace is answered at once with
+  status ``"rejected"`` (no result).
 
 An IVP's arithmetic does not depend on what else is in flight, so a result
 served in the mix equals the same solve alone.
@@ -108,6 +110,15 @@ class Server:
                 try:
                     t = g.admit(f, y0, ts, init=init, calibrate=self.calibrate, maxiters=self.maxiters)
                 except _batch.BatchFull:
+                    if g.active == 0:
+                        # larger than the group's whole workspace: it can never be admitted
+                        r = recs[rid]
+                        r.done = clock() - t0
+                        r.status = "rejected"
+                        ndone += 1
+                        if on_done is not None:
+                            on_done(r, ndone)
+                        continue
                     blocked.add(key)
                     still.append(rid)
                     continue
@@ -149,7 +160,7 @@ class Server:
 
 
 def latency_stats(recs):
-    lat = np.array([r.done - r.arrival for r in recs if r.done is not None])
+    lat = np.array([r.done - r.arrival for r in recs if r.done is not None and r.status != "rejected"])
     if lat.size == 0:
         return {"p50_ms": None, "p99_ms": None, "mean_ms": None}
     return {

@@ -212,3 +212,22 @@ def test_server_results_same_as_alone(gpu):
         assert r.iterations == info["iterations"], r.rid
         assert np.array_equal(r.means, ys.mean.cpu().numpy()), r.rid
         assert r.done >= r.admitted >= r.arrival
+
+
+def test_server_rejects_a_request_that_cannot_fit(gpu):
+    """A request larger than its group's whole workspace is answered as rejected (the loop does not
+    wait for it forever); the others are served."""
+    import pof.ivp
+    import pof.serve
+
+    ivp = pof.ivp.logistic()
+    # budget 1 byte: every group gets the minimum workspace of 4096 entries (IVPs up to 2047 steps)
+    srv = pof.serve.Server(budget_bytes=1, n_groups=1, slots=4, maxiters=30)
+    reqs = [(ivp.f, ivp.y0, np.linspace(0.0, 10.0, 100), 2, "constant"),
+            (ivp.f, ivp.y0, np.linspace(0.0, 10.0, 5000), 2, "constant"),
+            (ivp.f, ivp.y0, np.linspace(0.0, 10.0, 200), 2, "constant")]
+    recs = srv.run(reqs, np.zeros(3))
+    srv.close()
+    assert [r.status for r in recs][1] == "rejected" and recs[1].means is None
+    assert recs[0].status in ("converged", "maxiter") and recs[2].status in ("converged", "maxiter")
+    assert pof.serve.latency_stats(recs)["p50_ms"] is not None

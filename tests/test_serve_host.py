@@ -33,3 +33,12 @@ def test_latency_stats():
     assert abs(st["mean_ms"] - 25.0) < 1e-9
     assert abs(st["p99_ms"] - np.percentile([10, 20, 30, 40], 99)) < 1e-9
     assert serve.latency_stats([serve.Record(0, 0.0)])["p50_ms"] is None
+
+
+def test_latency_stats_skip_rejected():
+    a = serve.Record(0, 0.0)
+    a.done = 0.010
+    b = serve.Record(1, 0.0)
+    b.done = 5.0
+    b.status = "rejected"
+    assert abs(serve.latency_stats([a, b])["p50_ms"] - 10.0) < 1e-9
