@@ -386,7 +386,20 @@ def precision_run(ns=8192, iters=3, check_ns=32, check_iters=3):
         return t, finite, elem, norm, cov_err, ssq_diff
 
     t, finite, elem, norm, cov_err, ssq_diff = measure("float64")
-    t32, finite32, elem32, norm32, cov32, ssq32 = measure("float32")
+    try:
+        t32, finite32, elem32, norm32, cov32, ssq32 = measure("float32")
+        fp32 = {
+            "what": "scan combines' matrix products as FP32 MFMA GEMMs (FP32 accumulate); factorisations, "
+                    "triangular solves, elements and observation statistics FP64",
+            "ms_per_iteration": 1e3 * t32 / iters,
+            "finite": finite32,
+            "max_rel_err_means": elem32,
+            "normwise_err_means": norm32,
+            "cov_rel_err": cov32,
+            "sigma_squared_rel_diff": ssq32,
+        }
+    except RuntimeError as e:  # an optional experiment: its failure must not cost the line
+        fp32 = {"error": str(e)}
     return {
         "workload": f"Lorenz-96 d=32 F=8 seed 0, t=[0,{5.0 * ns / 8192:g}] Ns={ns} q=3 (D=128), f64, constant init, "
                     f"{iters} fixed iterations, large-state engine",
@@ -398,16 +411,7 @@ def precision_run(ns=8192, iters=3, check_ns=32, check_iters=3):
         "normwise_err_means": norm,
         "cov_rel_err": cov_err,
         "sigma_squared_rel_diff": ssq_diff,
-        "fp32_products": {
-            "what": "scan combines' matrix products as FP32 MFMA GEMMs (FP32 accumulate); factorisations, "
-                    "triangular solves, elements and observation statistics FP64",
-            "ms_per_iteration": 1e3 * t32 / iters,
-            "finite": finite32,
-            "max_rel_err_means": elem32,
-            "normwise_err_means": norm32,
-            "cov_rel_err": cov32,
-            "sigma_squared_rel_diff": ssq32,
-        },
+        "fp32_products": fp32,
     }
 
 
