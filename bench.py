@@ -335,7 +335,7 @@ def served_run(n_req, rate, maxiters, seed=5, verbose=False):
     }
 
 
-def precision_run(ns=8192, iters=3, check_ns=32, check_iters=3):
+def precision_run(ns=8192, iters=3, check_ns=32, check_iters=3, fp32_products=True):
     """Lorenz-96 (d = 32, F = 8, y0 = F + 0.01 N(0, 1) from seed 0), t in [0, 5] with 8192 steps, q = 3:
     D = 128 on the large-state engine (batched LQ / GEMM primitives per scan level), float64.
 
@@ -387,6 +387,8 @@ def precision_run(ns=8192, iters=3, check_ns=32, check_iters=3):
 
     t, finite, elem, norm, cov_err, ssq_diff = measure("float64")
     try:
+        if not fp32_products:
+            raise RuntimeError("not run (--no-fp32-products)")
         t32, finite32, elem32, norm32, cov32, ssq32 = measure("float32")
         fp32 = {
             "what": "scan combines' matrix products as FP32 MFMA GEMMs (FP32 accumulate); factorisations, "
@@ -447,6 +449,8 @@ def main():
     ap.add_argument("--precision-only", action="store_true",
                     help="only the Lorenz-96 D = 128 measurement (printed as its own JSON object)")
     ap.add_argument("--precision-ns", type=int, default=8192)
+    ap.add_argument("--no-fp32-products", action="store_true",
+                    help="skip the Lorenz-96 run with FP32 combine products")
     ap.add_argument("--batch-ivps", type=int, default=2048)
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--serve-rate", type=float, default=None,
@@ -463,7 +467,8 @@ def main():
         import torch
 
         torch.cuda.set_device(0)
-        print(json.dumps({"precision": precision_run(args.precision_ns)}), flush=True)
+        print(json.dumps({"precision": precision_run(args.precision_ns, fp32_products=not args.no_fp32_products)}),
+              flush=True)
         return
     g = gpu_run(args.steps, args.warmup)
     if args.verbose:
@@ -484,7 +489,8 @@ def main():
                   if args.serve_requests > 0 else None)
         if args.verbose:
             print(json.dumps(served), file=sys.stderr, flush=True)
-    precision = None if args.no_precision else precision_run(args.precision_ns)
+    precision = (None if args.no_precision
+                 else precision_run(args.precision_ns, fp32_products=not args.no_fp32_products))
     if args.verbose and precision:
         print(json.dumps(precision), file=sys.stderr, flush=True)
     cpu = None if args.no_cpu_baseline else cpu_baseline()

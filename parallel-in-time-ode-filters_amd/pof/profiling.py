@@ -106,9 +106,16 @@ def stats_markdown(path, command=""):
 
 
 def kernel_class(name):
-    """rocBLAS / Tensile GEMM kernels (``Cijk_*``) form one class; other kernels by short name."""
+    """rocBLAS / Tensile GEMM kernels (``Cijk_*``) form one class per data type (``_DB_`` FP64,
+    ``_SB_`` FP32); other kernels by short name."""
     k = short_name(name)
-    return "rocBLAS DGEMM (Cijk_*, FP64 MFMA)" if k.startswith("Cijk_") else k
+    if not k.startswith("Cijk_"):
+        return k
+    if "_DB_" in k:
+        return "rocBLAS DGEMM (Cijk_*, FP64 MFMA)"
+    if "_SB_" in k:
+        return "rocBLAS SGEMM (Cijk_*, FP32 MFMA)"
+    return "rocBLAS GEMM (Cijk_*)"
 
 
 def mfma_rows(pmc_csv, stats_csv=None):
