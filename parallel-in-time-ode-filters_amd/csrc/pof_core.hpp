@@ -80,7 +80,7 @@ struct LdsNeeds {
   static constexpr int NT = (K == kLdsFull) ? 5
                             : (K == kLdsState) ? 4
                             : (K == kLdsSeq) ? 5
-                            : (K == kLdsPost) ? 3
+                            : (K == kLdsPost) ? 2
                             : (K == kLdsLeaf) ? 1
                                               : 2;  // kLdsSmooth, kLdsLeaf1
   static constexpr int NV = (K == kLdsFull || K == kLdsState || K == kLdsSeq) ? ((K == kLdsFull) ? 8 : 4)

@@ -99,7 +99,7 @@ __device__ __forceinline__ ModelArgs view(const GroupArgs& g, int j) {
 // allows more waves per SIMD than their natural register allocation: the
 // bound trades a small spill for occupancy.  Measured to pay at D = 12 only
 // (leaf 3 -> 4, post 2 -> 3 waves, +0.9% on Lorenz-63 Ns = 65536); the same
-// bounds on leaf<15> and post<10> ran slower.
+// bounds on leaf<15>, post<10> and post<15> ran slower.
 template <int D, bool EK1>
 constexpr int leaf_min_waves(bool first) {
   return (first || !EK1) ? 1 : (D == 12) ? 4 : 1;
@@ -251,12 +251,11 @@ __global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK), (post_min_waves<D, EK1>(
     }
     return;
   }
-  // three tiles: L (dead once Phi is formed) then E; Phi11 (dead once S's
-  // rows are formed) then S; D
+  // two tiles: L (dead once Phi is formed) then E; Phi11 (dead once S's rows
+  // are formed) then S.  D = Phi22 is stored from the lanes that hold it.
   double* TL = cx.tile(0);
   double* TP11 = cx.tile(1);
   double* TE = TL;
-  double* TD = cx.tile(2);
   double* TS = TP11;
   double* Vm = cx.vec(0);
   double* Vfm = cx.vec(1);
@@ -321,7 +320,8 @@ __global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK), (post_min_waves<D, EK1>(
     for (int t = 0; t < NL; ++t) TP11[r * LD + t * S + sb] = ph[t];
   }
   wsync();
-  // E row = Phi21 row Phi11^-1 ; D = Phi22 ; moved to row r - D
+  // E row = Phi21 row Phi11^-1 (moved to row r - D through LDS); D = Phi22,
+  // stored when the smoothed factors are wanted
   {
     double e[NL];
 #pragma unroll
@@ -329,9 +329,12 @@ __global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK), (post_min_waves<D, EK1>(
     rsolve_lower_s<S, D, LD>(e, TP11, sb);
     if (r >= D && r < 2 * D) {
 #pragma unroll
-      for (int t = 0; t < NL; ++t) {
-        TE[(r - D) * LD + t * S + sb] = e[t];
-        TD[(r - D) * LD + t * S + sb] = ph[NL + t];
+      for (int t = 0; t < NL; ++t) TE[(r - D) * LD + t * S + sb] = e[t];
+      if (wc) {
+        double dd[NL];
+#pragma unroll
+        for (int t = 0; t < NL; ++t) dd[t] = ph[NL + t];
+        store_dist<S, D>(sel.Dm + (ro * D + (r - D)) * D, dd, sb);
       }
     }
   }
@@ -381,15 +384,13 @@ __global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK), (post_min_waves<D, EK1>(
   }
   }
   if (act) {
-    double e[NL], dd[NL];
+    double e[NL];
     load_dist<S, D>(e, TE + r * LD, sb);
-    load_dist<S, D>(dd, TD + r * LD, sb);
     double gg = Vm[r];
 #pragma unroll
     for (int k = 0; k < D; ++k) gg = fma(-TE[r * LD + k], Vfm[k], gg);
     if (sb == 0) sel.g[ro * D + r] = gg;
     store_dist<S, D>(sel.E + (ro * D + r) * D, e, sb);
-    if (wc) store_dist<S, D>(sel.Dm + (ro * D + r) * D, dd, sb);
   }
   wsync();
 }
