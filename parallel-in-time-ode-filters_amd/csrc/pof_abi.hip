@@ -296,19 +296,31 @@ static int build_setup(const pof_problem* pr, Setup& s, std::string& err) {
 }
 
 // get_initial_trajectory (convenience.py:76-92)
+// The state every point of a constant initial trajectory holds: [y0, f(y0), 0, ...]
+// per dimension, times PI (initialization.py:42-56, convenience.py:84-87).
+static int constant_state(const pof_problem* pr, const Setup& s, std::vector<double>& x, std::string& err) {
+  const int D = s.D, d = s.d, q1 = s.q + 1;
+  std::vector<double> f(d);
+  if (!field_eval(pr->kind, d, pr->params, pr->y0_host, f.data())) {
+    err = "field unsupported";
+    return POF_E_ARG;
+  }
+  x.assign(D, 0.0);
+  for (int i = 0; i < d; ++i) {
+    x[(size_t)i * q1] = pr->y0_host[i] * s.PI[(size_t)i * q1];
+    x[(size_t)i * q1 + 1] = f[i] * s.PI[(size_t)i * q1 + 1];
+  }
+  return POF_OK;
+}
+
 static int initial_means(const pof_problem* pr, const Setup& s, std::vector<double>& m, std::string& err) {
   const int D = s.D, d = s.d, q1 = s.q + 1;
   const size_t N = (size_t)s.Ns + 1;
   m.assign(N * D, 0.0);
   if (pr->init == POF_INIT_CONSTANT) {
-    // [y0, f(y0), 0, ...] per dimension, times PI (initialization.py:42-56, convenience.py:84-87)
-    std::vector<double> f(d);
-    if (!field_eval(pr->kind, d, pr->params, pr->y0_host, f.data())) { err = "field unsupported"; return POF_E_ARG; }
-    std::vector<double> x(D, 0.0);
-    for (int i = 0; i < d; ++i) {
-      x[(size_t)i * q1] = pr->y0_host[i] * s.PI[(size_t)i * q1];
-      x[(size_t)i * q1 + 1] = f[i] * s.PI[(size_t)i * q1 + 1];
-    }
+    std::vector<double> x;
+    const int rc = constant_state(pr, s, x, err);
+    if (rc != POF_OK) return rc;
     for (size_t t = 0; t < N; ++t) std::copy(x.begin(), x.end(), m.begin() + t * D);
     return POF_OK;
   }
@@ -400,6 +412,7 @@ struct pof_batch {
   pof::GroupArgs g{};
   // writable device tables (g holds the same pointers)
   double *F = nullptr, *LQ = nullptr, *N1z = nullptr, *Pd = nullptr, *x0m = nullptr, *x0L = nullptr;
+  double* cvec = nullptr;  // [slots, D] the state of a constant initial trajectory
   double *params = nullptr, *scale = nullptr, *scal_dev = nullptr, *scal_host = nullptr;
   int *kind = nullptr, *Ns = nullptr, *cur = nullptr, *wchol = nullptr, *pfull = nullptr;
   long long* base = nullptr;
@@ -407,7 +420,7 @@ struct pof_batch {
   int* maps_dev = nullptr;
   size_t maps_cap = 0;
   // host mirrors
-  std::vector<double> hPd, hx0m, hx0L, hparams, hscale;
+  std::vector<double> hPd, hx0m, hx0L, hparams, hscale, hcvec;
   std::vector<int> hkind, hNs, hcur, hwchol;  // hcur: 0, uploaded at admission (the device flips it)
   std::vector<int> hpfull;
   std::vector<long long> hbase;
@@ -420,6 +433,7 @@ struct pof_batch {
   std::vector<std::pair<long long, const double*>> init_dev;         // ... or to copy on the device
   std::vector<long long> init_zero;                                  // ... or to clear (entries, count D each)
   std::vector<long long> init_zero_n;
+  std::vector<int> init_const;  // slots whose initial means repeat their cvec row (filled on the device)
   std::vector<int> active;
   pof::LaunchPlan plan;
   bool plan_dirty = true;
@@ -461,6 +475,12 @@ int n_levels_host(long long n0) {
 }
 
 long long entries_for(long long n_times) { return 2 * n_times; }
+
+// A constant initial trajectory: rows 0 .. n - 1 of m (D each) = v.
+__global__ void k_fill_rows(double* m, const double* v, long long n, int D) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n * D) m[e] = v[e % D];
+}
 
 // After an iteration: every active IVP's result buffer becomes its next linearisation point.
 __global__ void k_flip_cur(int* cur, const int* ivp, int m) {
@@ -563,7 +583,7 @@ int batch_new(pof_ctx* ctx, int D, int d, long long entries, int slots, bool aff
   POF_A(b->F, DD) POF_A(b->LQ, DD) POF_A(b->N1z, DD) POF_A(b->Pd, S * D) POF_A(b->x0m, S * D)
   POF_A(b->x0L, S * DD) POF_A(b->params, S * kMaxParams) POF_A(b->scale, S) POF_A(b->scal_dev, S * 8)
   POF_A(b->kind, S) POF_A(b->Ns, S) POF_A(b->cur, S) POF_A(b->wchol, S) POF_A(b->base, S) POF_A(b->outp, 4 * S)
-  POF_A(b->pfull, S)
+  POF_A(b->pfull, S) POF_A(b->cvec, S * D)
 #undef POF_A
   if (he == hipSuccess) he = hipHostMalloc((void**)&b->scal_host, S * 8 * sizeof(double));
   if (he == hipSuccess) he = hipStreamCreateWithFlags(&b->own_st, hipStreamNonBlocking);
@@ -609,6 +629,7 @@ int batch_new(pof_ctx* ctx, int D, int d, long long entries, int slots, bool aff
   g.lq_lower = 0;
   b->hPd.assign(S * D, 0.0);
   b->hx0m.assign(S * D, 0.0);
+  b->hcvec.assign(S * D, 0.0);
   b->hx0L.assign(S * DD, 0.0);
   b->hparams.assign(S * kMaxParams, 0.0);
   b->hscale.assign(S, 1.0);
@@ -728,9 +749,12 @@ int settle(pof_batch* b) {
   return POF_OK;
 }
 
+// m_const (D values), if given, is the state of a constant initial trajectory,
+// filled on the device instead of uploading Ns + 1 copies.
 int admit_raw(pof_batch* b, int Nsteps, const double* Pd, const double* x0m, const double* x0L, int kind,
               const double* params, std::vector<double>* m_host, const double* m_dev, const pof_opts* opts,
-              pof_out* out, int scale_mode, int mode, bool pfull, long long* ticket) {
+              pof_out* out, int scale_mode, int mode, bool pfull, long long* ticket,
+              const double* m_const = nullptr) {
   pof_ctx* ctx = b->ctx;
   if (b->launched) return set_err(ctx, POF_E_ARG, "admit between pof_batch_launch and pof_batch_finish");
   bool busy = false;
@@ -777,7 +801,10 @@ int admit_raw(pof_batch* b, int Nsteps, const double* Pd, const double* x0m, con
   b->hout[2 * S + j] = out->full_means_dev;
   b->hout[3 * S + j] = out->full_chols_dev;
   b->dirty_slots.push_back(j);
-  if (m_host) {
+  if (m_const) {
+    std::copy(m_const, m_const + D, &b->hcvec[(size_t)j * D]);
+    b->init_const.push_back(j);
+  } else if (m_host) {
     b->init_host.emplace_back(start, std::move(*m_host));
   } else if (m_dev) {
     b->init_dev.emplace_back(start, m_dev);
@@ -823,7 +850,8 @@ int batch_launch(pof_batch* b) {
   }
   if (b->plan.m == 0) return POF_OK;
   // newly admitted IVPs: per-slot tables and initial means
-  if (!b->dirty_slots.empty() || !b->init_host.empty() || !b->init_dev.empty() || !b->init_zero.empty()) {
+  if (!b->dirty_slots.empty() || !b->init_host.empty() || !b->init_dev.empty() || !b->init_zero.empty() ||
+      !b->init_const.empty()) {
     const int D = b->D;
     const size_t DD = (size_t)D * D, S = (size_t)b->slots;
     if (!b->dirty_slots.empty()) {
@@ -847,6 +875,7 @@ int batch_launch(pof_batch* b) {
       HIP_TRY(hipMemcpyAsync(b->cur + l, &b->hcur[l], n * sizeof(int), hipMemcpyHostToDevice, b->st));
       HIP_TRY(hipMemcpyAsync(b->wchol + l, &b->hwchol[l], n * sizeof(int), hipMemcpyHostToDevice, b->st));
       HIP_TRY(hipMemcpyAsync(b->pfull + l, &b->hpfull[l], n * sizeof(int), hipMemcpyHostToDevice, b->st));
+      HIP_TRY(hipMemcpyAsync(b->cvec + l * D, &b->hcvec[l * D], n * D * 8, hipMemcpyHostToDevice, b->st));
     }
     b->dirty_slots.clear();
     HIP_TRY(hipMemcpyAsync(b->kind, b->hkind.data(), S * sizeof(int), hipMemcpyHostToDevice, b->st));
@@ -862,6 +891,13 @@ int batch_launch(pof_batch* b) {
       HIP_TRY(hipMemcpyAsync(b->g.means0 + (size_t)pr.first * D, pr.second, (size_t)n * D * 8,
                              hipMemcpyDeviceToDevice, b->st));
     }
+    for (int j : b->init_const) {
+      const long long rows = (long long)b->hNs[j] + 1;
+      const long long e = rows * D;
+      hipLaunchKernelGGL(k_fill_rows, dim3((unsigned)((e + 255) / 256)), dim3(256), 0, b->st,
+                         b->g.means0 + (size_t)b->hbase[j] * D, (const double*)(b->cvec + (size_t)j * D), rows, D);
+      HIP_TRY(hipGetLastError());
+    }
     for (size_t i = 0; i < b->init_zero.size(); ++i)
       HIP_TRY(hipMemsetAsync(b->g.means0 + (size_t)b->init_zero[i] * D, 0, (size_t)b->init_zero_n[i] * D * 8,
                              b->st));
@@ -871,6 +907,7 @@ int batch_launch(pof_batch* b) {
     b->init_dev.clear();
     b->init_zero.clear();
     b->init_zero_n.clear();
+    b->init_const.clear();
   }
   // factor flag of this iteration: the smoothed factors are needed only from
   // an IVP's last iteration, known in advance only with fixed_iters
@@ -1166,6 +1203,12 @@ int admit_problem(pof_batch* b, const pof_problem* prob, const pof_opts* opts, p
   if (s.D != b->D || s.d != b->d)
     return set_err(ctx, POF_E_ARG, "problem (D=%d, d=%d) does not match the batch", s.D, s.d);
   std::vector<double> m0;
+  if (prob->init == POF_INIT_CONSTANT) {
+    rc = constant_state(prob, s, m0, ctx->err);
+    if (rc != POF_OK) return rc;
+    return admit_raw(b, s.Ns, s.P.data(), s.x0m.data(), s.x0L.data(), prob->kind, prob->params, nullptr, nullptr,
+                     opts, out, 0, opts->sequential ? 1 : 0, false, ticket, m0.data());
+  }
   rc = (prob->init == POF_INIT_COARSE) ? coarse_means(ctx, prob, s, m0) : initial_means(prob, s, m0, ctx->err);
   if (rc != POF_OK) return rc;
   return admit_raw(b, s.Ns, s.P.data(), s.x0m.data(), s.x0L.data(), prob->kind, prob->params, &m0, nullptr, opts, out,
