@@ -15,6 +15,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/pof.h"
@@ -332,18 +333,36 @@ static int initial_means(const pof_problem* pr, const Setup& s, std::vector<doub
     for (int i = 0; i < d; ++i)
       for (int k = 0; k < q1; ++k) m0[(size_t)i * q1 + k] = der[(size_t)k * d + i];
     std::copy(m0.begin(), m0.end(), m.begin());
-    std::vector<double> Ph, PIh;
-    for (size_t t = 1; t < N; ++t) {
-      nordsieck(d, s.q, pr->ts_host[t], Ph, PIh);
-      for (int b = 0; b < d; ++b)
-        for (int i = 0; i < q1; ++i) {
-          double acc = 0.0;
-          for (int j = 0; j < q1; ++j) {
-            const double Fij = (Ph[(size_t)b * q1 + i] * s.A1[(size_t)i * q1 + j]) * PIh[(size_t)b * q1 + j];
-            acc += Fij * m0[(size_t)b * q1 + j];
+    // rows t in [lo, hi); each row is independent (two pow calls per derivative order),
+    // so long grids are split over host threads with the same arithmetic per row
+    auto rows = [&](size_t lo, size_t hi) {
+      std::vector<double> Ph, PIh;
+      for (size_t t = lo; t < hi; ++t) {
+        nordsieck(d, s.q, pr->ts_host[t], Ph, PIh);
+        for (int b = 0; b < d; ++b)
+          for (int i = 0; i < q1; ++i) {
+            double acc = 0.0;
+            for (int j = 0; j < q1; ++j) {
+              const double Fij = (Ph[(size_t)b * q1 + i] * s.A1[(size_t)i * q1 + j]) * PIh[(size_t)b * q1 + j];
+              acc += Fij * m0[(size_t)b * q1 + j];
+            }
+            m[t * D + (size_t)b * q1 + i] = acc;
           }
-          m[t * D + (size_t)b * q1 + i] = acc;
-        }
+      }
+    };
+    const size_t kRowsPerThread = 8192;
+    const size_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const size_t nt = std::min(hw, (N + kRowsPerThread - 1) / kRowsPerThread);
+    if (nt <= 1) {
+      rows(1, N);
+    } else {
+      std::vector<std::thread> pool;
+      const size_t chunk = (N - 1 + nt - 1) / nt;
+      for (size_t k = 0; k < nt; ++k) {
+        const size_t lo = 1 + k * chunk, hi = std::min(N, lo + chunk);
+        if (lo < hi) pool.emplace_back(rows, lo, hi);
+      }
+      for (auto& th : pool) th.join();
     }
     return POF_OK;
   }
