@@ -231,3 +231,18 @@ def test_server_rejects_a_request_that_cannot_fit(gpu):
     assert [r.status for r in recs][1] == "rejected" and recs[1].means is None
     assert recs[0].status in ("converged", "maxiter") and recs[2].status in ("converged", "maxiter")
     assert pof.serve.latency_stats(recs)["p50_ms"] is not None
+
+
+def test_tiny_ivps_in_a_batch_same_as_alone(gpu):
+    """IVPs of 2 and 3 time points in one batch with a longer one: each equals its solve alone."""
+    import pof.batch
+    import pof.ivp
+    import pof.solver
+
+    ivp = pof.ivp.logistic()
+    probs = [(ivp.f, ivp.y0, np.linspace(0.0, 1.0, n), 2, "constant") for n in (2, 3, 200, 2)]
+    res = pof.batch.solve_batch(probs, maxiters=60)
+    for (f, y0, ts, q, init), (ys, info) in zip(probs, res):
+        ys1, info1 = pof.solver.solve(f=f, y0=y0, ts=ts, order=q, init=init, maxiters=60)
+        assert info["iterations"] == info1["iterations"]
+        assert _equal(ys.mean, ys1.mean) and _equal(ys.chol, ys1.chol)

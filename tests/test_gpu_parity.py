@@ -212,3 +212,26 @@ def test_ieks_step_matches_oracle(gpu):
     assert rel(out.mean.cpu().numpy(), ref.mean) < 1e-10
     assert rel(cov(out.chol.cpu().numpy()), cov(ref.chol)) < 1e-8
     assert abs(ssq - rssq) <= 1e-9 * abs(rssq)
+
+
+@pytest.mark.parametrize("N", [2, 3, 4, 5])
+@pytest.mark.parametrize("sequential", [False, True])
+def test_tiny_grids_match_oracle(gpu, N, sequential):
+    """The smallest grids (Ns = 1 .. 4: scans of one or two levels, a single combine or none)
+    against the oracle, parallel and sequential; same tolerances as test_solve_matches_oracle."""
+    import torch
+
+    import pof.solver
+    from oracle import solver as osolver
+
+    ivp, oivp = _pair("logistic")
+    ts = np.linspace(0.0, 1.0, N)
+    print(f"tiny grid N={N} sequential={sequential}", flush=True)
+    ys, info = pof.solver.solve(f=ivp.f, y0=ivp.y0, ts=ts, order=2, init="constant", maxiters=100,
+                                sequential=sequential)
+    torch.cuda.synchronize()
+    ref, rinfo = osolver.solve(oivp, ts, 2, init="constant", maxiters=100, is_sequential=sequential)
+    assert ys.mean.shape == (N, 1) and ys.chol.shape == (N, 1, 3)
+    assert info["iterations"] == rinfo["iterations"]
+    assert rel(ys.mean.cpu().numpy(), ref.mean) < 1e-9
+    assert abs(info["obj"] - rinfo["obj"]) <= 1e-6 * abs(rinfo["obj"]) + 1e-12
