@@ -122,6 +122,15 @@ int pof_supported(int D, int d);
 int pof_profile_enable(pof_ctx* ctx, int on);
 int pof_profile_read(const pof_ctx* ctx, int n_classes, double* ms, long long* launches, long long* work);
 
+/* Replay of IEKS iterations from a HIP graph (on by default; no reference
+ * counterpart: the reference's loop is an XLA while_loop, pof/solver.py:36-57).
+ * Once a batch's set of IVPs has been unchanged for two launches, one
+ * iteration's kernel launches are captured into a graph and each later
+ * iteration is one graph launch, until an IVP is admitted or retires.  Results
+ * are bitwise the same either way; off (on = 0) launches every kernel from the
+ * host.  Instrumented runs (pof_profile_enable) always launch from the host. */
+int pof_graphs_enable(pof_ctx* ctx, int on);
+
 /* Full parallel IEKS solve.  Replaces pof.solver.solve (pof/solver.py:11-73). */
 int pof_solve(pof_ctx* ctx, const pof_problem* prob, const pof_opts* opts, pof_out* out, void* stream);
 

@@ -394,6 +394,8 @@ struct pof_ctx {
   std::vector<pof_batch*> aux;
   // large-state engine (batched primitives), created on first use
   pof::large::Engine* large = nullptr;
+  // replay unchanged iterations from a HIP graph (pof_graphs_enable)
+  bool graphs = true;
 };
 
 // A group of IVPs with one (D, d) sharing one device workspace.  IVP slots are
@@ -460,6 +462,11 @@ struct pof_batch {
   bool need_sync = false;
   bool unsynced = false;  // launches whose host-side copies may still be pending
   long long next_ticket = 1;
+  // one iteration's launches captured as a graph, valid while the plan and the
+  // mode are unchanged (their kernel arguments are then identical every launch)
+  hipGraphExec_t gexec = nullptr;
+  int gmode = -1;
+  int gstable = 0;  // launches since the plan was last rebuilt
 };
 
 namespace {
@@ -542,6 +549,7 @@ void batch_free(pof_batch* b) {
   if (!b) return;
   (void)hipSetDevice(b->ctx->device);
   if (b->st) (void)hipStreamSynchronize(b->st);
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   for (void* p : b->allocs) (void)hipFree(p);
   if (b->scal_host) (void)hipHostFree(b->scal_host);
   if (b->maps_dev) (void)hipFree(b->maps_dev);
@@ -836,12 +844,60 @@ int admit_raw(pof_batch* b, int Nsteps, const double* Pd, const double* x0m, con
   return POF_OK;
 }
 
+// Launches of one iteration of b's plan in b's mode, on b->st.
+hipError_t launch_iteration(pof_batch* b, KernelTimer* tm) {
+  hipError_t e;
+  if (b->mode >= 3)
+    e = b->e->smooth_only(b->g, b->plan, b->scal_dev, b->mode == 4 ? 1 : 0, b->st, tm);
+  else if (b->mode >= 1)
+    e = b->e->seq_iteration(b->g, b->plan, b->scal_dev, b->mode == 2 ? 1 : 0, b->st, tm);
+  else
+    e = b->e->iteration(b->g, b->plan, b->scal_dev, b->st, tm);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_flip_cur, dim3((b->plan.m + 255) / 256), dim3(256), 0, b->st, b->cur, b->plan.dev, b->plan.m);
+  return hipGetLastError();
+}
+
+void drop_graph(pof_batch* b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  b->gexec = nullptr;
+  b->gmode = -1;
+}
+
+// Capture one iteration of b's plan and mode into b->gexec.  No capture (eager
+// launches) on the null stream or on a stream the caller is capturing already.
+hipError_t capture_iteration(pof_batch* b) {
+  if (!b->st) return hipSuccess;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  hipError_t e = hipStreamIsCapturing(b->st, &cs);
+  if (e != hipSuccess || cs != hipStreamCaptureStatusNone) return e;
+  e = hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return e;
+  const hipError_t el = launch_iteration(b, nullptr);
+  hipGraph_t gr = nullptr;
+  e = hipStreamEndCapture(b->st, &gr);  // ends the capture whatever the launches returned
+  if (e == hipSuccess) e = el;
+  hipGraphExec_t ge = nullptr;
+  if (e == hipSuccess) e = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
+  if (gr) (void)hipGraphDestroy(gr);
+  if (e != hipSuccess) return e;
+  b->gexec = ge;
+  b->gmode = b->mode;
+  return hipSuccess;
+}
+
+// Iterations launched eagerly after a plan change before the next is captured:
+// a plan that changes at every launch (IVPs retiring) is never captured.
+constexpr int kGraphAfter = 2;
+
 int batch_launch(pof_batch* b) {
   pof_ctx* ctx = b->ctx;
   if (b->launched) return set_err(ctx, POF_E_ARG, "pof_batch_launch called twice without pof_batch_finish");
   HIP_TRY(hipSetDevice(ctx->device));
   if (b->plan_dirty) {
     if (settle(b) != POF_OK) return POF_E_HIP;
+    drop_graph(b);
+    b->gstable = 0;
     b->active.clear();
     for (int i = 0; i < b->slots; ++i)
       if (b->sl[i].used) b->active.push_back(i);
@@ -948,15 +1004,16 @@ int batch_launch(pof_batch* b) {
     HIP_TRY(hipMemcpyAsync(b->wchol, b->hwchol.data(), (size_t)b->slots * sizeof(int), hipMemcpyHostToDevice, b->st));
   }
   KernelTimer* tm = ctx->profiling ? &ctx->timer : nullptr;
-  if (b->mode >= 3) {
-    HIP_TRY(b->e->smooth_only(b->g, b->plan, b->scal_dev, b->mode == 4 ? 1 : 0, b->st, tm));
-  } else if (b->mode >= 1) {
-    HIP_TRY(b->e->seq_iteration(b->g, b->plan, b->scal_dev, b->mode == 2 ? 1 : 0, b->st, tm));
+  // an instrumented run times every launch from the host: no replay then
+  const bool replay = ctx->graphs && !tm;
+  if (b->gexec && b->gmode != b->mode) drop_graph(b);
+  if (replay && !b->gexec && b->gstable >= kGraphAfter) HIP_TRY(capture_iteration(b));
+  if (replay && b->gexec) {
+    HIP_TRY(hipGraphLaunch(b->gexec, b->st));
   } else {
-    HIP_TRY(b->e->iteration(b->g, b->plan, b->scal_dev, b->st, tm));
+    HIP_TRY(launch_iteration(b, tm));
   }
-  hipLaunchKernelGGL(k_flip_cur, dim3((b->plan.m + 255) / 256), dim3(256), 0, b->st, b->cur, b->plan.dev, b->plan.m);
-  HIP_TRY(hipGetLastError());
+  ++b->gstable;
   for (int j : b->active) b->hcur[j] ^= 1;  // host mirror of the flip
   HIP_TRY(collect_timing(ctx, b->st));
   if (b->need_sync) {
@@ -1285,6 +1342,12 @@ int pof_profile_enable(pof_ctx* ctx, int on) {
     ctx->prof_launches[i] = 0;
     ctx->prof_work[i] = 0;
   }
+  return POF_OK;
+}
+
+int pof_graphs_enable(pof_ctx* ctx, int on) {
+  if (!ctx) return POF_E_ARG;
+  ctx->graphs = on != 0;
   return POF_OK;
 }
 

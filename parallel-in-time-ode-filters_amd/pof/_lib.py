@@ -95,6 +95,7 @@ SIGNATURES = {
     "pof_supported": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "pof_profile_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pof_profile_read": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _P(ctypes.c_longlong), _P(ctypes.c_longlong)]),
+    "pof_graphs_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pof_solve": (ctypes.c_int, [_vp, _P(Problem), _P(Opts), _P(Out), _vp]),
     "pof_solve_large": (ctypes.c_int, [_vp, _P(Problem), _P(Opts), _P(Out), _vp]),
     "pof_setup": (ctypes.c_int, [_P(Problem), _dp, _dp, _dp, _dp, _dp, _dp]),
@@ -191,6 +192,12 @@ def context(device=None):
         c = h
         _ctx[key] = c
     return c
+
+
+def set_graphs(on, device=None):
+    """Replay unchanged IEKS iterations from a HIP graph (``on=True``, the default) or launch every kernel."""
+    ctx = context(device)
+    check(load().pof_graphs_enable(ctx, 1 if on else 0), ctx, "pof_graphs_enable")
 
 
 def stream_ptr(device=None):
