@@ -38,7 +38,13 @@ How each part is pinned (the tests that check the oracle itself):
   the README figure's decoded means (``readmefig.svg``), and the zero
   ODE residual at convergence.
 
+The filter's ``nll`` (one-step predictive terms) and the smoother's ``obj``
+(with the reference's call order, ``||QL^-1 (m_k - F m_{k+1})||^2``) are pinned
+on tiny linear models against the dense joint Gaussian, and so is
+``sigma_squared`` for d = 1.
+
 Parity unpinned: the raw entries of Cholesky factors (only ``L L^T`` is
-unique), and ``info["nll"]`` / ``info["obj"]`` values beyond what the
-convergence pins imply (no reference-held number for them exists).
+unique), and ``sigma_squared`` for d > 1, whose terms the reference whitens
+with the factor's transpose (``yhat^T (L^T L)^-1 yhat``), a value that depends
+on the signs of the LAPACK factors along the chain.
 """

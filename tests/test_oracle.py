@@ -186,8 +186,9 @@ def _brute(F, QL, obs, x0):
     sm_m = [ms[k * D : (k + 1) * D] for k in range(Ns + 1)]
     sm_P = [Ps[k * D : (k + 1) * D, k * D : (k + 1) * D] for k in range(Ns + 1)]
     Syy = G @ S @ G.T + Rn
-    nll = 0.5 * y @ np.linalg.solve(Syy, y) + 0.5 * np.linalg.slogdet(2 * np.pi * Syy)[1]
-    return np.array(filt_m), np.array(filt_P), np.array(sm_m), np.array(sm_P), nll
+    maha = y @ np.linalg.solve(Syy, y)
+    nll = 0.5 * maha + 0.5 * np.linalg.slogdet(2 * np.pi * Syy)[1]
+    return np.array(filt_m), np.array(filt_P), np.array(sm_m), np.array(sm_P), nll, maha
 
 
 @pytest.mark.parametrize("d,q,Ns,seed,noise", [(1, 2, 5, 0, False), (2, 1, 6, 1, False), (2, 2, 4, 2, False),
@@ -195,7 +196,7 @@ def _brute(F, QL, obs, x0):
 def test_parallel_filtsmooth_brute_force(d, q, Ns, seed, noise):
     rng = np.random.default_rng(seed)
     F, QL, obs, x0 = _random_model(rng, d, q, Ns, noise=noise)
-    fm, fP, sm, sP, nll = _brute(F, QL, obs, x0)
+    fm, fP, sm, sP, nll, maha = _brute(F, QL, obs, x0)
     filt, pnll, _, pssq = opar.linear_noiseless_filtering(x0, F, QL, obs)
     out, obj = opar.smoothing(F, QL, filt)
     cov = lambda L: np.einsum("nij,nkj->nik", L, L)
@@ -204,16 +205,22 @@ def test_parallel_filtsmooth_brute_force(d, q, Ns, seed, noise):
     np.testing.assert_allclose(cov(filt.chol), fP, atol=1e-8 * np.abs(fP).max())
     np.testing.assert_allclose(out.mean, sm, rtol=1e-8, atol=1e-8 * np.abs(sm).max())
     np.testing.assert_allclose(cov(out.chol), sP, atol=1e-8 * scale)
+    # nll: -log N(0; y, Syy) of all observations at once, which the reference's sum of one-step
+    # predictive terms (filter.py:95-101) equals by the prediction-error decomposition
     assert abs(pnll - nll) < 1e-8 * max(1.0, abs(nll))
+    # sigma^2 (filter.py:104-114): for d = 1 the sum of squared whitened innovations is the
+    # Mahalanobis term y^T Syy^-1 y of the same joint.  For d > 1 the reference whitens with the
+    # factor's transpose (utils.py:110-112: solve_triangular(L^T, m), lower=False), i.e. each term
+    # is yhat^T (L^T L)^-1 yhat, which depends on the signs of the LAPACK factors along the chain
     if d == 1:
-        # sigma^2: mean of squared whitened innovations (sign-free for d = 1)
-        H, c, _ = obs
-        pm = np.einsum("ij,nj->ni", F, fm[:-1])
-        yhat = np.einsum("nij,nj->ni", H, pm) + c
-        Spred = np.einsum("nij,njk,nlk->nil", H, np.einsum("ij,njk,lk->nil", F, fP[:-1], F) + QL @ QL.T, H)
-        Spred = Spred + np.einsum("nij,nkj->nik", obs.cholR, obs.cholR)
-        ssq = np.mean(yhat[:, 0] ** 2 / Spred[:, 0, 0])
-        assert abs(pssq - ssq) < 1e-8 * ssq
+        assert abs(pssq - maha / Ns) < 1e-8 * pssq
+    # obj (smoother.py:20 with utils.py:97-101): the call passes (means[:-1], means[1:]) as
+    # (mnext, m), so each term is ||QL^-1 (m_k - F m_{k+1})||^2, evaluated at the dense joint's
+    # smoothed means; the dynamics' own direction, m_{k+1} - F m_k, gives another value
+    r = np.linalg.solve(QL, (sm[:-1] - sm[1:] @ F.T).T)
+    assert abs(obj - np.sum(r * r)) < 1e-7 * np.sum(r * r)
+    r_fwd = np.linalg.solve(QL, (sm[1:] - sm[:-1] @ F.T).T)
+    assert abs(obj - np.sum(r_fwd * r_fwd)) > 1e-3 * np.sum(r * r)
 
 
 @pytest.mark.parametrize("d,q,Ns,seed,noise", [(2, 2, 23, 5, False), (3, 1, 17, 6, False), (1, 3, 30, 7, False),
