@@ -62,9 +62,14 @@ def gpu_run(steps, warmup):
     for _ in range(warmup):
         solve()
     torch.cuda.synchronize()
+    # each solve returns after its last iteration (it reads info on the host),
+    # so the per-step wall times sum to the timed region
+    step_s = []
     t0 = time.perf_counter()
     for _ in range(steps):
+        ts0 = time.perf_counter()
         ys, info = solve()
+        step_s.append(time.perf_counter() - ts0)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     finite = bool(torch.isfinite(ys.mean).all())
@@ -106,8 +111,25 @@ def gpu_run(steps, warmup):
 
     traffic = read_traffic(str(TRAFFIC_SUMMARY), f"k_{dom}<{D}")
     iter_flops = roofline.iteration_flops(NS, D, d)
+    iter_bytes = roofline.iteration_bytes(NS, D)
+    t_it = float(np.median(step_s)) / ITERS
+    t_hbm = iter_bytes / (roofline.PEAK_HBM_GBS * 1e9)
+    t_fp64 = iter_flops / (roofline.PEAK_FP64_TFLOPS * 1e12)
     return {
         "seconds": dt,
+        "ms_per_iteration_median": 1e3 * t_it,
+        "iteration_roofline": {
+            "what": "whole IEKS iteration at algorithmic counts: bytes s Ns (3D^2 + 4D), flops Ns (74.3 D^3 + "
+                    "4/3 (D + d)^3); time = median over the timed steps / iterations per step",
+            "bytes": iter_bytes,
+            "flops": iter_flops,
+            "ms": 1e3 * t_it,
+            "achieved_gbs": iter_bytes / t_it / 1e9,
+            "achieved_tflops": iter_flops / t_it / 1e12,
+            "hbm_frac": t_hbm / t_it,
+            "fp64_frac": t_fp64 / t_it,
+            "bound": "fp64" if t_fp64 > t_hbm else "hbm",
+        },
         "finite": finite,
         "iterations": info["iterations"],
         "sigma_squared": info["sigma_squared"],
@@ -202,18 +224,26 @@ def cpu_baseline():
         osol.ieks_step(setup, states)
         return time.perf_counter() - t0
 
+    def runs():
+        run()  # warm-up
+        return sorted(run() for _ in range(3))
+
     if threadpool_limits is not None:
         with threadpool_limits(limits=1):
-            t = run()
+            ts_ = runs()
     else:
-        t = run()
+        ts_ = runs()
+    t = ts_[1]
     return {
         "value": 1.0 / (ITERS * t),
         "unit": "solves/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"one IEKS iteration of the Lorenz-63 Ns={NS} q={Q} f64 workload ({t:.2f} s, BLAS/LAPACK on 1 "
-                  f"thread); solves/s = 1 / ({ITERS} x iteration time)",
+        "sample": f"one IEKS iteration of the Lorenz-63 Ns={NS} q={Q} f64 workload, median of 3 after one warm-up "
+                  f"({t:.2f} s; min {ts_[0]:.2f} s), BLAS/LAPACK on 1 thread; solves/s = 1 / ({ITERS} x iteration time)",
+        "iteration_s_min": ts_[0],
+        "iteration_s_median": t,
+        "host_cores": len(os.sched_getaffinity(0)),
     }
 
 
@@ -517,7 +547,9 @@ def main():
             "batch": 1,
         },
         "ms_per_iteration": ms_step / ITERS,
+        "ms_per_iteration_median": g["ms_per_iteration_median"],
         "roofline": g["roofline"],
+        "iteration_roofline": g["iteration_roofline"],
         "cpu_baseline": cpu,
         "kernel_classes_ms_per_solve": {k: round(v["ms"], 4) for k, v in g["per_class"].items()},
         "finite": g["finite"],

@@ -54,7 +54,7 @@ def bytes_per_unit(cls, D, d, s=8):
 
 
 def iteration_flops(Ns, D, d):
-    """SURVEY-style per-iteration total: Ns (74.3 D^3 + 4/3 (D + d)^3)."""
+    """Algorithmic flops of one IEKS iteration: Ns (74.3 D^3 + 4/3 (D + d)^3)."""
     return Ns * (74.3 * D**3 + (4.0 / 3.0) * (D + d) ** 3)
 
 

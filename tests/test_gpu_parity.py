@@ -2,7 +2,7 @@
 
 Tolerances (float64 everywhere): means agree to 1e-9 relative to the largest
 mean entry (the oracle follows the same algorithm; differences are rounding);
-iteration counts are equal.  The north-star bar for means is 1e-6.
+iteration counts are equal.  The acceptance bar for means is 1e-6 relative.
 
 Covariances: the returned factors are ``ssq * L`` (double calibration), and
 only ``c c^T`` is unique.  ``L L^T`` is a function of the model alone and is
@@ -108,7 +108,7 @@ def test_fixed_iterations_lorenz63(gpu):
     torch.cuda.synchronize()
     ref, rinfo = osolver.solve(op.lorenz63(), ts, 3, init="constant", fixed_iters=4)
     assert info["iterations"] == 4.0
-    # chaotic field, unconverged iterates: rounding is amplified; north-star bar 1e-6
+    # chaotic field, unconverged iterates: rounding is amplified; acceptance bar 1e-6
     assert rel(ys.mean.cpu().numpy(), ref.mean) < 1e-6
     s_gpu, s_ref = info["sigma_squared"], rinfo["sigma_squared"]
     assert rel(cov(ys.chol.cpu().numpy()) / s_gpu**2, cov(ref.chol) / s_ref**2) < COV_TOL
