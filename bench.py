@@ -189,6 +189,8 @@ def plain_checks(prefix=512, maxiters=200):
     torch.cuda.synchronize()
     ref, rinfo = osol.solve(op.lorenz63(), tp, Q, init="constant", fixed_iters=ITERS)
     elem, norm = workloads.max_rel_err(yp.mean.cpu().numpy(), ref.mean)
+    seq, _ = osol.solve(op.lorenz63(), tp, Q, init="constant", fixed_iters=ITERS, is_sequential=True)
+    elem_s, norm_s = workloads.max_rel_err(yp.mean.cpu().numpy(), seq.mean)
     c = yp.chol.cpu().numpy() / ip["sigma_squared"]
     cr = ref.chol / rinfo["sigma_squared"]
     cc = np.einsum("nij,nkj->nik", c, c)
@@ -198,6 +200,8 @@ def plain_checks(prefix=512, maxiters=200):
         "max_rel_err_means": elem,
         "normwise_err_means": norm,
         "cov_rel_err": float(np.linalg.norm(cc - ccr) / np.linalg.norm(ccr)),
+        "max_rel_err_means_vs_sequential": elem_s,
+        "normwise_err_means_vs_sequential": norm_s,
     }
     return conv, acc
 
@@ -323,7 +327,7 @@ def _progress(label, every):
     return hook
 
 
-def served_run(n_req, rate, maxiters, seed=5, verbose=False):
+def served_run(n_req, rate, maxiters, seed=5, verbose=False, oracle_sample=2):
     """Poisson request stream served with continuous batching vs a FIFO plain per-request scan."""
     import numpy as np
     import torch
@@ -352,6 +356,18 @@ def served_run(n_req, rate, maxiters, seed=5, verbose=False):
     pst = pof.serve.latency_stats(precs)
     same = all(np.array_equal(a.means, b.means) for a, b in zip(recs, precs))
     srv.close()
+    # accuracy against the FP64 oracle on the cheapest converged requests
+    from oracle import problems as op
+    from oracle import solver as osol
+
+    conv = [i for i, r in enumerate(recs) if r.status == "converged"]
+    conv.sort(key=lambda i: len(reqs[i][2]) * (reqs[i][0].d * (reqs[i][3] + 1)) ** 3)
+    errs = []
+    for i in conv[:oracle_sample]:
+        f, y0, ts, q, init = reqs[i]
+        (om, _), _ = osol.solve(op.Problem(f.name, f.params, y0, ts[0], ts[-1]), ts, q, init=init,
+                                maxiters=maxiters)
+        errs.append(workloads.max_rel_err(recs[i].means, om))
     return {
         "workload": f"{n_req} requests, Poisson arrivals at {rate:.1f}/s (seed {seed}), same IVP mix with "
                     f"Ns log-uniform [256, 65536], maxiters {maxiters}",
@@ -362,6 +378,9 @@ def served_run(n_req, rate, maxiters, seed=5, verbose=False):
         "plain_fifo_p50_ms": pst["p50_ms"],
         "plain_fifo_p99_ms": pst["p99_ms"],
         "same_as_alone": bool(same),
+        "max_rel_err_vs_oracle": max(e[0] for e in errs) if errs else None,
+        "normwise_err_vs_oracle": max(e[1] for e in errs) if errs else None,
+        "oracle_sample": len(errs),
     }
 
 
