@@ -172,6 +172,111 @@ def mfma_markdown(pmc_csv, stats_csv=None, command=""):
     return "\n".join(out) + "\n"
 
 
+ISSUE_INSTS = ["SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_LDS", "SQ_INSTS_VALU_FMA_F64",
+               "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_TRANS_F64"]
+
+
+def _per_dispatch(path):
+    per = collections.defaultdict(dict)
+    names = {}
+    with open(path) as fh:
+        for r in csv.DictReader(fh):
+            did = r.get("Dispatch_Id") or r.get("Correlation_Id")
+            per[did][r["Counter_Name"]] = float(r["Counter_Value"])
+            names[did] = r["Kernel_Name"]
+    return per, names
+
+
+def issue_rows(busy_csv, insts_csv, match="pof::"):
+    """Per kernel class: dispatches, VALU / SALU busy (rocprofv3's VALUBusy and SALUBusy, in percent,
+    weighted by each dispatch's GRBM_GUI_ACTIVE cycles) and instructions per wave from the raw SQ
+    counters of a second pass (``ISSUE_INSTS``)."""
+    per, names = _per_dispatch(busy_csv)
+    busy = collections.defaultdict(lambda: [0, 0.0, 0.0, 0.0])  # n, sum valu*g, sum salu*g, sum g
+    for did, c in per.items():
+        if match not in names[did]:
+            continue
+        a = busy[short_name(names[did])]
+        g = c.get("GRBM_GUI_ACTIVE", 0.0)
+        a[0] += 1
+        a[1] += c.get("VALUBusy", 0.0) * g
+        a[2] += c.get("SALUBusy", 0.0) * g
+        a[3] += g
+    per, names = _per_dispatch(insts_csv)
+    ins = collections.defaultdict(lambda: collections.defaultdict(float))
+    for did, c in per.items():
+        if match not in names[did]:
+            continue
+        a = ins[short_name(names[did])]
+        for k in ISSUE_INSTS:
+            a[k] += c.get(k, 0.0)
+    rows = []
+    for k, (n, vg, sg, g) in sorted(busy.items(), key=lambda x: -x[1][3]):
+        c = ins.get(k, {})
+        w = c.get("SQ_WAVES", 0.0) or 1.0
+        f64 = sum(c.get(x, 0.0) for x in ISSUE_INSTS[4:])
+        valu = c.get("SQ_INSTS_VALU", 0.0)
+        rows.append({
+            "kernel": k, "dispatches": n, "valu_busy": vg / g if g else 0.0, "salu_busy": sg / g if g else 0.0,
+            "cycles_share": g, "valu_per_wave": valu / w, "salu_per_wave": c.get("SQ_INSTS_SALU", 0.0) / w,
+            "lds_per_wave": c.get("SQ_INSTS_LDS", 0.0) / w, "f64_per_wave": f64 / w,
+            "f64_share": f64 / valu if valu else 0.0,
+        })
+    tot = sum(r["cycles_share"] for r in rows) or 1.0
+    for r in rows:
+        r["cycles_share"] /= tot
+    return rows
+
+
+def busy_by_grid(busy_csv, kernels):
+    """{(kernel, grid size): (dispatches, VALUBusy weighted by GRBM_GUI_ACTIVE)} for the kernels whose
+    short name is in ``kernels``: one entry per scan level width."""
+    agg = collections.defaultdict(lambda: [0, 0.0, 0.0])
+    cyc = {}
+    with open(busy_csv) as fh:
+        rows = list(csv.DictReader(fh))
+    per = collections.defaultdict(dict)
+    meta = {}
+    for r in rows:
+        did = r.get("Dispatch_Id") or r.get("Correlation_Id")
+        per[did][r["Counter_Name"]] = float(r["Counter_Value"])
+        meta[did] = (short_name(r["Kernel_Name"]), int(float(r.get("Grid_Size") or 0)))
+    for did, c in per.items():
+        k, g = meta[did]
+        if k not in kernels:
+            continue
+        a = agg[(k, g)]
+        gg = c.get("GRBM_GUI_ACTIVE", 0.0)
+        a[0] += 1
+        a[1] += c.get("VALUBusy", 0.0) * gg
+        a[2] += gg
+    for key, (n, vg, g) in agg.items():
+        cyc[key] = (n, vg / g if g else 0.0)
+    return cyc
+
+
+def issue_markdown(busy_csv, insts_csv, command="", levels=()):
+    rows = issue_rows(busy_csv, insts_csv)
+    out = ["# Instruction issue (rocprofv3 --pmc VALUBusy SALUBusy GRBM_GUI_ACTIVE, then SQ instruction counts)", ""]
+    if command:
+        out += ["Commands:", "", "```", command, "```", ""]
+    out += ["| kernel | dispatches | active cycles share % | VALU busy % | SALU busy % | VALU / wave | SALU / wave "
+            "| LDS / wave | FP64 arith / wave | FP64 share of VALU % |",
+            "|---|---|---|---|---|---|---|---|---|---|"]
+    for r in rows[:16]:
+        out.append(f"| `{r['kernel'][:60]}` | {r['dispatches']} | {100 * r['cycles_share']:.1f} | {r['valu_busy']:.1f} "
+                   f"| {r['salu_busy']:.1f} | {r['valu_per_wave']:.0f} | {r['salu_per_wave']:.0f} "
+                   f"| {r['lds_per_wave']:.0f} | {r['f64_per_wave']:.0f} | {100 * r['f64_share']:.1f} |")
+    if levels:
+        lv = busy_by_grid(busy_csv, set(levels))
+        out += ["", "VALU busy % by launch width (one scan level per width):", "",
+                "| kernel | grid (threads) | dispatches | VALU busy % |", "|---|---|---|---|"]
+        for (k, g) in sorted(lv, key=lambda x: (x[0], -x[1])):
+            n, v = lv[(k, g)]
+            out.append(f"| `{k}` | {g} | {n} | {v:.1f} |")
+    return "\n".join(out) + "\n"
+
+
 def read_traffic(md_path, kernel):
     """Bytes per launch of the first kernel whose name contains ``kernel``, or None."""
     try:
@@ -202,8 +307,15 @@ def main(argv=None):
     m.add_argument("csv")
     m.add_argument("--stats", default=None)
     m.add_argument("--command", default="")
+    i = sub.add_parser("issue")
+    i.add_argument("--busy", required=True)
+    i.add_argument("--insts", required=True)
+    i.add_argument("--command", default="")
+    i.add_argument("--levels", nargs="*", default=[], help="kernel short names to split by launch width")
     a = ap.parse_args(argv)
-    if a.cmd == "stats":
+    if a.cmd == "issue":
+        sys.stdout.write(issue_markdown(a.busy, a.insts, a.command, a.levels))
+    elif a.cmd == "stats":
         sys.stdout.write(stats_markdown(a.csv, a.command))
     elif a.cmd == "mfma":
         sys.stdout.write(mfma_markdown(a.csv, a.stats, a.command))
