@@ -858,6 +858,7 @@ hipError_t launch_iteration(pof_batch* b, KernelTimer* tm) {
   return hipGetLastError();
 }
 
+// The caller has drained b->st (settle): no replay of the graph is in flight.
 void drop_graph(pof_batch* b) {
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   b->gexec = nullptr;
@@ -1006,7 +1007,11 @@ int batch_launch(pof_batch* b) {
   KernelTimer* tm = ctx->profiling ? &ctx->timer : nullptr;
   // an instrumented run times every launch from the host: no replay then
   const bool replay = ctx->graphs && !tm;
-  if (b->gexec && b->gmode != b->mode) drop_graph(b);
+  if (b->gexec && b->gmode != b->mode) {
+    // a replay of the old graph may still be in flight: drain before destroying it
+    if (settle(b) != POF_OK) return POF_E_HIP;
+    drop_graph(b);
+  }
   if (replay && !b->gexec && b->gstable >= kGraphAfter) HIP_TRY(capture_iteration(b));
   if (replay && b->gexec) {
     HIP_TRY(hipGraphLaunch(b->gexec, b->st));
