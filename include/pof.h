@@ -86,7 +86,10 @@ typedef struct {
  * reference (JAX runs one dtype); F32: the large-state engine forms the filter
  * and smoother combines' matrix-matrix products with FP32 MFMA GEMMs (FP32
  * accumulate), while every triangularisation, triangular solve, element and
- * observation statistic stays FP64. */
+ * observation statistic stays FP64.  Experimental: F32 needs calibrate = 0
+ * (POF_E_UNSUPPORTED otherwise; sigma_squared is not resolved in FP32) and its
+ * means miss the FP64 ones by about 1.3e-6 relative at D = 128, above the
+ * 1e-6 bar; the Python API refuses it. */
 #define POF_PRODUCTS_F64 0
 #define POF_PRODUCTS_F32 1
 

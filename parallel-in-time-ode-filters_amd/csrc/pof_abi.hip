@@ -429,6 +429,7 @@ struct pof_batch {
   hipStream_t st = nullptr;      // stream the batch works on
   hipStream_t own_st = nullptr;  // stream created with the batch
   hipEvent_t ev = nullptr;
+  hipEvent_t join_ev = nullptr;  // orders the batch's own stream with a caller's null stream
   std::vector<void*> allocs;
   pof::GroupArgs g{};
   // writable device tables (g holds the same pointers)
@@ -554,6 +555,7 @@ void batch_free(pof_batch* b) {
   if (b->scal_host) (void)hipHostFree(b->scal_host);
   if (b->maps_dev) (void)hipFree(b->maps_dev);
   if (b->ev) (void)hipEventDestroy(b->ev);
+  if (b->join_ev) (void)hipEventDestroy(b->join_ev);
   if (b->own_st) (void)hipStreamDestroy(b->own_st);
   delete b;
 }
@@ -615,6 +617,7 @@ int batch_new(pof_ctx* ctx, int D, int d, long long entries, int slots, bool aff
   if (he == hipSuccess) he = hipHostMalloc((void**)&b->scal_host, S * 8 * sizeof(double));
   if (he == hipSuccess) he = hipStreamCreateWithFlags(&b->own_st, hipStreamNonBlocking);
   if (he == hipSuccess) he = hipEventCreateWithFlags(&b->ev, hipEventDisableTiming);
+  if (he == hipSuccess) he = hipEventCreateWithFlags(&b->join_ev, hipEventDisableTiming);
   if (he != hipSuccess) {
     batch_free(b);
     return set_err(ctx, POF_E_HIP, "batch allocation failed: %s", hipGetErrorString(he));
@@ -1150,12 +1153,20 @@ int batch_finish(pof_batch* b, int max_done, long long* done, int* n_done) {
   return POF_OK;
 }
 
+void batch_abort(pof_batch* b);
+
 // The context's reusable batch for (D, d) with room for `entries` entries and `slots` IVPs.
+// A cached batch serves one call at a time and is empty between calls.
 int cached_batch(pof_ctx* ctx, int D, int d, long long entries, int slots, pof_batch** out) {
   for (size_t i = 0; i < ctx->cache.size(); ++i) {
     pof_batch* b = ctx->cache[i];
     if (b->D != D || b->d != d) continue;
     if (b->entries >= entries && b->slots >= slots) {
+      for (const auto& sl : b->sl)
+        if (sl.used) {
+          batch_abort(b);
+          break;
+        }
       *out = b;
       return POF_OK;
     }
@@ -1185,15 +1196,61 @@ int run_to_completion(pof_batch* b) {
   }
 }
 
-// make `b` work on the caller's stream for one call (ordering with the caller's work)
+// Make `b` work in the order of the caller's stream for one call.  A non-null
+// stream becomes the batch's stream.  The null stream (the default stream, which
+// torch uses unless told otherwise) is not captured into graphs and does not
+// order the batch's own non-blocking stream by itself, so the batch keeps its
+// own stream and is joined to the caller's with events: the work enqueued here
+// starts after everything the caller enqueued before the call, and the
+// caller's later work starts after the batch's.
 struct StreamScope {
   pof_batch* b;
   hipStream_t saved;
-  StreamScope(pof_batch* b_, void* stream) : b(b_), saved(b_->st) {
-    if (stream) b->st = (hipStream_t)stream;
+  hipStream_t caller;
+  hipError_t status = hipSuccess;  // of the entry join; check before enqueuing work
+  StreamScope(pof_batch* b_, void* stream) : b(b_), saved(b_->st), caller((hipStream_t)stream) {
+    if (caller) {
+      b->st = caller;
+      return;
+    }
+    status = hipEventRecord(b->join_ev, nullptr);
+    if (status == hipSuccess) status = hipStreamWaitEvent(b->st, b->join_ev, 0);
   }
-  ~StreamScope() { b->st = saved; }
+  ~StreamScope() {
+    if (!caller && hipEventRecord(b->join_ev, b->st) == hipSuccess)
+      (void)hipStreamWaitEvent(nullptr, b->join_ev, 0);
+    b->st = saved;
+  }
 };
+
+// Drop every IVP of b after a failed call (the cached batches of pof_solve,
+// pof_eks_solve, pof_ieks_step and pof_solve_batch serve one call at a time):
+// wait for its stream, errors ignored, then free every slot and entry range,
+// so that no later call runs a stale IVP or writes through its dangling
+// output pointers.  The per-slot device tables are rewritten at the next
+// admission of each slot.
+void batch_abort(pof_batch* b) {
+  (void)hipStreamSynchronize(b->st);
+  (void)hipGetLastError();
+  drop_graph(b);
+  b->gstable = 0;
+  b->launched = false;
+  b->unsynced = false;
+  b->need_sync = false;
+  for (auto& s : b->sl) s = pof_batch::Slot{};
+  std::fill(b->hwchol.begin(), b->hwchol.end(), 0);
+  b->free_.clear();
+  b->free_[0] = b->entries;
+  b->dirty_slots.clear();
+  b->init_host.clear();
+  b->init_dev.clear();
+  b->init_zero.clear();
+  b->init_zero_n.clear();
+  b->init_const.clear();
+  b->active.clear();
+  b->plan.m = 0;
+  b->plan_dirty = true;
+}
 
 // Single-pass sequential EKS of `prob` on batch `b` (pof/solver.py:76-96):
 // one sequential filter (EKF linearised at the predicted means) + smoother
@@ -1210,6 +1267,10 @@ int eks_run(pof_batch* b, const pof_problem* prob, const Setup& s, int calibrate
                      out, 2, 2, true, nullptr);
   if (rc != POF_OK) return rc;
   rc = run_to_completion(b);
+  if (rc != POF_OK) {
+    batch_abort(b);
+    return rc;
+  }
   out->iterations = 1.0;
   return rc;
 }
@@ -1493,6 +1554,12 @@ static int solve_large(pof_ctx* ctx, const pof_problem* prob, const pof_opts* op
   if (opts->sequential) return set_err(ctx, POF_E_UNSUPPORTED, "the large-state engine runs the parallel IEKS only");
   if (opts->products != POF_PRODUCTS_F64 && opts->products != POF_PRODUCTS_F32)
     return set_err(ctx, POF_E_ARG, "products must be POF_PRODUCTS_F64 or POF_PRODUCTS_F32");
+  // FP32 combine products leave rounding of ~1e-7 relative in the filtered means; sigma_squared (the
+  // mean square of the whitened one-step residuals, tiny near convergence) is then rounding noise, and
+  // with it every calibrated factor
+  if (opts->products == POF_PRODUCTS_F32 && opts->calibrate)
+    return set_err(ctx, POF_E_UNSUPPORTED,
+                   "FP32 combine products do not resolve sigma_squared: use POF_PRODUCTS_F64 to calibrate");
   if (s.Ns > 65534) return set_err(ctx, POF_E_ARG, "the large-state engine takes at most 65534 steps");
   std::vector<double> m0;
   rc = (prob->init == POF_INIT_COARSE) ? POF_E_UNSUPPORTED : pof::initial_means(prob, s, m0, ctx->err);
@@ -1541,9 +1608,11 @@ int pof_solve(pof_ctx* ctx, const pof_problem* prob, const pof_opts* opts, pof_o
   rc = cached_batch(ctx, s.D, s.d, entries_for(prob->n_times), 1, &b);
   if (rc != POF_OK) return rc;
   StreamScope scope(b, stream);
+  HIP_TRY(scope.status);
   rc = admit_problem(b, prob, opts, out, nullptr);
-  if (rc != POF_OK) return rc;
-  return run_to_completion(b);
+  if (rc == POF_OK) rc = run_to_completion(b);
+  if (rc != POF_OK) batch_abort(b);
+  return rc;
 }
 
 int pof_coarse_means(pof_ctx* ctx, const pof_problem* prob, double* means_host) {
@@ -1569,6 +1638,7 @@ int pof_eks_solve(pof_ctx* ctx, const pof_problem* prob, int calibrate, pof_out*
   rc = cached_batch(ctx, s.D, s.d, entries_for(prob->n_times), 1, &b);
   if (rc != POF_OK) return rc;
   StreamScope scope(b, stream);
+  HIP_TRY(scope.status);
   return eks_run(b, prob, s, calibrate, out);
 }
 
@@ -1583,6 +1653,7 @@ int pof_ieks_step(pof_ctx* ctx, const pof_problem* prob, const double* states_me
   rc = cached_batch(ctx, s.D, s.d, entries_for(prob->n_times), 1, &b);
   if (rc != POF_OK) return rc;
   StreamScope scope(b, stream);
+  HIP_TRY(scope.status);
   pof_opts o{};
   o.maxiters = 1;
   o.fixed_iters = 1;
@@ -1593,9 +1664,11 @@ int pof_ieks_step(pof_ctx* ctx, const pof_problem* prob, const double* states_me
   o.sequential = sequential ? 1 : 0;
   rc = admit_raw(b, s.Ns, s.P.data(), s.x0m.data(), s.x0L.data(), prob->kind, prob->params, nullptr,
                  states_mean_dev, &o, &po, 0, o.sequential, false, nullptr);
-  if (rc != POF_OK) return rc;
-  rc = run_to_completion(b);
-  if (rc != POF_OK) return rc;
+  if (rc == POF_OK) rc = run_to_completion(b);
+  if (rc != POF_OK) {
+    batch_abort(b);
+    return rc;
+  }
   if (nll) *nll = po.nll;
   if (obj) *obj = po.obj;
   if (ssq) *ssq = po.sigma_squared;
@@ -1641,7 +1714,8 @@ int pof_smoothing(pof_ctx* ctx, int D, int64_t Ns, const double* F_host, const d
     pof_batch* b;
     ~Guard() { batch_free(b); }
   } guard{b};
-  if (stream) b->st = (hipStream_t)stream;
+  StreamScope scope(b, stream);
+  HIP_TRY(scope.status);
   const size_t DD = (size_t)D * D;
   HIP_TRY(upload_transition(b, F_host, QL_host));
   // state 0 enters as the prior of the slot, states 1..Ns as the level-0 filter prefixes
@@ -1686,7 +1760,8 @@ int pof_ekf(pof_ctx* ctx, const pof_problem* prob, const double* x0_mean_host, c
     pof_batch* b;
     ~Guard() { batch_free(b); }
   } guard{b};
-  if (stream) b->st = (hipStream_t)stream;
+  StreamScope scope(b, stream);
+  HIP_TRY(scope.status);
   const int D = s.D;
   const size_t DD = (size_t)D * D;
   HIP_TRY(upload_transition(b, F_host, QL_host));
@@ -1727,7 +1802,8 @@ int pof_linear_filtsmooth(pof_ctx* ctx, int D, int d, int64_t Ns, const double* 
     pof_batch* b;
     ~Guard() { batch_free(b); }
   } guard{b};
-  if (stream) b->st = (hipStream_t)stream;
+  StreamScope scope(b, stream);
+  HIP_TRY(scope.status);
   // the caller's model: F, QL, N1 = tria([0 | QL]), x0; no preconditioning (P = 1)
   const size_t DD = (size_t)D * D;
   HIP_TRY(upload_transition(b, F_host, QL_host));
@@ -1847,9 +1923,16 @@ int pof_solve_batch(pof_ctx* ctx, const pof_problem* probs, const pof_opts* opts
     int rc = cached_batch(ctx, g.D, g.d, g.entries, g.count, &g.b);
     if (rc != POF_OK) return rc;
   }
+  // on any failure after the first admission, every group's batch is emptied
+  // (a later call must not run this call's IVPs)
+  auto fail = [&](int rc) {
+    for (auto& g : groups)
+      if (g.b) batch_abort(g.b);
+    return rc;
+  };
   for (int i = 0; i < n; ++i) {
     int rc = admit_problem(groups[gid[i]].b, &probs[i], &opts[i], &outs[i], nullptr);
-    if (rc != POF_OK) return rc;
+    if (rc != POF_OK) return fail(rc);
   }
   // iterate every group on its own stream, each at its own pace: a group's next
   // iteration is enqueued as soon as its previous one has been finished, so a
@@ -1857,7 +1940,7 @@ int pof_solve_batch(pof_ctx* ctx, const pof_problem* probs, const pof_opts* opts
   for (auto& g : groups) {
     if (pof_batch_active(g.b) == 0) continue;
     int rc = batch_launch(g.b);
-    if (rc != POF_OK) return rc;
+    if (rc != POF_OK) return fail(rc);
   }
   for (;;) {
     bool any = false;
@@ -1867,13 +1950,13 @@ int pof_solve_batch(pof_ctx* ctx, const pof_problem* probs, const pof_opts* opts
       if (g.b->need_sync) {
         const hipError_t q = hipEventQuery(g.b->ev);
         if (q == hipErrorNotReady) continue;
-        if (q != hipSuccess) return set_err(ctx, POF_E_HIP, "hipEventQuery: %s", hipGetErrorString(q));
+        if (q != hipSuccess) return fail(set_err(ctx, POF_E_HIP, "hipEventQuery: %s", hipGetErrorString(q)));
       }
       int rc = batch_finish(g.b, 0, nullptr, nullptr);
-      if (rc != POF_OK) return rc;
+      if (rc != POF_OK) return fail(rc);
       if (pof_batch_active(g.b) > 0) {
         rc = batch_launch(g.b);
-        if (rc != POF_OK) return rc;
+        if (rc != POF_OK) return fail(rc);
       }
     }
     if (!any) break;

@@ -109,27 +109,49 @@ def test_lorenz96_matches_oracle(gpu, d, q, N, fixed):
     assert abs(info["obj"] - rinfo["obj"]) <= 1e-6 * abs(rinfo["obj"]) + 1e-12
 
 
-def test_lorenz96_fp32_combine_products(gpu):
-    """products="float32" (the scan combines' matrix products as FP32 MFMA GEMMs, everything else
-    FP64) on the D = 128 configuration: means and L L^T / ssq^2 stay within 1e-5 of the FP64 solve.
-    sigma_squared is not compared: whitening the near-zero residual with S^-T amplifies the FP32
-    rounding (measured off by orders of magnitude), so the mode is an experiment, not a default."""
+def test_lorenz96_fp32_combine_products_fenced(gpu):
+    """FP32 combine products (the scan combines' matrix products as FP32 MFMA GEMMs, everything
+    else FP64) are refused by the public API: the means miss the 1e-6 bar against the FP64 oracle
+    and sigma_squared, which scales every returned factor, is rounding noise in FP32.  The C ABI
+    keeps them as an experiment with calibrate = 0 only; their means are compared here with the
+    FP64 oracle directly (no HIP-to-HIP comparison), at the 1e-5 they reach, which is why the
+    public API refuses them."""
+    import ctypes
+
     import torch
 
     import pof.ivp
     import pof.solver
+    from oracle import problems as op
+    from oracle import solver as osolver
+    from pof import _lib
+    from pof.convenience import ProblemSpec
 
     ivp = pof.ivp.lorenz96(d=32)
     ts = np.linspace(0.0, 16 * 5.0 / 8192, 17)
     kw = dict(f=ivp.f, y0=ivp.y0, ts=ts, order=3, init="constant", fixed_iters=3)
-    y64, i64 = pof.solver.solve(engine="large", **kw)
-    y32, i32 = pof.solver.solve(products="float32", **kw)
-    torch.cuda.synchronize()
-    m64, m32 = y64.mean.cpu().numpy(), y32.mean.cpu().numpy()
-    print(f"fp32 combine products: means {rel(m32, m64):.2e}", flush=True)
-    assert rel(m32, m64) < 1e-5
-    c64 = cov(y64.chol.cpu().numpy()) / i64["sigma_squared"] ** 2
-    c32 = cov(y32.chol.cpu().numpy()) / i32["sigma_squared"] ** 2
-    assert rel(c32, c64) < 1e-5
+    with pytest.raises(NotImplementedError):
+        pof.solver.solve(products="float32", **kw)
     with pytest.raises(ValueError):
         pof.solver.solve(products="float16", **kw)
+    lib = _lib.load()
+    ctx = _lib.context(0)
+    spec = ProblemSpec(ivp.f, ivp.y0, ts, 3, "constant")
+    prob = spec.cstruct()
+    N, d, D = len(ts), 32, 128
+    means = torch.empty((N, d), dtype=torch.float64, device="cuda")
+    out = _lib.Out()
+    out.means_dev = means.data_ptr()
+    opts = _lib.Opts()
+    opts.maxiters, opts.fixed_iters, opts.products = 10, 3, _lib.PRODUCTS_F32
+    opts.calibrate = 1
+    rc = lib.pof_solve_large(ctx, ctypes.byref(prob), ctypes.byref(opts), ctypes.byref(out), _lib.stream_ptr())
+    assert rc == _lib.POF_E_UNSUPPORTED
+    opts.calibrate = 0
+    rc = lib.pof_solve_large(ctx, ctypes.byref(prob), ctypes.byref(opts), ctypes.byref(out), _lib.stream_ptr())
+    _lib.check(rc, ctx, "pof_solve_large")
+    torch.cuda.synchronize()
+    ref, _ = osolver.solve(op.lorenz96(d=32), ts, 3, init="constant", fixed_iters=3)
+    err = rel(means.cpu().numpy(), ref.mean)
+    print(f"fp32 combine products: means vs the FP64 oracle {err:.2e}", flush=True)
+    assert err < 1e-5
