@@ -31,8 +31,6 @@ def _mix(seed, n, nmax=300):
         y0 = ivp.y0 + 0.1 * rng.standard_normal(ivp.y0.shape)
         N = int(rng.integers(16, nmax))
         q = int(rng.integers(1, 5))
-        if ivp.f.name == "lorenz63" and q == 4:
-            q = 3  # (D, d) = (15, 3) is the largest d = 3 table entry used here
         ts = np.linspace(ivp.t0, ivp.tmax, N)
         probs.append((ivp.f, y0, ts, q, "constant"))
     return probs
@@ -61,27 +59,59 @@ def test_batch_equals_single_solves(gpu):
         assert info["nll"] == info1["nll"] and info["obj"] == info1["obj"]
 
 
-def test_batch_members_match_oracle(gpu):
-    import pof.batch
+def _oracle_err(f, y0, ts, q, init, m, **kw):
     from oracle import problems as op
     from oracle import solver as osolver
 
-    # non-chaotic members only: converged iterates, so rounding stays at the 1e-12 level
-    probs = [p for p in _mix(7, 12, nmax=120) if p[0].name != "lorenz63"]
+    oprob = op.Problem(f.name, f.params, y0, ts[0], ts[-1])
+    (om, _), oinfo = osolver.solve(oprob, ts, q, init=init, **kw)
+    return np.abs(m - om).max() / max(np.abs(om).max(), 1e-300), oinfo
+
+
+def test_batch_members_match_oracle(gpu):
+    """Every member of a mixed batch (logistic, FHN, Lorenz-63 with q up to 4, i.e. D up to 15)
+    against the oracle: iterations exact, means to 1e-9 (1e-8 for q = 4)."""
+    import pof.batch
+
+    probs = _mix(7, 12, nmax=120)
+    assert any(p[0].name == "lorenz63" and p[3] == 4 for p in probs)
     res = pof.batch.solve_batch(probs, maxiters=60)
     for (f, y0, ts, q, init), (ys, info) in zip(probs, res):
-        oprob = op.Problem(f.name, f.params, y0, ts[0], ts[-1])
-        (om, _), oinfo = osolver.solve(oprob, ts, q, init=init, maxiters=60)
-        m = ys.mean.cpu().numpy()
+        print(f"member {f.name} q={q} N={len(ts)}", flush=True)
+        err, oinfo = _oracle_err(f, y0, ts, q, init, ys.mean.cpu().numpy(), maxiters=60)
         assert info["iterations"] == oinfo["iterations"], (f.name, q, len(ts))
         if oinfo["iterations"] > 60:
             # not converged (e.g. a logistic y0 < 0 blows up in finite time): the
             # iterates amplify rounding, the oracle's own results move by ~1e-9
             # under a 1e-15 change of y0; only the iteration count is compared
             continue
-        err = np.abs(m - om).max() / max(np.abs(om).max(), 1e-300)
         # q = 4 priors (cond(Q) = 4.8e5) amplify rounding to ~5e-9; the acceptance bar is 1e-6
         assert err < (1e-8 if q == 4 else 1e-9), (f.name, q, len(ts), err)
+
+
+def test_batch_lorenz_members_fixed_iterations_match_oracle(gpu):
+    """Lorenz-63 members of q = 3 and 4 (D = 12 and 15) over the full t in [0, 10], three fixed
+    iterations (unconverged iterates of a chaotic field), batched with logistic and FHN members,
+    against the oracle: means to the 1e-6 acceptance bar."""
+    import pof.batch
+    import pof.ivp
+
+    rng = np.random.default_rng(3)
+    probs = []
+    for q in (3, 4, 4, 3):
+        ivp = pof.ivp.lorenz63(rho=float(rng.uniform(20, 30)))
+        probs.append((ivp.f, ivp.y0 + 0.1 * rng.standard_normal(3), np.linspace(0.0, 10.0, int(rng.integers(129, 300))),
+                      q, "constant"))
+    lg = pof.ivp.logistic(r=2.0)
+    probs.append((lg.f, lg.y0, np.linspace(0.0, 10.0, 64), 4, "constant"))
+    fh = pof.ivp.fhn_classic()
+    probs.append((fh.f, fh.y0, np.linspace(0.0, 20.0, 100), 2, "constant"))
+    res = pof.batch.solve_batch(probs, fixed_iters=3)
+    for (f, y0, ts, q, init), (ys, info) in zip(probs, res):
+        print(f"member {f.name} q={q} N={len(ts)}", flush=True)
+        assert info["iterations"] == 3.0 and info["status"] == "fixed"
+        err, _ = _oracle_err(f, y0, ts, q, init, ys.mean.cpu().numpy(), fixed_iters=3)
+        assert err < 1e-6, (f.name, q, len(ts), err)
 
 
 def test_fixed_iterations_batch(gpu):
@@ -212,6 +242,12 @@ def test_server_results_same_as_alone(gpu):
         assert r.iterations == info["iterations"], r.rid
         assert np.array_equal(r.means, ys.mean.cpu().numpy()), r.rid
         assert r.done >= r.admitted >= r.arrival
+        # and every served result against the oracle (iterations exact; means as in
+        # test_batch_members_match_oracle, for the members that converged)
+        err, oinfo = _oracle_err(f, y0, ts, q, init, r.means, maxiters=60)
+        assert r.iterations == oinfo["iterations"], r.rid
+        if oinfo["iterations"] <= 60:
+            assert err < (1e-8 if q == 4 else 1e-9), (r.rid, f.name, q, len(ts), err)
 
 
 def test_server_rejects_a_request_that_cannot_fit(gpu):
@@ -246,3 +282,41 @@ def test_tiny_ivps_in_a_batch_same_as_alone(gpu):
         ys1, info1 = pof.solver.solve(f=f, y0=y0, ts=ts, order=q, init=init, maxiters=60)
         assert info["iterations"] == info1["iterations"]
         assert _equal(ys.mean, ys1.mean) and _equal(ys.chol, ys1.chol)
+
+
+def test_nan_request_is_isolated(gpu):
+    """NaN is the reference's only failure signal (pof/convergence_criteria.py:6): an IVP with a
+    NaN initial value ends after its first iteration with status "nan", in a batch and in a
+    served stream, and every other result, including those of its own (d, q) group, stays bitwise
+    equal to its solve alone."""
+    import pof.batch
+    import pof.ivp
+    import pof.serve
+    import pof.solver
+
+    rng = np.random.default_rng(17)
+    probs = _mix(13, 8, nmax=200)
+    for _ in range(4):
+        ivp = pof.ivp.fhn_classic(a=float(rng.uniform(0.1, 0.3)), b=float(rng.uniform(0.1, 0.3)),
+                                  c=float(rng.uniform(2, 4)))
+        probs.append((ivp.f, ivp.y0 + 0.1 * rng.standard_normal(2), np.linspace(0.0, 20.0, 150), 3, "constant"))
+    bad = 9
+    f, y0, ts, q, init = probs[bad]
+    probs[bad] = (f, np.full_like(np.asarray(y0, dtype=np.float64), np.nan), ts, q, init)
+    alone = [pof.solver.solve(f=f, y0=y0, ts=ts, order=q, init=init, maxiters=60) for f, y0, ts, q, init in probs]
+    assert alone[bad][1]["status"] == "nan" and alone[bad][1]["iterations"] == 1.0
+
+    res = pof.batch.solve_batch(probs, maxiters=60)
+    srv = pof.serve.Server(budget_bytes=2 << 30, n_groups=4, slots=8, maxiters=60)
+    recs = srv.run(probs, np.linspace(0.0, 0.02, len(probs)))
+    srv.close()
+    for i, ((ys1, info1), (ys, info), r) in enumerate(zip(alone, res, recs)):
+        if i == bad:
+            assert info["status"] == "nan" and info["iterations"] == 1.0
+            assert r.status == "nan" and r.iterations == 1.0
+            continue
+        assert info["status"] == info1["status"] != "nan", i
+        assert info["iterations"] == info1["iterations"], i
+        assert _equal(ys.mean, ys1.mean) and _equal(ys.chol, ys1.chol), i
+        assert r.status == info1["status"] and r.iterations == info1["iterations"], i
+        assert np.array_equal(r.means, ys1.mean.cpu().numpy()), i

@@ -65,6 +65,10 @@ CASES = [
     ("vanderpol", {"stiffness_constant": 1.0}, 128, 2, "constant"),
     ("henonheiles", {"tmax": 10.0}, 32, 2, "constant"),
     ("lotkavolterra", {}, 50, 2, "prior"),
+    # d = 3, q = 4: D = 15, the largest state of the batched mix (its own kernel instantiations)
+    ("lorenz63", {"tmax": 1.0}, 129, 4, "constant"),
+    ("lorenz63", {"tmax": 0.5}, 65, 4, "prior"),
+    ("rigid_body", {}, 128, 4, "constant"),
 ]
 
 
@@ -94,7 +98,8 @@ def test_solve_matches_oracle(gpu, name, kw, N, q, init):
     assert ys.chol.shape == (N, oivp.d, oivp.d * (q + 1))
 
 
-def test_fixed_iterations_lorenz63(gpu):
+@pytest.mark.parametrize("q", [3, 4])
+def test_fixed_iterations_lorenz63(gpu, q):
     import torch
 
     import pof.ivp
@@ -104,15 +109,16 @@ def test_fixed_iterations_lorenz63(gpu):
 
     ts = np.linspace(0.0, 10.0, 257)
     ivp = pof.ivp.lorenz63()
-    ys, info = pof.solver.solve(f=ivp.f, y0=ivp.y0, ts=ts, order=3, init="constant", fixed_iters=4)
+    print(f"lorenz63 N=257 q={q} 4 fixed iterations", flush=True)
+    ys, info = pof.solver.solve(f=ivp.f, y0=ivp.y0, ts=ts, order=q, init="constant", fixed_iters=4)
     torch.cuda.synchronize()
-    ref, rinfo = osolver.solve(op.lorenz63(), ts, 3, init="constant", fixed_iters=4)
+    ref, rinfo = osolver.solve(op.lorenz63(), ts, q, init="constant", fixed_iters=4)
     assert info["iterations"] == 4.0
     # chaotic field, unconverged iterates: rounding is amplified; acceptance bar 1e-6
     assert rel(ys.mean.cpu().numpy(), ref.mean) < 1e-6
     s_gpu, s_ref = info["sigma_squared"], rinfo["sigma_squared"]
     assert rel(cov(ys.chol.cpu().numpy()) / s_gpu**2, cov(ref.chol) / s_ref**2) < COV_TOL
-    assert abs(s_gpu - s_ref) <= ssq_tolerance(op.lorenz63(), ts, 3, "constant", s_ref, fixed_iters=4)
+    assert abs(s_gpu - s_ref) <= ssq_tolerance(op.lorenz63(), ts, q, "constant", s_ref, fixed_iters=4)
 
 
 def test_linear_filtsmooth_random_model(gpu):
