@@ -204,6 +204,21 @@ int pof_linearize_model(pof_ctx* ctx, const pof_problem* prob, int method, const
                         const double* m_dev, const double* L_dev, int64_t n, double* H_dev, double* c_dev,
                         double* R_dev, void* stream);
 
+/* pof_linearize_model with no device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, kernels and results as pof_linearize_model, but its device
+ * buffer (P, then a work slice per state for UNCERTAIN / UNSCENTED: 362
+ * doubles per state at D = 15, d = 3) is a library-owned 64 MiB array that
+ * each GPU holds until the process ends, instead of a hipMalloc per call.
+ * POF_E_ARG when n needs more than that array (n <= 23,000 or so at D = 15 for
+ * UNCERTAIN / UNSCENTED; EK1, EK0 and REGULARIZED need no per-state scratch).
+ * Every call on one GPU shares the array: calls that overlap on one GPU
+ * (several host threads or streams at once) race. */
+int pof_linearize_model_noalloc(pof_ctx* ctx, const pof_problem* prob, int method, const double* params,
+                                const double* m_dev, const double* L_dev, int64_t n, double* H_dev, double* c_dev,
+                                double* R_dev, void* stream);
+
 /* One IEKS iteration.  Replaces pof.step.ieks_step (step.py:33-45): linearise at
  * states_dev [Ns+1, D], parallel (sequential = 0) or sequential (1) filter +
  * smoother, chol *= sqrt(ssq).  Outputs full states [Ns+1, D] / [Ns+1, D, D]

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -396,6 +397,10 @@ struct pof_ctx {
   pof::large::Engine* large = nullptr;
   // replay unchanged iterations from a HIP graph (pof_graphs_enable)
   bool graphs = true;
+  // this device's copy of the file-scope scratch array (pof_linearize_model_noalloc), or null with
+  // scratch_err saying why its address could not be taken
+  double* scratch = nullptr;
+  std::string scratch_err;
 };
 
 // A group of IVPs with one (D, d) sharing one device workspace.  IVP slots are
@@ -469,6 +474,18 @@ struct pof_batch {
   int gmode = -1;
   int gstable = 0;  // launches since the plan was last rebuilt
 };
+
+// ------------------------------------------------------ no-allocation scratch
+// Device scratch of pof_linearize_model_noalloc: the Nordsieck scaling P and
+// the per-state work slices of the linearisation, in a file-scope array that
+// every GPU the library runs on holds from the load of the library to the end
+// of the process.  Host code never passes `scratch` itself (a host-side
+// stand-in): it passes the device address that hipGetSymbolAddress returns for
+// the current device, looked up once per device when a context is created.
+// All calls on one GPU share the array and it keeps the last call's contents;
+// calls that overlap on one GPU race.
+constexpr size_t kScratchDoubles = size_t(8) << 20;  // 64 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
 
 namespace {
 
@@ -1357,6 +1374,33 @@ int admit_problem(pof_batch* b, const pof_problem* prob, const pof_opts* opts, p
                    0, opts->sequential ? 1 : 0, false, ticket);
 }
 
+std::mutex g_scratch_mu;
+std::map<int, double*> g_scratch_ptr;
+
+// The device address of `scratch` on `device` (which must be current).
+hipError_t scratch_address(int device, double** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(device);
+  if (it != g_scratch_ptr.end()) {
+    *out = it->second;
+    return hipSuccess;
+  }
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch_ptr[device] = static_cast<double*>(p);
+  *out = static_cast<double*>(p);
+  return hipSuccess;
+}
+
+// Doubles of device buffer that pof_linearize_model(_noalloc) uses for n states:
+// P (D, padded to 32 doubles), then n work slices for the uncertain / unscented methods.
+size_t lin_buffer_doubles(int D, int d, int method, long long n) {
+  const bool needs_L = method == POF_LIN_UNCERTAIN || method == POF_LIN_UNSCENTED;
+  const size_t head = ((size_t)D + 31) / 32 * 32;
+  return head + (needs_L ? (size_t)n * (size_t)lin_work_host(D, d) : 0);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1370,6 +1414,11 @@ int pof_ctx_create(int device, pof_ctx** out) {
   if (hipSetDevice(device) != hipSuccess) return POF_E_HIP;
   pof_ctx* c = new pof_ctx();
   c->device = device;
+  const hipError_t se = scratch_address(device, &c->scratch);
+  if (se != hipSuccess) {
+    c->scratch = nullptr;
+    c->scratch_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(se);
+  }
   *out = c;
   return POF_OK;
 }
@@ -1502,9 +1551,14 @@ int pof_linearize(pof_ctx* ctx, const pof_problem* prob, const double* m_dev, in
   return POF_OK;
 }
 
-int pof_linearize_model(pof_ctx* ctx, const pof_problem* prob, int method, const double* params,
-                        const double* m_dev, const double* L_dev, int64_t n, double* H_dev, double* c_dev,
-                        double* R_dev, void* stream) {
+}  // extern "C"
+
+namespace {
+// pof_linearize_model on a device buffer of lin_buffer_doubles(...) doubles:
+// `buf` (the no-allocation variant's scratch), or null to allocate one for the call.
+int linearize_model(pof_ctx* ctx, const pof_problem* prob, int method, const double* params, const double* m_dev,
+                    const double* L_dev, int64_t n, double* H_dev, double* c_dev, double* R_dev, void* stream,
+                    double* buf, size_t buf_doubles) {
   if (!ctx) return POF_E_ARG;
   pof::Setup s;
   int rc = pof::build_setup(prob, s, ctx->err);
@@ -1522,14 +1576,22 @@ int pof_linearize_model(pof_ctx* ctx, const pof_problem* prob, int method, const
   if (method == POF_LIN_REGULARIZED && !(lp.lam > 0.0)) return set_err(ctx, POF_E_ARG, "lam must be > 0");
   const pof::EngineFns* e = engine_or_err(ctx, s.D, s.d);
   if (!e) return POF_E_UNSUPPORTED;
+  const size_t need = lin_buffer_doubles(s.D, s.d, method, n);
+  if (buf && need > buf_doubles)
+    return set_err(ctx, POF_E_ARG,
+                   "n = %lld states need %zu doubles of scratch; the no-allocation scratch holds %zu (n <= %lld here)",
+                   (long long)n, need, buf_doubles,
+                   needs_L ? (long long)((buf_doubles - lin_buffer_doubles(s.D, s.d, method, 0)) /
+                                         (size_t)lin_work_host(s.D, s.d))
+                           : 0x3fffffffLL);
   if (n == 0) return POF_OK;
   hipStream_t st = (hipStream_t)stream;
   HIP_TRY(hipSetDevice(ctx->device));
-  // Pd, then the per-state work slices (uncertain / unscented)
-  const size_t wper = (size_t)pof::lin_work_host(s.D, s.d);
-  const size_t wn = needs_L ? (size_t)n * wper : 0;
-  double* buf = nullptr;
-  HIP_TRY(hipMalloc((void**)&buf, (s.D + wn) * sizeof(double)));
+  const bool own = buf == nullptr;
+  if (own) HIP_TRY(hipMalloc((void**)&buf, need * sizeof(double)));
+  // P, then the per-state work slices (uncertain / unscented); each slice is
+  // written before it is read, so nothing needs clearing
+  const size_t head = lin_buffer_doubles(s.D, s.d, POF_LIN_EK1, 0);
   hipError_t he = hipMemcpyAsync(buf, s.P.data(), s.D * sizeof(double), hipMemcpyHostToDevice, st);
   pof::ModelArgs a{};
   a.Pd = buf;
@@ -1538,11 +1600,31 @@ int pof_linearize_model(pof_ctx* ctx, const pof_problem* prob, int method, const
   a.Ns = (int)n;
   for (int i = 0; i < pof::kMaxParams; ++i) a.params[i] = prob->params[i];
   if (he == hipSuccess)
-    he = e->linearize_alt(a, method, lp, m_dev, L_dev, (int)n, H_dev, c_dev, R_dev, wn ? buf + s.D : nullptr, st);
+    he = e->linearize_alt(a, method, lp, m_dev, L_dev, (int)n, H_dev, c_dev, R_dev, needs_L ? buf + head : nullptr,
+                          st);
+  // the host copy of P must outlive its transfer, and the call is complete on return
   if (he == hipSuccess) he = hipStreamSynchronize(st);
-  (void)hipFree(buf);
+  if (own) (void)hipFree(buf);
   if (he != hipSuccess) return set_err(ctx, POF_E_HIP, "linearize_model failed: %s", hipGetErrorString(he));
   return POF_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int pof_linearize_model(pof_ctx* ctx, const pof_problem* prob, int method, const double* params,
+                        const double* m_dev, const double* L_dev, int64_t n, double* H_dev, double* c_dev,
+                        double* R_dev, void* stream) {
+  return linearize_model(ctx, prob, method, params, m_dev, L_dev, n, H_dev, c_dev, R_dev, stream, nullptr, 0);
+}
+
+int pof_linearize_model_noalloc(pof_ctx* ctx, const pof_problem* prob, int method, const double* params,
+                                const double* m_dev, const double* L_dev, int64_t n, double* H_dev, double* c_dev,
+                                double* R_dev, void* stream) {
+  if (!ctx) return POF_E_ARG;
+  if (!ctx->scratch) return set_err(ctx, POF_E_HIP, "no-allocation scratch unavailable: %s", ctx->scratch_err.c_str());
+  return linearize_model(ctx, prob, method, params, m_dev, L_dev, n, H_dev, c_dev, R_dev, stream, ctx->scratch,
+                         kScratchDoubles);
 }
 
 // The whole solve on the large-state engine (batched primitives); `small` is the

@@ -107,6 +107,8 @@ SIGNATURES = {
     "pof_linearize": (ctypes.c_int, [_vp, _P(Problem), _vp, ctypes.c_int64, _vp, _vp, _vp]),
     "pof_linearize_model": (ctypes.c_int, [_vp, _P(Problem), ctypes.c_int, _dp, _vp, _vp, ctypes.c_int64, _vp, _vp,
                                            _vp, _vp]),
+    "pof_linearize_model_noalloc": (ctypes.c_int, [_vp, _P(Problem), ctypes.c_int, _dp, _vp, _vp, ctypes.c_int64, _vp,
+                                                   _vp, _vp, _vp]),
     "pof_ieks_step": (ctypes.c_int, [_vp, _P(Problem), _vp, ctypes.c_int, _vp, _vp, _dp, _dp, _dp, _vp]),
     "pof_linear_filtsmooth": (
         ctypes.c_int,
