@@ -184,6 +184,17 @@ def plain_checks(prefix=512, maxiters=200):
         "maxiters": maxiters,
         "finite": bool(torch.isfinite(ys.mean).all()),
     }
+    # the configuration is unstable for the parallel method itself: after two iterations the parallel
+    # iterates are far from the sequential ones (the same linear model, another association order)
+    y2, _ = pof.solver.solve(f=ivp.f, y0=ivp.y0, ts=ts, order=Q, init="constant", fixed_iters=2)
+    s2, _ = pof.solver.solve(f=ivp.f, y0=ivp.y0, ts=ts, order=Q, init="constant", fixed_iters=2, sequential=True)
+    torch.cuda.synchronize()
+    conv["stability"] = {
+        "what": "max |mean| after 2 fixed iterations on the full grid: parallel scan vs the sequential filter and "
+                "smoother (the same linear models; they differ only by rounding and its amplification)",
+        "parallel_max_abs_mean": float(y2.mean.abs().max()),
+        "sequential_max_abs_mean": float(s2.mean.abs().max()),
+    }
     tp = ts[: prefix + 1]
     yp, ip = pof.solver.solve(f=ivp.f, y0=ivp.y0, ts=tp, order=Q, init="constant", fixed_iters=ITERS)
     torch.cuda.synchronize()
@@ -206,52 +217,170 @@ def plain_checks(prefix=512, maxiters=200):
     return conv, acc
 
 
-def cpu_baseline():
-    """The oracle (NumPy FP64 parallel-scan IEKS) on one full-size iteration, one thread."""
+def known_answers():
+    """The reference's own full-size run at the benchmark's state dimension and length: rigid body
+    (d = 3), N = 65536 (65535 steps), q = 1 and 3 (D = 6 and 12), constant init, maxiters = 1000,
+    against experiments/3_work_precision_diagram/data/rigidbody_Tesla_V100-SXM2-32GB.csv (FP64 on a
+    V100): iteration counts, and rmse_final / rmse_traj against a DOP853 reference solution
+    (rtol 1e-13).  For q = 3 both rmse sit at the reference solution's error floor (~1e-11)."""
+    import csv
+
+    import numpy as np
+    import torch
+    from scipy.integrate import solve_ivp
+
+    import pof.ivp
+    import pof.solver
+    from oracle import problems as op
+
+    rows = {}
+    with open(ROOT / "tests" / "golden" / "work_precision_pins_large.csv") as f:
+        for r in csv.DictReader(f):
+            if r["problem"] == "rigidbody" and r["N"] == "65536":
+                rows[int(r["q"])] = r
+    ivp = pof.ivp.rigid_body()
+    p = op.rigid_body()
+    ts = np.linspace(ivp.t0, ivp.tmax, 65536)
+    sol = solve_ivp(lambda _, y: p.f(None, y), p.t_span, p.y0, method="DOP853", rtol=1e-13, atol=1e-15,
+                    dense_output=True)
+    yref = sol.sol(ts).T
+    out = {"workload": "rigid body, t in [0, 20], N = 65536, constant init, maxiters 1000, f64",
+           "source": "rigidbody_Tesla_V100-SXM2-32GB.csv:11 (reference, FP64, V100)"}
+    for q in (1, 3):
+        t0 = time.perf_counter()
+        ys, info = pof.solver.solve(f=ivp.f, y0=ivp.y0, ts=ts, order=q, init="constant", maxiters=1000)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        m = ys.mean.cpu().numpy()
+        r = rows[q]
+        out[f"q{q}"] = {
+            "D": 3 * (q + 1),
+            "iterations": info["iterations"],
+            "iterations_reference": float(r["iterations"]),
+            "rmse_final": float(np.linalg.norm(m[-1] - yref[-1])),
+            "rmse_final_reference": float(r["rmse_final"]),
+            "rmse_traj": float(np.linalg.norm(m - yref, axis=1).mean()),
+            "rmse_traj_reference": float(r["rmse_traj"]),
+            "seconds": dt,
+            "reference_seconds_v100": float(r["runtime"]),
+        }
+    return out
+
+
+def small_state_run(ns=65536, iters=10):
+    """Plain parallel IEKS at the smallest state of the north-star HBM target: logistic q = 3
+    (d = 1, D = 4), Ns = 2^16, 10 fixed iterations.  Reports ms per iteration and the
+    element-once HBM fraction (bytes s Ns (3 D^2 + 4 D) per iteration at 8 TB/s), and the
+    per-class kernel times of one instrumented solve."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    import pof.ivp
+    import pof.solver
+    from pof import _lib, roofline
+
+    ivp = pof.ivp.logistic()
+    ts = np.linspace(0.0, 10.0, ns + 1)
+
+    def solve():
+        return pof.solver.solve(f=ivp.f, y0=ivp.y0, ts=ts, order=3, init="constant", fixed_iters=iters)
+
+    solve()
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        solve()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    t_it = float(np.median(times)) / iters
+    D, d = 4, 1
+    by = roofline.iteration_bytes(ns, D)
+    fl = roofline.iteration_flops(ns, D, d)
+    lib = _lib.load()
+    ctx = _lib.context(0)
+    n = len(roofline.KERNEL_CLASSES)
+    ms = (ctypes.c_double * n)()
+    la = (ctypes.c_longlong * n)()
+    wk = (ctypes.c_longlong * n)()
+    lib.pof_profile_enable(ctx, 1)
+    solve()
+    torch.cuda.synchronize()
+    lib.pof_profile_read(ctx, n, ms, la, wk)
+    lib.pof_profile_enable(ctx, 0)
+    return {
+        "workload": f"plain parallel IEKS, logistic t=[0,10] Ns={ns} q=3 (D=4), f64, {iters} fixed iterations",
+        "ms_per_iteration": 1e3 * t_it,
+        "element_once_bytes": by,
+        "achieved_gbs": by / t_it / 1e9,
+        "hbm_frac": by / t_it / (roofline.PEAK_HBM_GBS * 1e9),
+        "fp64_frac": fl / t_it / (roofline.PEAK_FP64_TFLOPS * 1e12),
+        "kernel_classes_ms_per_iteration": {k: round(ms[i] / iters, 4) for i, k in enumerate(roofline.KERNEL_CLASSES)
+                                            if la[i] > 0},
+        "launches_per_iteration": sum(la) / iters,
+    }
+
+
+def _cfg2_iteration_job(_):
+    """One full-size oracle IEKS iteration of the Lorenz-63 workload in a worker process (seconds)."""
     import numpy as np
 
-    try:
-        from threadpoolctl import threadpool_limits
-    except ImportError:  # pragma: no cover
-        threadpool_limits = None
     from oracle import model as om
     from oracle import problems as op
     from oracle import solver as osol
 
-    p = op.lorenz63()
-    ts = np.linspace(0.0, 10.0, NS + 1)
-    setup = om.set_up_solver(p, ts, Q)
+    setup = om.set_up_solver(op.lorenz63(), np.linspace(0.0, 10.0, NS + 1), Q)
     states = om.get_initial_trajectory(setup, "constant")
+    t0 = time.perf_counter()
+    osol.ieks_step(setup, states)
+    return time.perf_counter() - t0
 
-    def run():
+
+def cpu_baseline():
+    """The oracle (NumPy FP64 parallel-scan IEKS) on the host: one full-size iteration of the
+    workload in each of as many processes as there are host cores, all at once (BLAS on one
+    thread each), so solves/s = cores / (10 x the mean iteration time) is the host's throughput on
+    independent solves.  The single-process time of one iteration is reported beside it."""
+    import multiprocessing as mp
+
+    import numpy as np
+
+    cores = len(os.sched_getaffinity(0))
+    env = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS")}
+    for k in env:
+        os.environ[k] = "1"
+    try:
+        ctx = mp.get_context("spawn")
+        with ctx.Pool(1) as pool:
+            single = pool.map(_cfg2_iteration_job, [0])[0]
         t0 = time.perf_counter()
-        osol.ieks_step(setup, states)
-        return time.perf_counter() - t0
-
-    def runs():
-        run()  # warm-up
-        return sorted(run() for _ in range(3))
-
-    if threadpool_limits is not None:
-        with threadpool_limits(limits=1):
-            ts_ = runs()
-    else:
-        ts_ = runs()
-    t = ts_[1]
+        with ctx.Pool(cores) as pool:
+            per = pool.map(_cfg2_iteration_job, range(cores), chunksize=1)
+        wall = time.perf_counter() - t0
+    finally:
+        for k, v in env.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    t = float(np.mean(per))
     return {
-        "value": 1.0 / (ITERS * t),
+        "value": cores / (ITERS * t),
         "unit": "solves/s",
-        "cores": 1,
+        "cores": cores,
         "kind": "port",
-        "sample": f"one IEKS iteration of the Lorenz-63 Ns={NS} q={Q} f64 workload, median of 3 after one warm-up "
-                  f"({t:.2f} s; min {ts_[0]:.2f} s), BLAS/LAPACK on 1 thread; solves/s = 1 / ({ITERS} x iteration time)",
-        "iteration_s_min": ts_[0],
-        "iteration_s_median": t,
-        "host_cores": len(os.sched_getaffinity(0)),
+        "sample": f"one IEKS iteration of the Lorenz-63 Ns={NS} q={Q} f64 workload in each of {cores} processes at "
+                  f"once (BLAS on 1 thread each; mean {t:.2f} s per iteration, {wall:.1f} s wall); solves/s = "
+                  f"{cores} / ({ITERS} x mean iteration time)",
+        "iteration_s_mean": t,
+        "iteration_s_single_process": single,
+        "single_core_solves_per_s": 1.0 / (ITERS * single),
     }
 
 
-def batched_run(n_ivps, maxiters, plain_sample, oracle_sample, seed=4):
+def batched_run(n_ivps, maxiters, plain_sample, seed=4, cpu=True):
     """Mixed-IVP batch (one launch per scan level per (d, q) group) vs the plain per-IVP scan.
 
     The batch is solved once untimed (allocation, warm-up) and once timed.  The
@@ -273,6 +402,21 @@ def batched_run(n_ivps, maxiters, plain_sample, oracle_sample, seed=4):
     res = pof.batch.solve_batch(probs, maxiters=maxiters)
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
+    # the same batch with every scan level in the latency form (the kernels' forms before tuning)
+    from pof import autotune
+
+    tuned = autotune.read_cache()
+    t_default = None
+    if tuned:
+        for D in tuned:
+            autotune.set_widths(D, [autotune.NEVER] * 4)
+        t0 = time.perf_counter()
+        res_d = pof.batch.solve_batch(probs, maxiters=maxiters)
+        torch.cuda.synchronize()
+        t_default = time.perf_counter() - t0
+        same_forms = all(bool(torch.equal(a[0].mean, b[0].mean)) for a, b in zip(res, res_d))
+        for D, w in tuned.items():
+            autotune.set_widths(D, w)
     rng = np.random.default_rng(0)
     idx = rng.choice(n_ivps, size=min(plain_sample, n_ivps), replace=False)
     t_plain = 0.0
@@ -285,18 +429,7 @@ def batched_run(n_ivps, maxiters, plain_sample, oracle_sample, seed=4):
         t_plain += time.perf_counter() - t0
         same = same and bool(torch.equal(ys.mean, res[i][0].mean)) and bool(torch.equal(ys.chol, res[i][0].chol))
     plain_total = t_plain / len(idx) * n_ivps
-    # accuracy against the FP64 oracle on the cheapest converged members
-    from oracle import problems as op
-    from oracle import solver as osol
-
-    conv = [i for i in range(n_ivps) if res[i][1]["status"] == "converged"]
-    conv.sort(key=lambda i: len(probs[i][2]) * (probs[i][0].d * (probs[i][3] + 1)) ** 3)
-    errs = []
-    for i in conv[:oracle_sample]:
-        f, y0, ts, q, init = probs[i]
-        (om, _), _ = osol.solve(op.Problem(f.name, f.params, y0, ts[0], ts[-1]), ts, q, init=init,
-                                maxiters=maxiters)
-        errs.append(workloads.max_rel_err(res[i][0].mean.cpu().numpy(), om))
+    oracle = stratified_oracle_check(probs, [r[1] for r in res], [r[0].mean for r in res], maxiters)
     status = [r[1]["status"] for r in res]
     its = np.array([r[1]["iterations"] for r in res])
     return {
@@ -304,6 +437,14 @@ def batched_run(n_ivps, maxiters, plain_sample, oracle_sample, seed=4):
                     f"Ns log-uniform [256, 16384], q uniform 1..4, f64, maxiters {maxiters}, constant init, seed {seed}",
         "solves_per_s": n_ivps / t_batch,
         "seconds": t_batch,
+        "autotuned": {
+            "what": "scan kernel forms from the autotuner's cache (pof/tuning_gfx950.json) against every level in "
+                    "the latency form",
+            "tuned_seconds": t_batch,
+            "default_seconds": t_default,
+            "speedup": (t_default / t_batch) if t_default else None,
+            "results_bitwise_equal": same_forms if tuned else None,
+        },
         "plain_solves_per_s": n_ivps / plain_total,
         "plain_seconds_estimated": plain_total,
         "plain_sample": int(len(idx)),
@@ -311,9 +452,156 @@ def batched_run(n_ivps, maxiters, plain_sample, oracle_sample, seed=4):
         "same_as_alone": same,
         "iterations_mean": float(its.mean()),
         "status_counts": {k: status.count(k) for k in sorted(set(status))},
-        "max_rel_err_vs_oracle": max(e[0] for e in errs) if errs else None,
-        "normwise_err_vs_oracle": max(e[1] for e in errs) if errs else None,
-        "oracle_sample": len(errs),
+        "oracle_check": oracle,
+        "max_normwise_err_vs_oracle": oracle["max_normwise_err_well_conditioned"],
+        "algorithmic_tflops": batch_flops(probs, [r[1]["iterations"] for r in res]) / t_batch / 1e12,
+        "cpu_baseline": batch_cpu_baseline(probs, [r[1]["iterations"] for r in res]) if cpu else None,
+    }
+
+
+def batch_flops(probs, iterations):
+    """Algorithmic flops of a batch: sum over IVPs of iterations x Ns (74.3 D^3 + 4/3 (D + d)^3)."""
+    from pof import roofline
+
+    tot = 0.0
+    for (f, _, ts, q, _), it in zip(probs, iterations):
+        tot += it * roofline.iteration_flops(len(ts) - 1, f.d * (q + 1), f.d)
+    return tot
+
+
+def stratified_oracle_check(probs, infos, means, maxiters, prefix=512, prefix_iters=3):
+    """Compare batch members with the FP64 oracle, one per (d, q) group and status.
+
+    For every (d, q) group (D = d (q + 1) up to 15) it takes the shortest member of each final
+    status (converged, maxiter), so the maxiter members and the q = 4 Lorenz-63 (D = 15)
+    members are covered.  A converged member is compared as solved (iterations exact, means).
+    A maxiter member (not converged; the oracle's run to maxiters would take minutes) is
+    compared on a prefix of its grid: the first `prefix` steps, `prefix_iters` fixed iterations,
+    solved on the GPU and by the oracle alike.
+
+    Each comparison also runs the oracle once more with y0 scaled by 1 + 1e-15 and reports that
+    spread (normwise): where the configuration amplifies rounding (an unstable parallel scan, an
+    iteration path that wanders before converging) the oracle moves by as much as the GPU differs
+    from it, and the GPU's error there measures the conditioning, not the implementation.  The
+    members whose own spread is below 1e-8 are the well-conditioned ones; the bar (1e-6) applies to
+    them.
+    """
+    import numpy as np
+    import torch
+
+    import pof.solver
+    from oracle import problems as op
+    from oracle import solver as osol
+    from pof import workloads
+
+    pick = {}
+    for i, ((f, _, ts, q, _), info) in enumerate(zip(probs, infos)):
+        if info["status"] not in ("converged", "maxiter"):
+            continue
+        key = (f.name, f.d, q, info["status"])
+        if key not in pick or len(ts) < len(probs[pick[key]][2]):
+            pick[key] = i
+    rows = []
+    worst = 0.0
+    for key in sorted(pick):
+        i = pick[key]
+        f, y0, ts, q, init = probs[i]
+        oprob = op.Problem(f.name, f.params, y0, ts[0], ts[-1])
+        y0p = np.asarray(y0, dtype=np.float64) * (1.0 + 1e-15)
+        if key[3] == "converged":
+            (om, _), oinfo = osol.solve(oprob, ts, q, init=init, maxiters=maxiters)
+            (op_, _), pinfo = osol.solve(op.Problem(f.name, f.params, y0p, ts[0], ts[-1]), ts, q, init=init,
+                                         maxiters=maxiters)
+            m = means[i].cpu().numpy()
+            what = f"converged, {int(infos[i]['iterations'])} iterations (oracle {int(oinfo['iterations'])}, " \
+                   f"perturbed oracle {int(pinfo['iterations'])})"
+            same_it = oinfo["iterations"] == infos[i]["iterations"]
+        else:
+            tp = ts[: prefix + 1]
+            yp, _ = pof.solver.solve(f=f, y0=y0, ts=tp, order=q, init=init, fixed_iters=prefix_iters)
+            torch.cuda.synchronize()
+            (om, _), _ = osol.solve(op.Problem(f.name, f.params, y0, tp[0], tp[-1]), tp, q, init=init,
+                                    fixed_iters=prefix_iters)
+            (op_, _), _ = osol.solve(op.Problem(f.name, f.params, y0p, tp[0], tp[-1]), tp, q, init=init,
+                                     fixed_iters=prefix_iters)
+            m = yp.mean.cpu().numpy()
+            what = f"{key[3]}: first {len(tp) - 1} steps, {prefix_iters} fixed iterations"
+            same_it = True
+        elem, norm = workloads.max_rel_err(m, om)
+        _, spread = workloads.max_rel_err(op_, om)
+        well = spread < 1e-8
+        if well:
+            worst = max(worst, norm)
+        rows.append({"problem": f.name, "d": f.d, "q": q, "D": f.d * (q + 1), "Ns": len(ts) - 1, "check": what,
+                     "iterations_equal": bool(same_it), "max_rel_err": elem, "normwise_err": norm,
+                     "oracle_spread_normwise": spread, "well_conditioned": bool(well)})
+    wc = [r for r in rows if r["well_conditioned"]]
+    return {"members": rows, "groups": len({(r["d"], r["q"]) for r in rows}),
+            "well_conditioned": len(wc),
+            "max_normwise_err_well_conditioned": worst,
+            "max_rel_err_well_conditioned": max((r["max_rel_err"] for r in wc), default=None),
+            "all_iterations_equal_well_conditioned": all(r["iterations_equal"] for r in wc),
+            "ill_conditioned": [f"{r['problem']} q={r['q']} Ns={r['Ns']}: GPU {r['normwise_err']:.1e} vs oracle "
+                                f"spread {r['oracle_spread_normwise']:.1e}" for r in rows if not r["well_conditioned"]]}
+
+
+def _oracle_solve_job(args):
+    """Oracle IEKS iterations of one IVP in a worker process (the CPU baseline of the batch):
+    the seconds per iteration, over `iters` fixed iterations after the set-up."""
+    name, params, y0, ts, q, iters = args
+    from oracle import model as om
+    from oracle import problems as op
+    from oracle import solver as osol
+
+    setup = om.set_up_solver(op.Problem(name, params, y0, ts[0], ts[-1]), ts, q)
+    states = om.get_initial_trajectory(setup, "constant")
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        states = osol.ieks_step(setup, states)[0]
+    return (time.perf_counter() - t0) / iters
+
+
+def batch_cpu_baseline(probs, iterations, sample=48, iters=2, seed=1):
+    """The oracle on a random sample of the batch's IVPs, one process per host core (BLAS on one
+    thread each): each sampled IVP runs `iters` IEKS iterations and its solve time is that time per
+    iteration times the iteration count its GPU solve took; solves/s = sample / (summed solve times
+    / cores), the pool's throughput at full occupancy."""
+    import multiprocessing as mp
+
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    idx = rng.choice(len(probs), size=min(sample, len(probs)), replace=False)
+    jobs = []
+    for i in idx:
+        f, y0, ts, q, _ = probs[i]
+        jobs.append((f.name, tuple(f.params), np.asarray(y0, dtype=np.float64), ts, q, iters))
+    cores = len(os.sched_getaffinity(0))
+    env = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS")}
+    for k in env:
+        os.environ[k] = "1"
+    try:
+        ctx = mp.get_context("spawn")
+        t0 = time.perf_counter()
+        with ctx.Pool(cores) as pool:
+            per = pool.map(_oracle_solve_job, jobs, chunksize=1)
+        wall = time.perf_counter() - t0
+    finally:
+        for k, v in env.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    solve_s = sum(t * max(float(iterations[i]), 1.0) for t, i in zip(per, idx))
+    return {
+        "value": len(jobs) / (solve_s / cores),
+        "unit": "solves/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{len(jobs)} IVPs drawn at random (seed {seed}) from the batch; each ran {iters} oracle IEKS "
+                  f"iterations in a pool of {cores} processes (BLAS on 1 thread each, {wall:.1f} s wall), its "
+                  f"solve time taken as seconds per iteration x the iteration count of its GPU solve; "
+                  f"{solve_s:.0f} s of CPU work for the {len(jobs)} solves",
     }
 
 
@@ -327,7 +615,7 @@ def _progress(label, every):
     return hook
 
 
-def served_run(n_req, rate, maxiters, seed=5, verbose=False, oracle_sample=2):
+def served_run(n_req, rate, maxiters, seed=5, verbose=False):
     """Poisson request stream served with continuous batching vs a FIFO plain per-request scan."""
     import numpy as np
     import torch
@@ -356,18 +644,10 @@ def served_run(n_req, rate, maxiters, seed=5, verbose=False, oracle_sample=2):
     pst = pof.serve.latency_stats(precs)
     same = all(np.array_equal(a.means, b.means) for a, b in zip(recs, precs))
     srv.close()
-    # accuracy against the FP64 oracle on the cheapest converged requests
-    from oracle import problems as op
-    from oracle import solver as osol
+    import torch as _t
 
-    conv = [i for i, r in enumerate(recs) if r.status == "converged"]
-    conv.sort(key=lambda i: len(reqs[i][2]) * (reqs[i][0].d * (reqs[i][3] + 1)) ** 3)
-    errs = []
-    for i in conv[:oracle_sample]:
-        f, y0, ts, q, init = reqs[i]
-        (om, _), _ = osol.solve(op.Problem(f.name, f.params, y0, ts[0], ts[-1]), ts, q, init=init,
-                                maxiters=maxiters)
-        errs.append(workloads.max_rel_err(recs[i].means, om))
+    oracle = stratified_oracle_check(reqs, [{"status": r.status, "iterations": r.iterations} for r in recs],
+                                     [_t.as_tensor(r.means) for r in recs], maxiters)
     return {
         "workload": f"{n_req} requests, Poisson arrivals at {rate:.1f}/s (seed {seed}), same IVP mix with "
                     f"Ns log-uniform [256, 65536], maxiters {maxiters}",
@@ -378,9 +658,8 @@ def served_run(n_req, rate, maxiters, seed=5, verbose=False, oracle_sample=2):
         "plain_fifo_p50_ms": pst["p50_ms"],
         "plain_fifo_p99_ms": pst["p99_ms"],
         "same_as_alone": bool(same),
-        "max_rel_err_vs_oracle": max(e[0] for e in errs) if errs else None,
-        "normwise_err_vs_oracle": max(e[1] for e in errs) if errs else None,
-        "oracle_sample": len(errs),
+        "oracle_check": oracle,
+        "max_normwise_err_vs_oracle": oracle["max_normwise_err_well_conditioned"],
     }
 
 
@@ -415,8 +694,10 @@ def precision_run(ns=8192, iters=3, check_ns=32, check_iters=3, fp32_products=Tr
 
     def measure(products):
         def run(n_it, grid):
-            return pof.solver.solve(f=ivp.f, y0=ivp.y0, ts=grid, order=3, init="constant", fixed_iters=n_it,
-                                    engine="large", products=products)
+            if products == "float64":
+                return pof.solver.solve(f=ivp.f, y0=ivp.y0, ts=grid, order=3, init="constant", fixed_iters=n_it,
+                                        engine="large")
+            return fp32_solve(ivp, grid, n_it)
 
         run(1, ts)
         torch.cuda.synchronize()
@@ -434,14 +715,38 @@ def precision_run(ns=8192, iters=3, check_ns=32, check_iters=3, fp32_products=Tr
         ssq_diff = abs(ic["sigma_squared"] - rinfo["sigma_squared"]) / abs(rinfo["sigma_squared"])
         return t, finite, elem, norm, cov_err, ssq_diff
 
+    def fp32_solve(ivp_, grid, n_it):
+        """The FP32-product experiment through the C ABI (the Python API refuses it): calibrate = 0,
+        means only (sigma_squared, and with it every factor, is rounding noise in FP32)."""
+        import ctypes
+
+        from pof import _lib
+        from pof.convenience import ProblemSpec
+        from pof.utils import MVNSqrt
+
+        spec = ProblemSpec(ivp_.f, ivp_.y0, grid, 3, "constant")
+        prob = spec.cstruct()
+        means = torch.empty((len(grid), spec.d), dtype=torch.float64, device="cuda")
+        chols = torch.empty((len(grid), spec.d, spec.D), dtype=torch.float64, device="cuda")
+        out = _lib.Out()
+        out.means_dev, out.chols_dev = means.data_ptr(), chols.data_ptr()
+        opts = _lib.Opts()
+        opts.maxiters, opts.fixed_iters, opts.calibrate, opts.products = 10, n_it, 0, _lib.PRODUCTS_F32
+        ctx = _lib.context(0)
+        _lib.check(_lib.load().pof_solve_large(ctx, ctypes.byref(prob), ctypes.byref(opts), ctypes.byref(out),
+                                               _lib.stream_ptr()), ctx, "pof_solve_large")
+        return MVNSqrt(means, chols), {"sigma_squared": float(out.sigma_squared)}
+
     t, finite, elem, norm, cov_err, ssq_diff = measure("float64")
     try:
         if not fp32_products:
             raise RuntimeError("not run (--no-fp32-products)")
         t32, finite32, elem32, norm32, cov32, ssq32 = measure("float32")
         fp32 = {
-            "what": "scan combines' matrix products as FP32 MFMA GEMMs (FP32 accumulate); factorisations, "
-                    "triangular solves, elements and observation statistics FP64",
+            "what": "experiment, refused by the public API: scan combines' matrix products as FP32 MFMA GEMMs "
+                    "(FP32 accumulate); factorisations, triangular solves, elements and observation statistics "
+                    "FP64; calibrate = 0 (sigma_squared is rounding noise in FP32, so cov_rel_err and "
+                    "sigma_squared_rel_diff compare uncalibrated, meaningless factors)",
             "ms_per_iteration": 1e3 * t32 / iters,
             "finite": finite32,
             "max_rel_err_means": elem32,
@@ -529,7 +834,7 @@ def main():
             print(json.dumps({"converged": converged, "accuracy": accuracy}), file=sys.stderr, flush=True)
     batched = served = None
     if not args.no_batch:
-        batched = batched_run(args.batch_ivps, maxiters=100, plain_sample=32, oracle_sample=4)
+        batched = batched_run(args.batch_ivps, maxiters=100, plain_sample=32, cpu=not args.no_cpu_baseline)
         if args.verbose:
             print(json.dumps(batched), file=sys.stderr, flush=True)
         # mean Ns of log-uniform [256, 65536] over [256, 16384]: 3.04x more work per request
@@ -540,6 +845,10 @@ def main():
             print(json.dumps(served), file=sys.stderr, flush=True)
     precision = (None if args.no_precision
                  else precision_run(args.precision_ns, fp32_products=not args.no_fp32_products))
+    known = None if args.no_checks else known_answers()
+    small = None if args.no_checks else small_state_run()
+    if args.verbose:
+        print(json.dumps({"known_answers": known, "small_state": small}), file=sys.stderr, flush=True)
     if args.verbose and precision:
         print(json.dumps(precision), file=sys.stderr, flush=True)
     cpu = None if args.no_cpu_baseline else cpu_baseline()
@@ -577,6 +886,8 @@ def main():
         "batched": batched,
         "served": served,
         "precision": precision,
+        "known_answers": known,
+        "small_state": small,
     }
     print(json.dumps(out), flush=True)
 

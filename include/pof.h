@@ -134,6 +134,21 @@ int pof_profile_read(const pof_ctx* ctx, int n_classes, double* ms, long long* l
  * host.  Instrumented runs (pof_profile_enable) always launch from the host. */
 int pof_graphs_enable(pof_ctx* ctx, int on);
 
+/* Kernel forms of the scan levels (no reference counterpart; the reference's
+ * scan is jax.lax.associative_scan, parallel_filtsmooth/filter.py:35 and
+ * smoother.py:15-17).  Each scan kernel of state dimension D <= 16 exists in a
+ * latency form (a matrix row split over 2 or 4 lanes, one combine per
+ * wavefront) and a throughput form (one lane per row, several combines per
+ * wavefront); a level with at least widths[k] work items (combines, summed
+ * over the batch's IVPs) runs in the throughput form.  widths = {filter
+ * up-sweep, filter down-sweep, smoother up-sweep, smoother down-sweep}, each
+ * >= 1 (1: always the throughput form; a width above every level: never).
+ * Results are bitwise the same in either form.  Set per context and state
+ * dimension, typically from the autotuner's cache (pof/autotune.py); the
+ * built-in default is 8192 for each. */
+int pof_tuning_set(pof_ctx* ctx, int D, const int32_t* widths);
+int pof_tuning_get(const pof_ctx* ctx, int D, int32_t* widths);
+
 /* Full parallel IEKS solve.  Replaces pof.solver.solve (pof/solver.py:11-73). */
 int pof_solve(pof_ctx* ctx, const pof_problem* prob, const pof_opts* opts, pof_out* out, void* stream);
 

@@ -401,6 +401,10 @@ struct pof_ctx {
   // scratch_err saying why its address could not be taken
   double* scratch = nullptr;
   std::string scratch_err;
+  // scan-level widths of the throughput kernel forms per state dimension (pof_tuning_set);
+  // tune_version counts changes, so that batches rebuild plans and graphs
+  std::map<int, pof::Tuning> tuning;
+  int tune_version = 0;
 };
 
 // A group of IVPs with one (D, d) sharing one device workspace.  IVP slots are
@@ -473,6 +477,7 @@ struct pof_batch {
   hipGraphExec_t gexec = nullptr;
   int gmode = -1;
   int gstable = 0;  // launches since the plan was last rebuilt
+  int tune_version = -1;  // the context's tuning the plan was built with
 };
 
 // ------------------------------------------------------ no-allocation scratch
@@ -915,6 +920,7 @@ int batch_launch(pof_batch* b) {
   pof_ctx* ctx = b->ctx;
   if (b->launched) return set_err(ctx, POF_E_ARG, "pof_batch_launch called twice without pof_batch_finish");
   HIP_TRY(hipSetDevice(ctx->device));
+  if (b->tune_version != ctx->tune_version) b->plan_dirty = true;  // other kernel forms: new plan and graph
   if (b->plan_dirty) {
     if (settle(b) != POF_OK) return POF_E_HIP;
     drop_graph(b);
@@ -930,6 +936,9 @@ int batch_launch(pof_batch* b) {
     b->hmaps.clear();
     int rc = plan_tables(b, b->active, b->hmaps, b->plan);
     if (rc != POF_OK) return set_err(ctx, rc, "launch tables overflow");
+    const auto tu = ctx->tuning.find(b->D);
+    if (tu != ctx->tuning.end()) b->plan.tune = tu->second;
+    b->tune_version = ctx->tune_version;
     if (b->hmaps.size() > b->maps_cap) {
       HIP_TRY(hipStreamSynchronize(b->st));
       if (b->maps_dev) HIP_TRY(hipFree(b->maps_dev));
@@ -1463,6 +1472,32 @@ int pof_profile_enable(pof_ctx* ctx, int on) {
 int pof_graphs_enable(pof_ctx* ctx, int on) {
   if (!ctx) return POF_E_ARG;
   ctx->graphs = on != 0;
+  return POF_OK;
+}
+
+int pof_tuning_set(pof_ctx* ctx, int D, const int32_t* widths) {
+  if (!ctx || D < 1 || !widths) return set_err(ctx, POF_E_ARG, "pof_tuning_set: null argument or D < 1");
+  for (int i = 0; i < 4; ++i)
+    if (widths[i] < 1) return set_err(ctx, POF_E_ARG, "pof_tuning_set: widths must be >= 1");
+  pof::Tuning t;
+  t.filter_up = widths[0];
+  t.filter_down = widths[1];
+  t.smooth_up = widths[2];
+  t.smooth_down = widths[3];
+  ctx->tuning[D] = t;
+  ++ctx->tune_version;
+  return POF_OK;
+}
+
+int pof_tuning_get(const pof_ctx* ctx, int D, int32_t* widths) {
+  if (!ctx || !widths) return POF_E_ARG;
+  pof::Tuning t;
+  const auto it = ctx->tuning.find(D);
+  if (it != ctx->tuning.end()) t = it->second;
+  widths[0] = t.filter_up;
+  widths[1] = t.filter_down;
+  widths[2] = t.smooth_up;
+  widths[3] = t.smooth_down;
   return POF_OK;
 }
 

@@ -28,6 +28,14 @@
 #include "pof_problems.hpp"
 #include "pof_types.hpp"
 
+// No implicit contraction of a * b + c into fused multiply-adds in these
+// building blocks: HIP compiles with -ffp-contract=fast by default, and which
+// products the compiler then fuses depends on the shape of the surrounding
+// code.  Every fused multiply-add here is an explicit fma(), so two code shapes
+// that perform the same operations in the same order (the latency and
+// throughput forms of a scan kernel) round identically.
+#pragma clang fp contract(off)
+
 namespace pof {
 
 __device__ __forceinline__ void wsync() {
@@ -57,12 +65,14 @@ constexpr int max_obs_dim(int D) {
 }
 
 // Lanes per worker and the padded row length for state dimension D with
-// rows split over S lanes.
-template <int D_, int S_>
+// rows split over S lanes.  W_ > 0 overrides the worker width (a power of two
+// dividing 64, at least 2 D S).
+template <int D_, int S_, int W_ = 0>
 struct Lay {
   static constexpr int D = D_;
   static constexpr int S = S_;
-  static constexpr int W = (2 * D_ * S <= 32) ? 32 : 64;
+  static constexpr int W = (W_ > 0) ? W_ : (2 * D_ * S <= 32) ? 32 : 64;
+  static_assert(W >= 2 * D_ * S || W == 64, "a worker holds the 2 D rows of a combine");
   static constexpr int DP = padded<D_, S>();
   static constexpr int LD = DP + 1;  // LDS row stride (odd: no bank conflicts down a column)
   static constexpr int NL = DP / S;  // registers of a D-column row
@@ -70,13 +80,26 @@ struct Lay {
 
 // LDS layout kinds: how many tiles / broadcast vectors a worker of each
 // kernel needs and whether the block stages the model constants F, LQ, N1z.
-enum LdsKind : int { kLdsFull = 0, kLdsState, kLdsSmooth, kLdsLeaf, kLdsPost, kLdsSeq, kLdsLeaf1 };
+// The *T kinds are the throughput forms of the three scan combines: the same
+// LDS needs, but one lane per matrix row and the narrowest worker that holds
+// the 2D rows (several combines per wavefront), for scan levels wide enough
+// to fill the chip, where the per-pivot work that every lane of a worker
+// repeats, not the latency of one combine, sets the time.
+enum LdsKind : int { kLdsFull = 0, kLdsState, kLdsSmooth, kLdsLeaf, kLdsPost, kLdsSeq, kLdsLeaf1, kLdsFullT,
+                     kLdsStateT, kLdsSmoothT };
+
+// the latency-oriented kind of a throughput kind (same LDS needs)
+constexpr int base_kind(int K) {
+  return (K == kLdsFullT) ? kLdsFull : (K == kLdsStateT) ? kLdsState : (K == kLdsSmoothT) ? kLdsSmooth : K;
+}
+constexpr bool is_throughput_kind(int K) { return K == kLdsFullT || K == kLdsStateT || K == kLdsSmoothT; }
 
 // Widest observation dimension of the per-worker engines.
 constexpr int kMaxObs = 4;
 
-template <int K>
+template <int K_>
 struct LdsNeeds {
+  static constexpr int K = base_kind(K_);
   static constexpr int NT = (K == kLdsFull) ? 5
                             : (K == kLdsState) ? 4
                             : (K == kLdsSeq) ? 5
@@ -97,22 +120,42 @@ struct LdsNeeds {
 // of a row repeats, keep one lane per row and two workers per wavefront.
 template <int D, int K>
 constexpr int kind_split() {
-  return (K == kLdsLeaf || K == kLdsLeaf1 || K == kLdsPost || K == kLdsSeq) ? 1 : split_of<D>();
+  return (K == kLdsLeaf || K == kLdsLeaf1 || K == kLdsPost || K == kLdsSeq || is_throughput_kind(K)) ? 1
+                                                                                                     : split_of<D>();
+}
+
+// Worker width: the throughput kinds take the narrowest power of two (at least
+// 8 lanes) that holds the 2D rows; the others the default of Lay.
+constexpr int pow2_at_least(int n, int lo) { return (lo >= n) ? lo : pow2_at_least(n, 2 * lo); }
+template <int D, int K>
+constexpr int kind_width() {
+  return is_throughput_kind(K) ? pow2_at_least(2 * D, 8) : 0;
+}
+
+// 1 when the throughput kinds differ from the latency-oriented ones (rows split over several lanes)
+template <int D>
+constexpr bool has_throughput_kinds() {
+  return split_of<D>() > 1;
 }
 
 template <int D_, int K = kLdsFull>
-struct Geo : Lay<D_, kind_split<D_, K>()> {
-  using L = Lay<D_, kind_split<D_, K>()>;
+struct Geo : Lay<D_, kind_split<D_, K>(), kind_width<D_, K>()> {
+  using L = Lay<D_, kind_split<D_, K>(), kind_width<D_, K>()>;
   static constexpr int TILE = D_ * L::LD;
   static constexpr int NT = LdsNeeds<K>::NT;        // tiles per worker
   static constexpr int NV = LdsNeeds<K>::NV;        // broadcast vectors per worker (2D entries each)
-  static constexpr int BC = 4 * L::DP + 2;          // pivot-row broadcast of two matrices of 2 DP columns
+  // pivot-row broadcast of two matrices of 2 DP columns; the throughput kinds broadcast rows in the
+  // padded layout of the latency kinds' split (whose reduction order they reproduce)
+  static constexpr int BC = 4 * (is_throughput_kind(K) ? padded<D_, split_of<D_>()>() : L::DP) + 2;
   static constexpr int SMALL = LdsNeeds<K>::SMALL ? kMaxObs * L::LD + kMaxObs * D_ : 0;
   static constexpr int WORKER = BC + NV * 2 * D_ + NT * TILE + SMALL;
   static constexpr int CONSTS = LdsNeeds<K>::CONSTS ? 3 * TILE : 0;  // F, LQ, N1z
   static constexpr size_t bytes_for(int wpb) { return sizeof(double) * (CONSTS + wpb * WORKER); }
   // workers per block: as many as fit in 64 KiB of LDS, at most 256 threads
-  static constexpr int WPB = (L::W == 32 && bytes_for(8) <= 65536)   ? 8
+  // (throughput kinds: the largest power of two up to 256 threads that fits)
+  static constexpr int wpb_fit(int w) { return (w <= 1 || bytes_for(w) <= 65536) ? w : wpb_fit(w / 2); }
+  static constexpr int WPB = is_throughput_kind(K)                   ? wpb_fit(256 / L::W)
+                             : (L::W == 32 && bytes_for(8) <= 65536) ? 8
                              : (bytes_for(256 / L::W) <= 65536)      ? 256 / L::W
                              : (bytes_for(2) <= 65536)               ? 2
                                                                      : 1;
@@ -337,10 +380,57 @@ __device__ __forceinline__ void own(const double (&f)[n], int sub, double (&x)[N
   }
 }
 
+// ------------------------------------------------ reductions in a split's order
+// A row reduction of the split form (S lanes per row) sums each lane's columns
+// t S + sub in a chain and adds the lanes' partial sums in the order of the DPP
+// butterfly of rowsum<S>.  The throughput forms keep a row in one lane (S = 1)
+// but reduce "as if split over V lanes" (V = the latency form's split), so the
+// two forms give bitwise equal results: V partial sums, combined like rowsum<V>.
+template <int V>
+__device__ __forceinline__ double vcombine(const double (&p)[V]) {
+  if constexpr (V == 1) {
+    return p[0];
+  } else if constexpr (V == 2) {
+    return p[0] + p[1];
+  } else {
+    static_assert(V == 4, "splits of 1, 2 or 4 lanes");
+    return (p[0] + p[1]) + (p[2] + p[3]);
+  }
+}
+
+// compact column of virtual column cv of a row laid out as two blocks (B1, B2),
+// each padded to a multiple of V (P1 = padded<B1, V>), or -1 for a padding column
+template <int B1, int B2, int P1>
+__host__ __device__ constexpr int vcol(int cv) {
+  return (cv < P1) ? ((cv < B1) ? cv : -1) : ((cv - P1 < B2) ? B1 + (cv - P1) : -1);
+}
+
 // sum_c x[c] v[c] over the row's n columns (x split, v replicated in LDS);
-// every lane of the row gets the sum.
-template <int S, int n, int NX>
+// every lane of the row gets the sum.  V > S (S = 1 only): the order of a split over V lanes.
+template <int S, int n, int V = S, int NX>
 __device__ __forceinline__ double row_dot(const double (&x)[NX], const double* v, int sub) {
+  if constexpr (V != S) {
+    static_assert(S == 1, "a reduction order of V lanes is reproduced on one lane per row");
+    constexpr int NXV = padded<n, V>() / V;
+    double part[V];
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+      double acc = 0.0;
+#pragma unroll
+      for (int t = 0; t < NXV; ++t) {
+        const int c = t * V + u;
+        const double xv = (c < n) ? x[(c < n) ? c : 0] : 0.0;
+        if ((t + 1) * V <= n) {
+          acc = fma(xv, v[c], acc);
+        } else if (t * V < n) {
+          const double vv = v[c < n ? c : 0];
+          acc = fma(xv, (c < n) ? vv : 0.0, acc);
+        }
+      }
+      part[u] = acc;
+    }
+    return vcombine<V>(part);
+  }
   double acc = 0.0;
 #pragma unroll
   for (int t = 0; t < NX; ++t) {
@@ -358,8 +448,29 @@ __device__ __forceinline__ double row_dot(const double (&x)[NX], const double* v
 // sum_{k < n} p[k * stride] v[k] (a row (stride 1) or a column (stride LD) of
 // an LDS tile times an LDS vector), split over the row's lanes and reduced
 // with DPP; every lane of the row gets the sum.
-template <int S, int n, int stride>
+template <int S, int n, int stride, int V = S>
 __device__ __forceinline__ double lds_dot(const double* p, const double* v, int sub) {
+  if constexpr (V != S) {
+    static_assert(S == 1, "a reduction order of V lanes is reproduced on one lane per row");
+    double part[V];
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+      double acc = 0.0;
+#pragma unroll
+      for (int t = 0; t < (n + V - 1) / V; ++t) {
+        const int k = t * V + u;
+        if ((t + 1) * V <= n) {
+          acc = fma(p[k * stride], v[k], acc);
+        } else {
+          const int kk = (k < n) ? k : 0;
+          const double x = p[kk * stride] * v[kk];
+          acc += (k < n) ? x : 0.0;
+        }
+      }
+      part[u] = acc;
+    }
+    return vcombine<V>(part);
+  }
   double acc = 0.0;
 #pragma unroll
   for (int t = 0; t < (n + S - 1) / S; ++t) {
@@ -372,6 +483,34 @@ __device__ __forceinline__ double lds_dot(const double* p, const double* v, int 
       acc += (k < n) ? x : 0.0;
     }
   }
+  return rowsum<S>(acc);
+}
+
+// sum_t x[t] y[t] over a row held in registers (both split the same way; padding
+// registers zero), reduced along the row; V > S as in row_dot.
+template <int S, int n, int V = S, int NX>
+__device__ __forceinline__ double reg_dot(const double (&x)[NX], const double (&y)[NX]) {
+  if constexpr (V != S) {
+    static_assert(S == 1, "a reduction order of V lanes is reproduced on one lane per row");
+    constexpr int NXV = padded<n, V>() / V;
+    double part[V];
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+      double acc = 0.0;
+#pragma unroll
+      for (int t = 0; t < NXV; ++t) {
+        const int c = t * V + u;
+        const double xv = (c < n) ? x[(c < n) ? c : 0] : 0.0;
+        const double yv = (c < n) ? y[(c < n) ? c : 0] : 0.0;
+        acc = fma(xv, yv, acc);
+      }
+      part[u] = acc;
+    }
+    return vcombine<V>(part);
+  }
+  double acc = 0.0;
+#pragma unroll
+  for (int t = 0; t < NX; ++t) acc = fma(x[t], y[t], acc);
   return rowsum<S>(acc);
 }
 
@@ -528,6 +667,131 @@ __device__ __forceinline__ void lq(double (&a)[(padded<B1, S>() + padded<B2, S>(
     }
   }
   wsync();
+}
+
+// lq() of a split over V lanes reproduced on one lane per row (the throughput
+// forms): the same reflectors, partial sums, chains and combination order as
+// lq<V, ...> on V lanes, so the factor is bitwise the same.  The lane holds the
+// row's compact columns; virtual column cv = t V + u (u = the virtual sub-lane)
+// maps to compact column vcol(cv), padding columns are zero.
+template <int V, int NR, int B1, int B2, int NG = 1, int I0 = 0, int I1 = 1 << 20, bool FINISH = true,
+          int W2 = 1 << 20>
+__device__ __forceinline__ void lq_virtual(double (&a)[B1 + B2], int row, double* bc) {
+  static_assert(NG == 1 || NG == 2, "one or two matrices");
+  constexpr int P1 = padded<B1, V>();
+  constexpr int NP = P1 + padded<B2, V>();
+  constexpr int NX = NP / V;
+  constexpr int NC = B1 + B2;
+  constexpr int K = (NR < NC) ? NR : NC;
+  const int g = (NG == 2 && row >= NR) ? 1 : 0;
+  const int li = row - g * NR;
+  const bool valid = row < NG * NR;
+  double* brow = bc + g * NP;
+#pragma unroll
+  for (int i = I0; i < ((I1 < K) ? I1 : K); ++i) {
+    if (i == NC - 1) break;  // a length-1 tail: H = I
+    const int pc = (i < B1) ? i : P1 + (i - B1);
+    const int t0 = pc / V, k0 = pc % V;
+    const int band = (i > W2) ? i : W2;
+    const int te = (i < B1 && W2 < B2 && P1 + band < NP) ? (P1 + band) / V + 1 : NX;
+    if (valid && li == i) {
+#pragma unroll
+      for (int cv = t0 * V; cv < te * V; ++cv) {
+        const int j = vcol<B1, B2, P1>(cv);
+        brow[cv] = (j >= 0) ? a[(j >= 0) ? j : 0] : 0.0;
+      }
+    }
+    wsync();
+    const double alpha = brow[pc];
+    double pv[NP];
+#pragma unroll
+    for (int cv = t0 * V; cv < te * V; ++cv) pv[cv] = brow[cv];
+    constexpr int CH = (NX > 16) ? 4 : 2;
+    double sqp[V], dtp[V];
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+      const int c0 = t0 * V + u;
+      const int j0 = vcol<B1, B2, P1>(c0);
+      const double p0 = (u > k0) ? pv[c0] : 0.0;
+      const double a0 = (j0 >= 0) ? a[(j0 >= 0) ? j0 : 0] : 0.0;
+      double sq[CH], dt[CH];
+#pragma unroll
+      for (int c = 0; c < CH; ++c) sq[c] = dt[c] = 0.0;
+      sq[0] = p0 * p0;
+      dt[0] = a0 * p0;
+#pragma unroll
+      for (int t = t0 + 1; t < te; ++t) {
+        const int cv = t * V + u;
+        const int j = vcol<B1, B2, P1>(cv);
+        const double av = (j >= 0) ? a[(j >= 0) ? j : 0] : 0.0;
+        sq[(t - t0) % CH] = fma(pv[cv], pv[cv], sq[(t - t0) % CH]);
+        dt[(t - t0) % CH] = fma(av, pv[cv], dt[(t - t0) % CH]);
+      }
+      if constexpr (CH == 4) {
+        sqp[u] = (sq[0] + sq[1]) + (sq[2] + sq[3]);
+        dtp[u] = (dt[0] + dt[1]) + (dt[2] + dt[3]);
+      } else {
+        sqp[u] = sq[0] + sq[1];
+        dtp[u] = dt[0] + dt[1];
+      }
+    }
+    const int jp = vcol<B1, B2, P1>(pc);  // the pivot column is never padding
+    const double ai = a[jp];
+    const double sig = vcombine<V>(sqp);
+    const double dot = vcombine<V>(dtp);
+    double nrm, rnrm;
+    sqrt_rsqrt(fma(alpha, alpha, sig), nrm, rnrm);
+    const double bc_ = -copysign(nrm, alpha);
+    const double den = alpha - bc_;
+    const double sc_ = rcp(den);
+    const double tc_ = den * copysign(rnrm, alpha);
+    const bool nz = sig != 0.0;
+    const double beta = nz ? bc_ : alpha;
+    const double scal = nz ? sc_ : 0.0;
+    const double tau = nz ? tc_ : 0.0;
+    if (valid && li == i) a[jp] = beta;
+    if (valid && li > i) {
+      const double tw = tau * fma(scal, dot, ai);
+      const double tws = tw * scal;
+#pragma unroll
+      for (int u = 0; u < V; ++u) {
+        const int c0 = t0 * V + u;
+        const int j0 = vcol<B1, B2, P1>(c0);
+        if (j0 >= 0) {
+          const double p0 = (u > k0) ? pv[c0] : 0.0;
+          a[j0] = (u == k0) ? a[j0] - tw : fma(-tws, p0, a[j0]);
+        }
+      }
+#pragma unroll
+      for (int t = t0 + 1; t < te; ++t) {
+#pragma unroll
+        for (int u = 0; u < V; ++u) {
+          const int cv = t * V + u;
+          const int j = vcol<B1, B2, P1>(cv);
+          if (j >= 0) a[j] = fma(-tws, pv[cv], a[j]);
+        }
+      }
+    }
+  }
+  if constexpr (FINISH) {
+#pragma unroll
+    for (int j = 0; j < NC; ++j) a[j] = (valid && j > li) ? 0.0 : a[j];
+  }
+  wsync();
+}
+
+// lq() with the reduction order of a split over V lanes: lq<S> itself when the
+// row is split over V = S lanes, lq_virtual<V> on one lane per row.
+template <int S, int V, int NR, int B1, int B2, int NG = 1, int I0 = 0, int I1 = 1 << 20, bool FINISH = true,
+          int W2 = 1 << 20>
+__device__ __forceinline__ void lqv(double (&a)[(padded<B1, S>() + padded<B2, S>()) / S], int row, int sub,
+                                    double* bc) {
+  if constexpr (S == V) {
+    lq<S, NR, B1, B2, NG, I0, I1, FINISH, W2>(a, row, sub, bc);
+  } else {
+    static_assert(S == 1, "a reduction order of V lanes is reproduced on one lane per row");
+    lq_virtual<V, NR, B1, B2, NG, I0, I1, FINISH, W2>(a, row, bc);
+  }
 }
 
 // --------------------------------------------------- triangular solves (split)
@@ -1005,6 +1269,7 @@ __device__ void filter_combine(const Ctx<D, SS>& cx, const FiltArrays& x1, const
                                size_t o) {
   using L = Lay<D, SS>;
   constexpr int S = L::S, LD = L::LD, NL = L::NL;
+  constexpr int VS = split_of<D>();  // reduction order: that of the latency form's split
   const int r = cx.row, s = cx.sub;
   const bool act = r < D;
   const int rr = act ? r : 0;  // clamped row for branch-free loads
@@ -1058,15 +1323,15 @@ __device__ void filter_combine(const Ctx<D, SS>& cx, const FiltArrays& x1, const
     double* Vs2 = cx.vec(5);
     double* Vu2 = cx.vec(6);
     {
-      const double t = lds_dot<S, D, LD>(TZ2 + rr, Vb1, s);
+      const double t = lds_dot<S, D, LD, VS>(TZ2 + rr, Vb1, s);
       if (act) Vt[r] = t;
     }
     wsync();
-    const double s2 = eta2r - lds_dot<S, D, 1>(TZ2 + rr * LD, Vt, s);
+    const double s2 = eta2r - lds_dot<S, D, 1, VS>(TZ2 + rr * LD, Vt, s);
     if (act) Vs2[r] = s2;
     wsync();
     {
-      const double uu = lds_dot<S, D, LD>(TU1 + rr, Vs2, s);
+      const double uu = lds_dot<S, D, LD, VS>(TU1 + rr, Vs2, s);
       if (act) Vu2[r] = uu;
     }
   }
@@ -1090,20 +1355,20 @@ __device__ void filter_combine(const Ctx<D, SS>& cx, const FiltArrays& x1, const
   if constexpr (state_only) {
     // Xi11 and Xi21 are final after the first D pivots; the last D pivots only
     // form Xi22, which enters Z alone
-    lq<S, 2 * D, D, D, 1, 0, D / 2, false, 0>(xr, r, s, cx.bc());
+    lqv<S, VS, 2 * D, D, D, 1, 0, D / 2, false, 0>(xr, r, s, cx.bc());
     load_dist<S, D>(a2, x2.A + (i2 * D + rr) * D, s);
     load_dist<S, D>(u2, x2.U + (i2 * D + rr) * D, s);
     b2r = x2.b[i2 * D + rr];
     zero(z1);
-    lq<S, 2 * D, D, D, 1, D / 2, D, true, 0>(xr, r, s, cx.bc());
+    lqv<S, VS, 2 * D, D, D, 1, D / 2, D, true, 0>(xr, r, s, cx.bc());
   } else {
-    lq<S, 2 * D, D, D, 1, 0, D, false, 0>(xr, r, s, cx.bc());
+    lqv<S, VS, 2 * D, D, D, 1, 0, D, false, 0>(xr, r, s, cx.bc());
     load_dist<S, D>(a2, x2.A + (i2 * D + rr) * D, s);
     load_dist<S, D>(u2, x2.U + (i2 * D + rr) * D, s);
     load_dist<S, D>(z1, x1.Z + (i1 * D + rz) * D, s);  // Z1 rows feed the rows D..2D-1
     b2r = x2.b[i2 * D + rr];
     eta1r = x1.eta[i1 * D + rr];
-    lq<S, 2 * D, D, D, 1, D>(xr, r, s, cx.bc());
+    lqv<S, VS, 2 * D, D, D, 1, D>(xr, r, s, cx.bc());
   }
   if (act) {
 #pragma unroll
@@ -1144,13 +1409,13 @@ __device__ void filter_combine(const Ctx<D, SS>& cx, const FiltArrays& x1, const
   // a store in flight would make the next wait on a load also wait for it)
   double bb;
   {
-    const double u = lds_dot<S, D, LD>(TU1 + rr, Veta2, s);
+    const double u = lds_dot<S, D, LD, VS>(TU1 + rr, Veta2, s);
     if (act) Vu[r] = u;
     wsync();
-    const double sv = Vb1[rr] + lds_dot<S, D, 1>(TU1 + rr * LD, Vu, s);
+    const double sv = Vb1[rr] + lds_dot<S, D, 1, VS>(TU1 + rr * LD, Vu, s);
     if (act) Vs[r] = sv;
     wsync();
-    bb = b2r + row_dot<S, D>(wr, Vs, s);
+    bb = b2r + row_dot<S, D, VS>(wr, Vs, s);
     pin1(bb);  // formed here: sunk to its store, it holds b2 and partial sums through the last factorisation
   }
   // eta = A1^T (I - Xi21 Xi11^-1 U1^T)(eta2 - Z2 Z2^T b1) + eta1 and A = W A1
@@ -1168,14 +1433,11 @@ __device__ void filter_combine(const Ctx<D, SS>& cx, const FiltArrays& x1, const
     {
       double tx[NL];
       load_dist<S, D>(tx, TX21 + rr * LD, s);
-      double acc = 0.0;
-#pragma unroll
-      for (int t = 0; t < NL; ++t) acc = fma(tx[t], w2[t], acc);
-      const double v2 = rowsum<S>(acc);
+      const double v2 = reg_dot<S, D, VS>(tx, w2);
       if (act) Vs3[r] = s2 - v2;
     }
     wsync();
-    eo = eta1r + lds_dot<S, D, LD>(TA1 + rr, Vs3, s);
+    eo = eta1r + lds_dot<S, D, LD, VS>(TA1 + rr, Vs3, s);
     pin1(eo);
     {
       double wf[D];
@@ -1207,7 +1469,7 @@ __device__ void filter_combine(const Ctx<D, SS>& cx, const FiltArrays& x1, const
   }
   // bands: U2 is lower triangular; Z1 too, except a leaf element's [Zr | 0]
   // (nonzero in its first d columns only)
-  lq<S, D, D, D, state_only ? 1 : 2, 0, 1 << 20, true, state_only ? 0 : max_obs_dim(D) - 1>(jr, r, s, cx.bc());
+  lqv<S, VS, D, D, D, state_only ? 1 : 2, 0, 1 << 20, true, state_only ? 0 : max_obs_dim(D) - 1>(jr, r, s, cx.bc());
   // output offsets formed here, not hoisted above the factorisation (where
   // they would hold registers through it)
   o = opaque_sz(o);
@@ -1237,6 +1499,7 @@ __device__ void smoother_combine(const Ctx<D, SS>& cx, const SmoArrays& x1, cons
                                  size_t o, bool with_chol) {
   using L = Lay<D, SS>;
   constexpr int S = L::S, LD = L::LD, NL = L::NL;
+  constexpr int VS = split_of<D>();  // reduction order: that of the latency form's split
   const int r = cx.row, s = cx.sub;
   const bool act = r < D;
   const int rr = act ? r : 0;
@@ -1270,7 +1533,7 @@ __device__ void smoother_combine(const Ctx<D, SS>& cx, const SmoArrays& x1, cons
   wsync();
   // results stay in registers; every global store is issued at the end so
   // that no wait on outstanding memory operations lands inside the LQ below
-  const double gg = g2r + row_dot<S, D>(e2, Vg1, s);
+  const double gg = g2r + row_dot<S, D, VS>(e2, Vg1, s);
   double e2f[D];
   gather<S, D>(e2, e2f);
   double er[state_only ? 1 : NL];
@@ -1284,7 +1547,7 @@ __device__ void smoother_combine(const Ctx<D, SS>& cx, const SmoArrays& x1, cons
       dr[t] = t1[t];
       dr[NL + t] = d2[t];
     }
-    lq<S, D, D, D>(dr, r, s, cx.bc());
+    lqv<S, VS, D, D, D>(dr, r, s, cx.bc());
   }
   if (act) {
     if (s == 0) (state_only ? so.m : y.g)[o * D + r] = gg;
@@ -1295,3 +1558,5 @@ __device__ void smoother_combine(const Ctx<D, SS>& cx, const SmoArrays& x1, cons
 }
 
 }  // namespace pof
+
+#pragma clang fp contract(fast)

@@ -35,9 +35,21 @@ struct KernelTimer {
   bool ready = false;
 };
 
+// Scan-level widths (work items per launch) from which each scan kernel runs
+// in its throughput form (one lane per matrix row, several combines per
+// wavefront) instead of its latency form (rows split over several lanes).
+// Only state dimensions whose latency form splits rows (D <= 16) have both.
+struct Tuning {
+  int filter_up = 1 << 30;
+  int filter_down = 1 << 30;
+  int smooth_up = 1 << 30;
+  int smooth_down = 1 << 30;
+};
+
 // Launch maps of one active set of a group (host view of a device int buffer):
 // the active IVP ids, then one prefix table per launch kind and level.
 struct LaunchPlan {
+  Tuning tune;
   const int* dev = nullptr;  // device buffer
   int m = 0;                 // active IVPs
   int fl = 0, sl = 0;        // filter / smoother levels (maximum over the active IVPs)

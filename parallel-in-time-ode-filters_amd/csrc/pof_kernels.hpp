@@ -150,12 +150,21 @@ constexpr int down_min_waves() {
   return D == 15 ? 4 : (D == 10) ? 5 : 1;
 }
 
-// Up-sweep: out[i] = in[2i] o in[2i+1]
-template <int D>
-__global__ __launch_bounds__((Geo<D, kLdsFull>::BLOCK), up_min_waves<D>()) void k_filter_up(GroupArgs g, LaunchMap lm,
-                                                                                             int l) {
+// Waves per SIMD of a scan kernel of kind K: the tuned bound of the
+// latency-oriented kinds; at least 2 for the throughput kinds, whose natural
+// allocation can exceed half the register file (D = 15: 256 VGPRs + 4 AGPRs,
+// one wave per SIMD).
+template <int D, int K>
+constexpr int scan_min_waves() {
+  return (K == kLdsFull) ? up_min_waves<D>() : (K == kLdsState) ? down_min_waves<D>() : is_throughput_kind(K) ? 2 : 1;
+}
+
+// Up-sweep: out[i] = in[2i] o in[2i+1]  (K: kLdsFull, or kLdsFullT on wide levels)
+template <int D, int K>
+__global__ __launch_bounds__((Geo<D, K>::BLOCK), (scan_min_waves<D, K>())) void k_filter_up(GroupArgs g, LaunchMap lm,
+                                                                                            int l) {
   extern __shared__ double smem[];
-  auto cx = make_ctx<D, kLdsFull>(g.F, g.LQ, g.N1z, smem);
+  auto cx = make_ctx<D, K>(g.F, g.LQ, g.N1z, smem);
   int j, i;
   if (!locate(lm, cx.wid, j, i)) return;
   const long long N = g.Ns[j], base = g.base[j];
@@ -168,11 +177,11 @@ __global__ __launch_bounds__((Geo<D, kLdsFull>::BLOCK), up_min_waves<D>()) void 
 // Down-sweep of level l: positions 2i and 2i+1 of the level-l prefix states.
 //   p[0] = state(e[0]); p[2i+1] = up[i]; p[2i] = up[i-1] o e[2i]  (i >= 1)
 // With top != 0 (a level of one element) only p[0] = state(e[0]) is written.
-template <int D>
-__global__ __launch_bounds__((Geo<D, kLdsState>::BLOCK), down_min_waves<D>()) void k_filter_down(GroupArgs g,
-                                                                                               LaunchMap lm, int l) {
+template <int D, int K>
+__global__ __launch_bounds__((Geo<D, K>::BLOCK), (scan_min_waves<D, K>())) void k_filter_down(GroupArgs g,
+                                                                                            LaunchMap lm, int l) {
   extern __shared__ double smem[];
-  auto cx = make_ctx<D, kLdsState>(g.F, g.LQ, g.N1z, smem);
+  auto cx = make_ctx<D, K>(g.F, g.LQ, g.N1z, smem);
   int j, i;
   if (!locate(lm, cx.wid, j, i)) return;
   const long long N = g.Ns[j], base = g.base[j];
@@ -396,10 +405,10 @@ __global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK), (post_min_waves<D, EK1>(
 }
 
 // Smoother up-sweep
-template <int D>
-__global__ __launch_bounds__((Geo<D, kLdsSmooth>::BLOCK)) void k_smooth_up(GroupArgs g, LaunchMap lm, int l) {
+template <int D, int K>
+__global__ __launch_bounds__((Geo<D, K>::BLOCK)) void k_smooth_up(GroupArgs g, LaunchMap lm, int l) {
   extern __shared__ double smem[];
-  auto cx = make_ctx<D, kLdsSmooth>(g.F, g.LQ, g.N1z, smem);
+  auto cx = make_ctx<D, K>(g.F, g.LQ, g.N1z, smem);
   int j, i;
   if (!locate(lm, cx.wid, j, i)) return;
   const long long n0 = (long long)g.Ns[j] + 1, base = g.base[j];
@@ -411,10 +420,10 @@ __global__ __launch_bounds__((Geo<D, kLdsSmooth>::BLOCK)) void k_smooth_up(Group
                              with_chol != 0);
 }
 
-template <int D>
-__global__ __launch_bounds__((Geo<D, kLdsSmooth>::BLOCK)) void k_smooth_down(GroupArgs g, LaunchMap lm, int l) {
+template <int D, int K>
+__global__ __launch_bounds__((Geo<D, K>::BLOCK)) void k_smooth_down(GroupArgs g, LaunchMap lm, int l) {
   extern __shared__ double smem[];
-  auto cx = make_ctx<D, kLdsSmooth>(g.F, g.LQ, g.N1z, smem);
+  auto cx = make_ctx<D, K>(g.F, g.LQ, g.N1z, smem);
   int j, i;
   if (!locate(lm, cx.wid, j, i)) return;
   const long long n0 = (long long)g.Ns[j] + 1, base = g.base[j];
@@ -923,18 +932,32 @@ struct Rec {
 };
 }  // namespace
 
+// Launch one scan level: the throughput kind KT when the level has at least
+// `thr` work items (and the kinds differ at this D), else the latency kind KL.
+#define POF_LAUNCH_LEVEL(kernel, KL, KT, thr, nw, ...)                                        \
+  do {                                                                                          \
+    if constexpr (has_throughput_kinds<D>()) {                                                  \
+      if ((nw) >= (thr)) {                                                                      \
+        POF_LAUNCH((kernel<D, KT>), KT, nw, __VA_ARGS__);                                       \
+        break;                                                                                  \
+      }                                                                                         \
+    }                                                                                           \
+    POF_LAUNCH((kernel<D, KL>), KL, nw, __VA_ARGS__);                                           \
+  } while (0)
+
 template <int D>
 void smooth_scan(const GroupArgs& g, const LaunchPlan& p, hipStream_t st, Rec& rec) {
   for (int l = 0; l + 1 < p.sl; ++l) {
     const int nw = p.sup_total[l];
     if (nw == 0) continue;
-    POF_LAUNCH((k_smooth_up<D>), kLdsSmooth, nw, st, g, p.map(p.sup_off[l], nw), l);
+    POF_LAUNCH_LEVEL(k_smooth_up, kLdsSmooth, kLdsSmoothT, p.tune.smooth_up, nw, st, g, p.map(p.sup_off[l], nw), l);
     rec(kKSmoothUp, nw);
   }
   for (int l = p.sl - 1; l >= 0; --l) {
     const int nw = p.sdn_total[l];
     if (nw == 0) continue;
-    POF_LAUNCH((k_smooth_down<D>), kLdsSmooth, nw, st, g, p.map(p.sdn_off[l], nw), l);
+    POF_LAUNCH_LEVEL(k_smooth_down, kLdsSmooth, kLdsSmoothT, p.tune.smooth_down, nw, st, g, p.map(p.sdn_off[l], nw),
+                     l);
     rec(kKSmoothDown, nw - p.sdn_ivps[l]);
   }
 }
@@ -972,13 +995,14 @@ hipError_t run_iteration(const GroupArgs& g, const LaunchPlan& p, double* scal, 
   for (int l = 0; l + 1 < p.fl; ++l) {
     const int nw = p.fup_total[l];
     if (nw == 0) continue;
-    POF_LAUNCH((k_filter_up<D>), kLdsFull, nw, st, g, p.map(p.fup_off[l], nw), l);
+    POF_LAUNCH_LEVEL(k_filter_up, kLdsFull, kLdsFullT, p.tune.filter_up, nw, st, g, p.map(p.fup_off[l], nw), l);
     rec(kKFilterUp, nw);
   }
   for (int l = p.fl - 1; l >= 0; --l) {
     const int nw = p.fdn_total[l];
     if (nw == 0) continue;
-    POF_LAUNCH((k_filter_down<D>), kLdsState, nw, st, g, p.map(p.fdn_off[l], nw), l);
+    POF_LAUNCH_LEVEL(k_filter_down, kLdsState, kLdsStateT, p.tune.filter_down, nw, st, g, p.map(p.fdn_off[l], nw),
+                     l);
     rec(kKFilterDown, nw - p.fdn_ivps[l]);
   }
   // ---- smoother elements + observation statistics

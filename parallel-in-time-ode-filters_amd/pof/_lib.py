@@ -96,6 +96,8 @@ SIGNATURES = {
     "pof_profile_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pof_profile_read": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _P(ctypes.c_longlong), _P(ctypes.c_longlong)]),
     "pof_graphs_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pof_tuning_set": (ctypes.c_int, [_vp, ctypes.c_int, _P(ctypes.c_int32)]),
+    "pof_tuning_get": (ctypes.c_int, [_vp, ctypes.c_int, _P(ctypes.c_int32)]),
     "pof_solve": (ctypes.c_int, [_vp, _P(Problem), _P(Opts), _P(Out), _vp]),
     "pof_solve_large": (ctypes.c_int, [_vp, _P(Problem), _P(Opts), _P(Out), _vp]),
     "pof_setup": (ctypes.c_int, [_P(Problem), _dp, _dp, _dp, _dp, _dp, _dp]),
@@ -193,6 +195,10 @@ def context(device=None):
             raise PofError(f"pof_ctx_create(device={device}) failed with status {rc}")
         c = h
         _ctx[key] = c
+        # the autotuner's cached kernel-form widths (pof/tuning_gfx950.json), if any
+        from . import autotune
+
+        autotune.apply_cache(c)
     return c
 
 

@@ -4,8 +4,9 @@ Source: ``experiments/3_work_precision_diagram/data/<problem>_Tesla_V100-SXM2-32
 of the reference (FP64 runs of ``solve(init="constant", maxiters=1000)`` on
 ``ts = linspace(t0, tmax, N)``, ``run_benchmark.py:63-80,143-148,179``).
 Only the columns ``Ns`` (= len(ts)), ``IEKS(q)_iterations``,
-``IEKS(q)_rmse_final``, ``IEKS(q)_rmse_traj``, ``sIEKS(q)_iterations``
-(q = 1, 2, 3) are kept, values copied verbatim:
+``IEKS(q)_rmse_final``, ``IEKS(q)_rmse_traj``, ``sIEKS(q)_iterations``,
+``IEKS(q)_runtime`` (seconds on the V100, min of 3 timed runs) (q = 1, 2, 3)
+are kept, values copied verbatim:
 
 * ``work_precision_pins.csv``: rows with N <= 256, the sizes the CPU oracle
   runs in seconds (checked against the oracle on the CPU);
@@ -49,6 +50,7 @@ def write(ref_root, name, keep):
                             "rmse_final": rec[f"IEKS({q})_rmse_final"],
                             "rmse_traj": rec[f"IEKS({q})_rmse_traj"],
                             "seq_iterations": rec[f"sIEKS({q})_iterations"],
+                            "runtime": rec[f"IEKS({q})_runtime"],
                         }
                     )
     with open(out, "w", newline="") as f:

@@ -73,8 +73,11 @@ def test_batch_members_match_oracle(gpu):
     against the oracle: iterations exact, means to 1e-9 (1e-8 for q = 4)."""
     import pof.batch
 
+    import pof.ivp
+
     probs = _mix(7, 12, nmax=120)
-    assert any(p[0].name == "lorenz63" and p[3] == 4 for p in probs)
+    lz = pof.ivp.lorenz63(rho=24.0, tmax=2.0)
+    probs.append((lz.f, lz.y0 + 0.05, np.linspace(0.0, 2.0, 97), 4, "constant"))
     res = pof.batch.solve_batch(probs, maxiters=60)
     for (f, y0, ts, q, init), (ys, info) in zip(probs, res):
         print(f"member {f.name} q={q} N={len(ts)}", flush=True)
@@ -85,8 +88,16 @@ def test_batch_members_match_oracle(gpu):
             # iterates amplify rounding, the oracle's own results move by ~1e-9
             # under a 1e-15 change of y0; only the iteration count is compared
             continue
-        # q = 4 priors (cond(Q) = 4.8e5) amplify rounding to ~5e-9; the acceptance bar is 1e-6
-        assert err < (1e-8 if q == 4 else 1e-9), (f.name, q, len(ts), err)
+        assert err < _member_tol(f.name, q), (f.name, q, len(ts), err)
+
+
+def _member_tol(name, q):
+    """Converged means against the oracle: 1e-9; q = 4 priors (cond(Q) = 4.8e5) amplify rounding
+    to ~5e-9 (1e-8 there), and the chaotic Lorenz-63 field over t in [0, 2] to 1.6e-7 (measured):
+    the acceptance bar, 1e-6, there."""
+    if name == "lorenz63":
+        return 1e-6
+    return 1e-8 if q == 4 else 1e-9
 
 
 def test_batch_lorenz_members_fixed_iterations_match_oracle(gpu):
@@ -247,7 +258,7 @@ def test_server_results_same_as_alone(gpu):
         err, oinfo = _oracle_err(f, y0, ts, q, init, r.means, maxiters=60)
         assert r.iterations == oinfo["iterations"], r.rid
         if oinfo["iterations"] <= 60:
-            assert err < (1e-8 if q == 4 else 1e-9), (r.rid, f.name, q, len(ts), err)
+            assert err < _member_tol(f.name, q), (r.rid, f.name, q, len(ts), err)
 
 
 def test_server_rejects_a_request_that_cannot_fit(gpu):
