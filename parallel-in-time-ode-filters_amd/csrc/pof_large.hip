@@ -3,25 +3,23 @@
 //
 // Each combine of a scan level is a sequence of batched primitives over every
 // combine of that level: a Householder LQ kernel (one workgroup per matrix,
-// LAPACK dlarfg's reflector convention, the same as the per-worker lq()),
-// rocBLAS strided-batched DGEMM for the plain products (FP64 MFMA on gfx950),
-// a blocked triangular-solve kernel, and small copy / fill / reduction kernels.  The
+// LAPACK dlarfg's reflector convention, the same as the per-worker lq()), a
+// strided-batched GEMM kernel on the matrix cores (v_mfma_f64_16x16x4f64, or
+// v_mfma_f32_16x16x4f32 for the FP32-product option), a blocked
+// triangular-solve kernel, and small copy / fill / reduction kernels.  The
 // host drives JAX's odd/even associative_scan recursion (pof/parallel_filtsmooth
 // filter.py:35, smoother.py:15-17) over strided views, so every factor is built
 // by the same sequence of operations as in the reference.
 //
-// Matrices are row-major float64.  A rocBLAS call on a row-major C = op(A) op(B)
-// is the column-major C^T = op(B)^T op(A)^T; a row-major lower-triangular L is
-// the column-major upper L^T.
+// Matrices are row-major float64.
 #include "pof_large.hpp"
-
-#include <rocblas/rocblas.h>
 
 #include "pof_problems.hpp"
 
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <type_traits>
 #include <vector>
 
 namespace pof {
@@ -360,28 +358,6 @@ __global__ __launch_bounds__(256) void k_copy_t(double* dst, long long ds, int d
   }
 }
 
-// FP32 staging of a GEMM operand: dst item b (rows x cols, contiguous) = (float) src item b.
-__global__ void k_to_f32(float* dst, long long ds, const double* src, long long ss, int sld, int rows, int cols,
-                         int cnt) {
-  const int b = blockIdx.y;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= cnt || e >= rows * cols) return;
-  const int r = e / cols, c = e % cols;
-  dst[(long long)b * ds + e] = (float)src[(long long)b * ss + (size_t)r * sld + c];
-}
-
-// dst item b (rows x cols) = alpha * (double) src item b + beta * dst (beta = 0: dst is not read).
-__global__ void k_from_f32(double* dst, long long ds, int dld, const float* src, long long ss, int rows, int cols,
-                           int cnt, double alpha, double beta) {
-  const int b = blockIdx.y;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= cnt || e >= rows * cols) return;
-  const int r = e / cols, c = e % cols;
-  double* o = dst + (long long)b * ds + (size_t)r * dld + c;
-  const double p = alpha * (double)src[(long long)b * ss + e];
-  *o = (beta == 0.0) ? p : fma(beta, *o, p);
-}
-
 // dst item (rows x cols) = dv on the diagonal, 0 elsewhere
 __global__ void k_fill(double* dst, long long ds, int dld, int rows, int cols, int cnt, double dv) {
   const int b = blockIdx.y;
@@ -529,6 +505,159 @@ __global__ void k_project(const double* m, const double* L, const double* P, int
   }
 }
 
+// ------------------------------------------------------------------ GEMM
+// C = alpha op(A) op(B) + beta C for item blockIdx.x of a strided batch
+// (row-major, op = transpose when tA / tB; a stride of 0 shares a matrix over
+// the batch; beta = 0 does not read C).  A workgroup of four waves computes a
+// BM x BN tile of C (blockIdx.y, blockIdx.z); the waves form a WM x WN grid,
+// each computing (BM / WM) x (BN / WN) as 16 x 16 tiles on the matrix cores.
+// K advances in chunks of 16 staged through LDS (zero-padded at the edges),
+// loaded along whichever index is contiguous in memory; the next chunk is
+// loaded into registers while the current one is multiplied.  F32: the
+// operands are rounded to FP32 as they enter LDS and multiplied with FP32
+// accumulate (v_mfma_f32_16x16x4f32); the epilogue is FP64.  Lane layout of the
+// 16 x 16 x 4 MFMAs: A[i][k] from lane i + 16 k, B[k][j] from lane j + 16 k;
+// value v of lane l holds result element (4 (l / 16) + v, l % 16) for FP32 but
+// (l / 16 + 4 v, l % 16) for FP64 (both checked on gfx950 with an in-bounds
+// probe of the builtins).
+typedef double gemm_d4 __attribute__((ext_vector_type(4)));
+typedef float gemm_f4 __attribute__((ext_vector_type(4)));
+
+constexpr int kGemmKc = 16;
+
+template <bool F32, int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(256) void k_gemm(int tA, int tB, int M, int N, int K, double alpha, const double* A,
+                                              long long sA, int lda, const double* B, long long sB, int ldb,
+                                              double beta, double* C, long long sC, int ldc) {
+  static_assert(WM * WN == 4, "four waves");
+  using T = typename std::conditional<F32, float, double>::type;
+  using V4 = typename std::conditional<F32, gemm_f4, gemm_d4>::type;
+  constexpr int TM = BM / WM / 16, TN = BN / WN / 16;  // MFMA tiles per wave
+  constexpr int LA = BM * kGemmKc / 256, LB = BN * kGemmKc / 256;  // elements each thread loads per chunk
+  static_assert(LA >= 1 && LB >= 1 && TM >= 1 && TN >= 1, "tile shape");
+  __shared__ T As[BM][kGemmKc + 1];
+  __shared__ T Bs[kGemmKc][BN + 1];
+  const long long z = blockIdx.x;
+  const double* Az = A + z * sA;
+  const double* Bz = B + z * sB;
+  double* Cz = C + z * sC;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.z * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave / WN, wc = wave % WN;
+  // element e (0 .. BM * 16 - 1) of the A chunk -> (i, k): along i when A is stored K x M (tA)
+  auto a_ik = [&](int e, int& i, int& k) {
+    if (tA) {
+      i = e % BM;
+      k = e / BM;
+    } else {
+      k = e % kGemmKc;
+      i = e / kGemmKc;
+    }
+  };
+  auto b_kj = [&](int e, int& kb, int& j) {
+    if (tB) {
+      kb = e % kGemmKc;
+      j = e / kGemmKc;
+    } else {
+      j = e % BN;
+      kb = e / BN;
+    }
+  };
+  double ra[LA], rb[LB];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < LA; ++r) {
+      int i, k;
+      a_ik(tid + 256 * r, i, k);
+      const int gi = m0 + i, gk = k0 + k;
+      ra[r] = (gi < M && gk < K) ? (tA ? Az[(size_t)gk * lda + gi] : Az[(size_t)gi * lda + gk]) : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < LB; ++r) {
+      int kb, j;
+      b_kj(tid + 256 * r, kb, j);
+      const int gj = n0 + j, gk = k0 + kb;
+      rb[r] = (gj < N && gk < K) ? (tB ? Bz[(size_t)gj * ldb + gk] : Bz[(size_t)gk * ldb + gj]) : 0.0;
+    }
+  };
+  V4 acc[TM][TN];
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b) acc[a][b] = V4{0, 0, 0, 0};
+  fetch(0);
+  for (int k0 = 0; k0 < K; k0 += kGemmKc) {
+#pragma unroll
+    for (int r = 0; r < LA; ++r) {
+      int i, k;
+      a_ik(tid + 256 * r, i, k);
+      As[i][k] = (T)ra[r];
+    }
+#pragma unroll
+    for (int r = 0; r < LB; ++r) {
+      int kb, j;
+      b_kj(tid + 256 * r, kb, j);
+      Bs[kb][j] = (T)rb[r];
+    }
+    __syncthreads();
+    if (k0 + kGemmKc < K) fetch(k0 + kGemmKc);  // in flight during the products below
+#pragma unroll
+    for (int kk = 0; kk < kGemmKc; kk += 4) {
+      T af[TM], bf[TN];
+#pragma unroll
+      for (int a = 0; a < TM; ++a) af[a] = As[(BM / WM) * wr + 16 * a + (lane & 15)][kk + (lane >> 4)];
+#pragma unroll
+      for (int b = 0; b < TN; ++b) bf[b] = Bs[kk + (lane >> 4)][(BN / WN) * wc + 16 * b + (lane & 15)];
+#pragma unroll
+      for (int a = 0; a < TM; ++a)
+#pragma unroll
+        for (int b = 0; b < TN; ++b) {
+          if constexpr (F32) {
+            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[a], bf[b], acc[a][b], 0, 0, 0);
+          } else {
+            acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a], bf[b], acc[a][b], 0, 0, 0);
+          }
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < TM; ++a)
+#pragma unroll
+    for (int b = 0; b < TN; ++b)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int gi =
+            m0 + (BM / WM) * wr + 16 * a + (F32 ? 4 * (lane >> 4) + v : (lane >> 4) + 4 * v);
+        const int gj = n0 + (BN / WN) * wc + 16 * b + (lane & 15);
+        if (gi < M && gj < N) {
+          double* c = Cz + (size_t)gi * ldc + gj;
+          const double prod = alpha * (double)acc[a][b][v];
+          *c = (beta == 0.0) ? prod : fma(beta, *c, prod);
+        }
+      }
+}
+
+// The N = 1 case (matrix-vector, plus the scalar-product rows): one thread per row of C,
+// FP64 (vector ALU): c_i = alpha sum_k op(A)[i][k] b_k + beta c_i.
+__global__ void k_gemv(int tA, int tB, int M, int K, double alpha, const double* A, long long sA, int lda,
+                       const double* B, long long sB, int ldb, double beta, double* C, long long sC, int ldc) {
+  const int i = blockIdx.y * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  const long long z = blockIdx.x;
+  const double* Az = A + z * sA;
+  const double* Bz = B + z * sB;
+  double acc = 0.0;
+  for (int k = 0; k < K; ++k) {
+    const double a = tA ? Az[(size_t)k * lda + i] : Az[(size_t)i * lda + k];
+    const double b = tB ? Bz[k] : Bz[(size_t)k * ldb];
+    acc = fma(a, b, acc);
+  }
+  double* c = C + z * sC + (size_t)i * ldc;
+  const double prod = alpha * acc;
+  *c = (beta == 0.0) ? prod : fma(beta, *c, prod);
+}
+
 // ------------------------------------------------------------- host helpers
 struct Arena;
 double* arena_take(Arena* ar, size_t n);
@@ -536,7 +665,6 @@ size_t arena_mark(Arena* ar);
 void arena_release(Arena* ar, size_t mark);
 
 struct Ops {
-  rocblas_handle h;
   hipStream_t st;
   Arena* ar;
   int err = 0;
@@ -552,57 +680,46 @@ struct Ops {
       msg = b;
     }
   }
-  static rocblas_operation op(bool t) { return t ? rocblas_operation_transpose : rocblas_operation_none; }
-  // row-major C (M x N) = alpha op(A) op(B) + beta C: FP64, or with f32 the product
-  // op(A) op(B) as an FP32 GEMM (FP32 operands, FP32 accumulate) for matrix-matrix
-  // shapes, scaled and added to C in FP64
+  // row-major C (M x N) = alpha op(A) op(B) + beta C: FP64 on the matrix cores, or with f32 the
+  // product op(A) op(B) of matrix-matrix shapes with FP32 operands and FP32 accumulate, scaled and
+  // added to C in FP64
   void gemm(bool tA, bool tB, int M, int N, int K, double alpha, V A, V B, double beta, V C, int cnt) {
-    if (f32 && M > 1 && N > 1 && K > 1)
-      gemm32(tA, tB, M, N, K, alpha, A, B, beta, C, cnt);
-    else
-      gemm64(tA, tB, M, N, K, alpha, A, B, beta, C, cnt);
-  }
-  void gemm32(bool tA, bool tB, int M, int N, int K, double alpha, V A, V B, double beta, V C, int cnt) {
-    if (err || cnt <= 0) return;
-    const int ra = tA ? K : M, ca = tA ? M : K, rb = tB ? N : K, cb = tB ? K : N;
-    const int na = A.s ? cnt : 1, nb = B.s ? cnt : 1;
-    const size_t mark = arena_mark(ar);
-    float* a32 = (float*)arena_take(ar, ((size_t)na * ra * ca + 1) / 2);
-    float* b32 = (float*)arena_take(ar, ((size_t)nb * rb * cb + 1) / 2);
-    float* p32 = (float*)arena_take(ar, ((size_t)cnt * M * N + 1) / 2);
-    if (!a32 || !b32 || !p32) {
-      fail("workspace (FP32 products)", -2);
-      arena_release(ar, mark);
-      return;
-    }
-    to_f32(a32, A, ra, ca, na);
-    to_f32(b32, B, rb, cb, nb);
-    const float one = 1.0f, zero = 0.0f;
-    const rocblas_status s = rocblas_sgemm_strided_batched(
-        h, op(tB), op(tA), N, M, K, &one, b32, cb, nb > 1 ? (long long)rb * cb : 0, a32, ca,
-        na > 1 ? (long long)ra * ca : 0, &zero, p32, N, (long long)M * N, cnt);
-    if (s != rocblas_status_success) fail("rocblas_sgemm_strided_batched", (int)s);
-    if (!err) {
-      const int e = M * N;
-      hipLaunchKernelGGL(k_from_f32, dim3((e + 255) / 256, cnt), dim3(256), 0, st, C.p, C.s, C.ld, p32,
-                         (long long)e, M, N, cnt, alpha, beta);
-      check("k_from_f32");
-    }
-    arena_release(ar, mark);
-  }
-  void to_f32(float* dst, V src, int rows, int cols, int cnt) {
-    if (err) return;
-    const int e = rows * cols;
-    hipLaunchKernelGGL(k_to_f32, dim3((e + 255) / 256, cnt), dim3(256), 0, st, dst, (long long)e, src.p, src.s,
-                       src.ld, rows, cols, cnt);
-    check("k_to_f32");
+    gemm_impl(f32 && M > 1 && N > 1 && K > 1, tA, tB, M, N, K, alpha, A, B, beta, C, cnt);
   }
   void gemm64(bool tA, bool tB, int M, int N, int K, double alpha, V A, V B, double beta, V C, int cnt) {
-    if (err || cnt <= 0) return;
-    const rocblas_status s =
-        rocblas_dgemm_strided_batched(h, op(tB), op(tA), N, M, K, &alpha, B.p, B.ld, B.s, A.p, A.ld, A.s, &beta,
-                                      C.p, C.ld, C.s, cnt);
-    if (s != rocblas_status_success) fail("rocblas_dgemm_strided_batched", (int)s);
+    gemm_impl(false, tA, tB, M, N, K, alpha, A, B, beta, C, cnt);
+  }
+  void gemm_impl(bool use32, bool tA, bool tB, int M, int N, int K, double alpha, V A, V B, double beta, V C,
+                 int cnt) {
+    if (err || cnt <= 0 || M <= 0 || N <= 0) return;
+    if (N == 1) {
+      hipLaunchKernelGGL(k_gemv, dim3(cnt, (M + 127) / 128), dim3(128), 0, st, tA ? 1 : 0, tB ? 1 : 0, M, K, alpha,
+                         A.p, A.s, A.ld, B.p, B.s, B.ld, beta, C.p, C.s, C.ld);
+      check("k_gemv");
+      return;
+    }
+    // tile shape by the product's shape: 128 x 16 for the skinny panel products (N <= 16),
+    // 128 x 64 for the large ones, 64 x 64 otherwise
+    if (N <= 16) {
+      launch_gemm<128, 16, 4, 1>(use32, tA, tB, M, N, K, alpha, A, B, beta, C, cnt);
+    } else if (M >= 128 && N >= 64) {
+      launch_gemm<128, 64, 2, 2>(use32, tA, tB, M, N, K, alpha, A, B, beta, C, cnt);
+    } else {
+      launch_gemm<64, 64, 2, 2>(use32, tA, tB, M, N, K, alpha, A, B, beta, C, cnt);
+    }
+    check("k_gemm");
+  }
+  template <int BM, int BN, int WM, int WN>
+  void launch_gemm(bool use32, bool tA, bool tB, int M, int N, int K, double alpha, V A, V B, double beta, V C,
+                   int cnt) {
+    const dim3 grid(cnt, (M + BM - 1) / BM, (N + BN - 1) / BN);
+    if (use32) {
+      hipLaunchKernelGGL((k_gemm<true, BM, BN, WM, WN>), grid, dim3(256), 0, st, tA ? 1 : 0, tB ? 1 : 0, M, N, K,
+                         alpha, A.p, A.s, A.ld, B.p, B.s, B.ld, beta, C.p, C.s, C.ld);
+    } else {
+      hipLaunchKernelGGL((k_gemm<false, BM, BN, WM, WN>), grid, dim3(256), 0, st, tA ? 1 : 0, tB ? 1 : 0, M, N, K,
+                         alpha, A.p, A.s, A.ld, B.p, B.s, B.ld, beta, C.p, C.s, C.ld);
+    }
   }
   // tria in place: the blocked LQ (panels of kNB rows in LDS, trailing rows updated
   // by three GEMMs per panel), or k_lq for matrices of at most one panel
@@ -882,7 +999,6 @@ void se_copy1(Ops& o, int D, const SE& dst, const SE& src) {
 
 struct Engine {
   int device = 0;
-  rocblas_handle h = nullptr;
   char* buf = nullptr;
   size_t cap = 0;
 };
@@ -890,18 +1006,12 @@ struct Engine {
 Engine* engine_create(int device) {
   Engine* e = new Engine();
   e->device = device;
-  if (rocblas_create_handle(&e->h) != rocblas_status_success) {
-    delete e;
-    return nullptr;
-  }
-  rocblas_set_pointer_mode(e->h, rocblas_pointer_mode_host);
   return e;
 }
 
 void engine_destroy(Engine* e) {
   if (!e) return;
   if (e->buf) (void)hipFree(e->buf);
-  if (e->h) rocblas_destroy_handle(e->h);
   delete e;
 }
 
@@ -972,9 +1082,8 @@ int solve(Engine* e, const Problem& p, const pof_opts& opts, pof_out* out, const
     e->cap = need;
   }
   Arena ar{e->buf, e->cap, 0};
-  Ops o{e->h, st, &ar};
+  Ops o{st, &ar};
   o.f32_combines = f32;
-  rocblas_set_stream(e->h, st);
   // constants: F, QL, N1z, P, x0 (one block)
   double* cst = ar.take(5 * DD + 2 * D);
   double* mA = ar.take((size_t)(Ns + 1) * D);
