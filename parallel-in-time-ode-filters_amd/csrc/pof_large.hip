@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <type_traits>
 #include <vector>
@@ -103,7 +104,9 @@ constexpr int kNB = 16;  // rows per panel of the blocked LQ
 // columns lane + 64 q) and the reflectors of k_lq.  Writes the panel's rows of
 // L back, the reflectors V (np x (n - i0), leading 1, zeros before it) and the
 // triangular factor T (np x np) with H_1 ... H_np = I - V^T T V.  Pivot column
-// k < kNB < 64 lives in slot 0 of lane k.
+// k < kNB < 64 lives in slot 0 of lane k, so only slot 0 needs the test c > k;
+// columns c >= n - i0 are loaded as zeros and stay zero (every update adds a
+// multiple of the pivot row, zero there too), so they need no test.
 template <int NQ>
 __global__ __launch_bounds__(64) void k_lq_panel(double* A, long long sA, int n, int ld, int i0, int np, double* Vb,
                                                  long long sV, int ldV, double* Tb, long long sT, int ldT) {
@@ -130,9 +133,14 @@ __global__ __launch_bounds__(64) void k_lq_panel(double* A, long long sA, int n,
   }
   auto reduce = [&]() {
     __syncthreads();
-    double s = 0.0;
+    double h[16];
 #pragma unroll
-    for (int t = 0; t < 16; ++t) s += red[rv][rs * 16 + t];
+    for (int t = 0; t < 16; ++t) h[t] = red[rv][rs * 16 + t];
+#pragma unroll
+    for (int w = 8; w > 0; w >>= 1)  // pairwise: four dependent adds instead of sixteen
+#pragma unroll
+      for (int t = 0; t < w; ++t) h[t] += h[t + w];
+    double s = h[0];
     s += __shfl_xor(s, 1, 64);
     s += __shfl_xor(s, 2, 64);
     if (rs == 0) tot[rv] = s;
@@ -144,24 +152,20 @@ __global__ __launch_bounds__(64) void k_lq_panel(double* A, long long sA, int n,
     betas[k] = 0.0;
     if (k < np) {
       __syncthreads();  // the previous step has read red, tot and colk
-      // value 0: the squared tail of pivot row k; value r > k: row r . tail of row k
+      // value 0: the squared tail of pivot row k; value r > k: row r . tail of row k.
+      // Slot 0 holds a tail column only in lanes beyond k: masked once, here.
+      const double pk0 = (lane > k) ? P[k][0] : 0.0;
       {
-        double sq = 0.0;
+        double sq = pk0 * pk0;
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-          const int c = lane + 64 * q;
-          sq = (c > k && c < nc) ? fma(P[k][q], P[k][q], sq) : sq;
-        }
+        for (int q = 1; q < NQ; ++q) sq = fma(P[k][q], P[k][q], sq);
         red[0][lane] = sq;
       }
 #pragma unroll
       for (int r = k + 1; r < kNB; ++r) {
-        double w = 0.0;
+        double w = P[r][0] * pk0;
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-          const int c = lane + 64 * q;
-          w = (c > k && c < nc) ? fma(P[r][q], P[k][q], w) : w;
-        }
+        for (int q = 1; q < NQ; ++q) w = fma(P[r][q], P[k][q], w);
         red[r][lane] = w;
       }
       if (lane == k) {
@@ -178,11 +182,11 @@ __global__ __launch_bounds__(64) void k_lq_panel(double* A, long long sA, int n,
         tau = (beta - alpha) / beta;
         scal = 1.0 / (alpha - beta);
       }
+      // the reflector in slot 0: 1 in the pivot lane, the scaled tail beyond it, 0 before
+      const double v0 = (lane > k) ? P[k][0] * scal : ((lane == k) ? 1.0 : 0.0);
+      P[k][0] = (lane > k) ? v0 : P[k][0];
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) {
-        const int c = lane + 64 * q;
-        P[k][q] = (c > k && c < nc) ? P[k][q] * scal : P[k][q];
-      }
+      for (int q = 1; q < NQ; ++q) P[k][q] *= scal;
       betas[k] = beta;
       if (lane == 0) taus[k] = tau;
       if (tau != 0.0) {
@@ -191,28 +195,29 @@ __global__ __launch_bounds__(64) void k_lq_panel(double* A, long long sA, int n,
           if (r < np) {
             // row r . v with v = [1, scal * tail]
             const double tw = tau * fma(scal, tot[r], colk[r]);
+            P[r][0] = fma(-tw, v0, P[r][0]);
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-              const int c = lane + 64 * q;
-              P[r][q] = (c > k && c < nc) ? fma(-tw, P[k][q], P[r][q]) : P[r][q];
-            }
-            if (lane == k) P[r][0] -= tw;
+            for (int q = 1; q < NQ; ++q) P[r][q] = fma(-tw, P[k][q], P[r][q]);
           }
         }
       }
     }
   }
-  // L rows back to A; V rows out
+  // L rows back to A; V rows out (slots q >= 1 hold columns 64 q + lane > k: zero in L, tail in V)
   double* Vo = Vb + (long long)blockIdx.x * sV;
 #pragma unroll
   for (int k = 0; k < kNB; ++k) {
     if (k < np) {
+      if (lane < nc) {
+        a[(size_t)k * ld + lane] = (lane < k) ? P[k][0] : ((lane == k) ? betas[k] : 0.0);
+        Vo[(size_t)k * ldV + lane] = (lane < k) ? 0.0 : ((lane == k) ? 1.0 : P[k][0]);
+      }
 #pragma unroll
-      for (int q = 0; q < NQ; ++q) {
+      for (int q = 1; q < NQ; ++q) {
         const int c = lane + 64 * q;
         if (c < nc) {
-          a[(size_t)k * ld + c] = (c < k) ? P[k][q] : ((c == k) ? betas[k] : 0.0);
-          Vo[(size_t)k * ldV + c] = (c < k) ? 0.0 : ((c == k) ? 1.0 : P[k][q]);
+          a[(size_t)k * ld + c] = 0.0;
+          Vo[(size_t)k * ldV + c] = P[k][q];
         }
       }
     }
@@ -222,14 +227,12 @@ __global__ __launch_bounds__(64) void k_lq_panel(double* A, long long sA, int n,
   for (int k = 1; k < kNB; ++k) {
     if (k < np) {
       __syncthreads();
+      const double pk0 = (lane > k) ? P[k][0] : 0.0;
 #pragma unroll
       for (int l = 0; l < k; ++l) {
-        double w = 0.0;
+        double w = P[l][0] * pk0;
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-          const int c = lane + 64 * q;
-          w = (c > k && c < nc) ? fma(P[l][q], P[k][q], w) : w;
-        }
+        for (int q = 1; q < NQ; ++q) w = fma(P[l][q], P[k][q], w);
         red[l][lane] = w;
       }
       if (lane == k) {
@@ -247,7 +250,7 @@ __global__ __launch_bounds__(64) void k_lq_panel(double* A, long long sA, int n,
   for (int k = 0; k < kNB; ++k) {
     double acc = 0.0;
 #pragma unroll
-    for (int l = 0; l < k; ++l) acc = (l >= lane) ? fma(trow[l], -taus[k] * G[l][k], acc) : acc;
+    for (int l = 0; l < k; ++l) acc = fma(trow[l], -taus[k] * G[l][k], acc);  // trow[l] = 0 for l < lane
     trow[k] = (k < np) ? ((lane < k) ? acc : ((lane == k) ? taus[k] : 0.0)) : 0.0;
   }
   if (lane < np) {
@@ -255,6 +258,127 @@ __global__ __launch_bounds__(64) void k_lq_panel(double* A, long long sA, int n,
 #pragma unroll
     for (int k = 0; k < kNB; ++k)
       if (k < np) T[k] = trow[k];
+  }
+}
+
+typedef double lq_d4 __attribute__((ext_vector_type(4)));
+
+// Trailing update of one panel of the blocked LQ, in one pass over the rows
+// below the panel: A_t <- A_t (I - V^T T V) for A_t = rows i0 + np .. i0 + np
+// + mr - 1 and columns i0 .. i0 + nc - 1 of an item, with V (np x nc) and T
+// (np x np) from k_lq_panel.  Per strip of 16 rows, on the FP64 matrix cores
+// (v_mfma_f64_16x16x4f64): W^T = V A_s^T, W2^T = T^T W^T, A_s^T -= V^T W2^T.
+// The strip stays in registers between the three products, so A_t is read
+// and written once per panel.
+//
+// Register layout of a strip: tile t holds columns 16 t .. 16 t + 15, lane
+// j + 16 g value v holds (row j, column 16 t + 4 g + v).  These registers are
+// at once the B operand of K-step v of W^T = V A_s^T (summation index 4 v + g
+// standing for column 4 g + v) and the accumulator of A_s^T's tile when its
+// row i stands for column 4 (i % 4) + i / 4 (the FP64 accumulator holds
+// element (l / 16 + 4 v, l % 16) in value v of lane l).  W^T's accumulator
+// value v is likewise the B operand of K-step v of the next product.
+//
+// Four waves per block, one strip each; V (zero-padded to 16 rows and 16 NT
+// columns) and T are staged in LDS.  Blocks are numbered so that the blocks of
+// one item run on one XCD (work-groups go round-robin over the 8 XCDs) and
+// share V through its L2.
+constexpr int kLqUpdWaves = 4;
+template <int NT, bool VEC>
+__global__ __launch_bounds__(64 * kLqUpdWaves, 2) void k_lq_update(double* A, long long sA, int ld, int i0, int np,
+                                                                    int mr, int nc, const double* Vb, long long sV,
+                                                                    int ldV, const double* Tb, long long sT, int ldT,
+                                                                    int cnt, int sbk) {
+  constexpr int LDV = 16 * NT + 1;
+  __shared__ double Vs[16 * LDV];
+  __shared__ double Ts[16 * 17];
+  const int L = blockIdx.x;
+  const int rr = L / 8;
+  const long long item = (long long)(rr / sbk) * 8 + (L % 8);
+  const int sb = rr % sbk;
+  if (item >= cnt) return;  // uniform over the block
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const double* vsrc = Vb + item * sV;
+  for (int e = tid; e < 16 * 16 * NT; e += 64 * kLqUpdWaves) {
+    const int p = e / (16 * NT), c = e % (16 * NT);
+    Vs[p * LDV + c] = (p < np && c < nc) ? vsrc[(size_t)p * ldV + c] : 0.0;
+  }
+  const double* tsrc = Tb + item * sT;
+  {
+    const int p = tid / 16, c = tid % 16;
+    Ts[p * 17 + c] = (p < np && c < np) ? tsrc[(size_t)p * ldT + c] : 0.0;
+  }
+  __syncthreads();
+  const int strip = sb * kLqUpdWaves + wave;
+  const int r0 = 16 * strip;
+  if (r0 >= mr) return;  // no barrier follows
+  const int j = lane & 15, g = lane >> 4;
+  const int nfull = nc / 16;           // tiles without a column beyond nc
+  const int ntr = (nc + 15) / 16;
+  const bool rowok = r0 + j < mr;
+  // a row past the trailing block reads the block's last row: every product below keeps
+  // the strip's rows apart (row j of the strip only meets column j of W^T), and it is not stored
+  double* arow = A + item * sA + (size_t)(i0 + np + min(r0 + j, mr - 1)) * ld + i0 + 4 * g;
+  lq_d4 X[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    if (t < nfull) {
+      if constexpr (VEC) {
+        const double2 lo = *(const double2*)(arow + 16 * t);
+        const double2 hi = *(const double2*)(arow + 16 * t + 2);
+        X[t] = lq_d4{lo.x, lo.y, hi.x, hi.y};
+      } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) X[t][v] = arow[16 * t + v];
+      }
+    } else if (t < ntr) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) X[t][v] = (16 * t + 4 * g + v < nc) ? arow[16 * t + v] : 0.0;
+    } else {
+      X[t] = lq_d4{0.0, 0.0, 0.0, 0.0};
+    }
+  }
+  // W^T (rows: panel rows p, columns: strip rows j) = V A_s^T
+  lq_d4 wt = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    if (t < ntr) {
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        wt = __builtin_amdgcn_mfma_f64_16x16x4f64(Vs[j * LDV + 16 * t + 4 * g + s], X[t][s], wt, 0, 0, 0);
+    }
+  }
+  // W2^T = T^T W^T
+  lq_d4 w2t = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int s = 0; s < 4; ++s) w2t = __builtin_amdgcn_mfma_f64_16x16x4f64(Ts[(4 * s + g) * 17 + j], wt[s], w2t, 0, 0, 0);
+  // A_s^T -= V^T W2^T: accumulator row i of tile t stands for column 16 t + 4 (i % 4) + i / 4
+  const int ci = 4 * (j & 3) + (j >> 2);
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    if (t < ntr) {
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        X[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-Vs[(4 * s + g) * LDV + 16 * t + ci], w2t[s], X[t], 0, 0, 0);
+    }
+  }
+  if (rowok) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      if (t < nfull) {
+        if constexpr (VEC) {
+          *(double2*)(arow + 16 * t) = double2{X[t][0], X[t][1]};
+          *(double2*)(arow + 16 * t + 2) = double2{X[t][2], X[t][3]};
+        } else {
+#pragma unroll
+          for (int v = 0; v < 4; ++v) arow[16 * t + v] = X[t][v];
+        }
+      } else if (t < ntr) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          if (16 * t + 4 * g + v < nc) arow[16 * t + v] = X[t][v];
+      }
+    }
   }
 }
 
@@ -658,6 +782,29 @@ __global__ void k_gemv(int tA, int tB, int M, int K, double alpha, const double*
   *c = (beta == 0.0) ? prod : fma(beta, *c, prod);
 }
 
+// The N = 1 case with A stored row-major (not transposed): one wave per row of
+// C, lanes along the row (coalesced), then a wave reduction.
+constexpr int kGemvRowWaves = 4;
+__global__ __launch_bounds__(64 * kGemvRowWaves) void k_gemv_rows(int tB, int M, int K, double alpha, const double* A,
+                                                                  long long sA, int lda, const double* B,
+                                                                  long long sB, int ldb, double beta, double* C,
+                                                                  long long sC, int ldc) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.y * kGemvRowWaves + (threadIdx.x >> 6);
+  if (i >= M) return;  // uniform per wave; no barrier follows
+  const long long z = blockIdx.x;
+  const double* ar = A + z * sA + (size_t)i * lda;
+  const double* Bz = B + z * sB;
+  double acc = 0.0;
+  for (int k = lane; k < K; k += 64) acc = fma(ar[k], tB ? Bz[k] : Bz[(size_t)k * ldb], acc);
+  acc = wave_sum(acc);
+  if (lane == 0) {
+    double* c = C + z * sC + (size_t)i * ldc;
+    const double prod = alpha * acc;
+    *c = (beta == 0.0) ? prod : fma(beta, *c, prod);
+  }
+}
+
 // ------------------------------------------------------------- host helpers
 struct Arena;
 double* arena_take(Arena* ar, size_t n);
@@ -693,8 +840,13 @@ struct Ops {
                  int cnt) {
     if (err || cnt <= 0 || M <= 0 || N <= 0) return;
     if (N == 1) {
-      hipLaunchKernelGGL(k_gemv, dim3(cnt, (M + 127) / 128), dim3(128), 0, st, tA ? 1 : 0, tB ? 1 : 0, M, K, alpha,
-                         A.p, A.s, A.ld, B.p, B.s, B.ld, beta, C.p, C.s, C.ld);
+      if (tA) {
+        hipLaunchKernelGGL(k_gemv, dim3(cnt, (M + 127) / 128), dim3(128), 0, st, 1, tB ? 1 : 0, M, K, alpha, A.p,
+                           A.s, A.ld, B.p, B.s, B.ld, beta, C.p, C.s, C.ld);
+      } else {
+        hipLaunchKernelGGL(k_gemv_rows, dim3(cnt, (M + kGemvRowWaves - 1) / kGemvRowWaves), dim3(64 * kGemvRowWaves),
+                           0, st, tB ? 1 : 0, M, K, alpha, A.p, A.s, A.ld, B.p, B.s, B.ld, beta, C.p, C.s, C.ld);
+      }
       check("k_gemv");
       return;
     }
@@ -739,9 +891,7 @@ struct Ops {
     const size_t mark = arena_mark(ar);
     V Vb{arena_take(ar, (size_t)cnt * kNB * n), (long long)kNB * n, n};
     V Tb{arena_take(ar, (size_t)cnt * kNB * kNB), (long long)kNB * kNB, kNB};
-    V Y{arena_take(ar, (size_t)cnt * m * kNB), (long long)m * kNB, kNB};
-    V Y2{arena_take(ar, (size_t)cnt * m * kNB), (long long)m * kNB, kNB};
-    if (!Vb.p || !Tb.p || !Y.p || !Y2.p) {
+    if (!Vb.p || !Tb.p) {
       fail("workspace (lq)", -2);
       return;
     }
@@ -765,14 +915,37 @@ struct Ops {
       check("k_lq_panel");
       const int mr = m - (i0 + np);
       if (mr <= 0) continue;
-      // A_trail (I - V^T T V): Y = A_trail V^T, Y2 = Y T, A_trail -= Y2 V
-      const V At = A.at(i0 + np, i0);
-      // the factorisation's own products stay FP64
-      gemm64(false, true, mr, np, nc, 1.0, At, Vb, 0.0, Y, cnt);
-      gemm64(false, false, mr, np, np, 1.0, Y, Tb, 0.0, Y2, cnt);
-      gemm64(false, false, mr, nc, np, -1.0, Y2, Vb, 1.0, At, cnt);
+      // A_trail (I - V^T T V) in one pass (FP64, as the rest of the factorisation)
+      lq_update(A, i0, np, mr, nc, Vb, Tb, cnt);
     }
     arena_release(ar, mark);
+  }
+  void lq_update(V A, int i0, int np, int mr, int nc, V Vb, V Tb, int cnt) {
+    const int strips = (mr + 15) / 16;
+    const int sbk = (strips + kLqUpdWaves - 1) / kLqUpdWaves;
+    const dim3 grid((unsigned)(((cnt + 7) / 8) * 8 * sbk));
+    const dim3 block(64 * kLqUpdWaves);
+    // 16-byte loads and stores when every row of the block starts on a 16-byte boundary
+    const bool vec = ((uintptr_t)A.p % 16 == 0) && (A.s % 2 == 0) && (A.ld % 2 == 0) && (i0 % 2 == 0);
+    if (nc <= 64) {
+      launch_update<4>(vec, grid, block, A, i0, np, mr, nc, Vb, Tb, cnt, sbk);
+    } else if (nc <= 128) {
+      launch_update<8>(vec, grid, block, A, i0, np, mr, nc, Vb, Tb, cnt, sbk);
+    } else {
+      launch_update<16>(vec, grid, block, A, i0, np, mr, nc, Vb, Tb, cnt, sbk);
+    }
+    check("k_lq_update");
+  }
+  template <int NT>
+  void launch_update(bool vec, dim3 grid, dim3 block, V A, int i0, int np, int mr, int nc, V Vb, V Tb, int cnt,
+                     int sbk) {
+    if (vec) {
+      hipLaunchKernelGGL((k_lq_update<NT, true>), grid, block, 0, st, A.p, A.s, A.ld, i0, np, mr, nc, Vb.p, Vb.s,
+                         Vb.ld, Tb.p, Tb.s, Tb.ld, cnt, sbk);
+    } else {
+      hipLaunchKernelGGL((k_lq_update<NT, false>), grid, block, 0, st, A.p, A.s, A.ld, i0, np, mr, nc, Vb.p, Vb.s,
+                         Vb.ld, Tb.p, Tb.s, Tb.ld, cnt, sbk);
+    }
   }
   // op(L) X = X in place (L lower n x n, X n x m)
   void trsm(bool trans, int n, int m, V L, V X, int cnt) {
@@ -1037,7 +1210,7 @@ size_t workspace_doubles(int D, int d, long long Ns, bool f32) {
   // per-call temporaries at the widest level (the filter combine), and the post step
   const size_t h = (size_t)(Ns / 2 + 1);
   w += h * (4 * DD + 2 * DD + 5 * DD + 3 * D) + 4096;
-  w += (size_t)(Ns + 1) * (2 * D * kNB + 2 * 2 * D * kNB + kNB * kNB) + 4096;  // blocked-LQ panels
+  w += (size_t)(Ns + 1) * (2 * D * kNB + kNB * kNB) + 4096;  // blocked-LQ panels (V, T)
   w += (size_t)Ns * (5 * DD + (size_t)d * D + 2 * D + 3 * d + 4) + 4096;
   w += 8 * n1 + 64 * 1024;
   // FP32 staging of one GEMM (both operands and the product; at most n1 items of D x D)
