@@ -326,6 +326,14 @@ __device__ __forceinline__ double from_sub_k(double v, int k) {
   return from_sub<S, 3>(v);
 }
 
+// The value of v in wavefront lane L (a compile-time lane): two scalar reads.
+__device__ __forceinline__ double readlane_d(double v, int L) {
+  const long long x = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane((int)x, L);
+  const int hi = __builtin_amdgcn_readlane((int)(x >> 32), L);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
 // x[t] = p[t S + sub] for columns < n, 0 for the padding columns.
 template <int S, int n, int NX>
 __device__ __forceinline__ void load_dist(double (&x)[NX], const double* p, int sub) {
@@ -580,8 +588,14 @@ __device__ __forceinline__ void tile_t_times_tile(int r, const double* T1, const
 // the band), so the reflector of pivot i is zero beyond column max(i, W2) of
 // the second block and those columns are left out of its sums and updates.
 // They would only add exact zeros: the factor is the same.
+//
+// WV is the worker's width when the caller knows it (0: unknown).  In a worker
+// that is a whole wavefront the pivot row's squared tail is read from the
+// pivot row's own dot product (its lanes multiply their row by a copy of it,
+// in the order of the chains below: the same bits) instead of a second chain
+// in every lane.
 template <int S, int NR, int B1, int B2, int NG = 1, int I0 = 0, int I1 = 1 << 20, bool FINISH = true,
-          int W2 = 1 << 20>
+          int W2 = 1 << 20, int WV = 0>
 __device__ __forceinline__ void lq(double (&a)[(padded<B1, S>() + padded<B2, S>()) / S], int row, int sub,
                                    double* bc) {
   static_assert(NG == 1 || NG == 2, "one or two matrices");
@@ -590,6 +604,9 @@ __device__ __forceinline__ void lq(double (&a)[(padded<B1, S>() + padded<B2, S>(
   constexpr int NX = NP / S;
   constexpr int NC = B1 + B2;
   constexpr int K = (NR < NC) ? NR : NC;
+  // a worker that is a whole wavefront (its width WV = 64 declared by the caller, or rows over
+  // more than 32 lanes): lane row S + sub of the worker is that lane of the wavefront
+  constexpr bool kFullWave = WV == 64 || NG * NR * S > 32;
   const int g = (NG == 2 && row >= NR) ? 1 : 0;
   const int li = row - g * NR;
   const bool valid = row < NG * NR;
@@ -616,27 +633,36 @@ __device__ __forceinline__ void lq(double (&a)[(padded<B1, S>() + padded<B2, S>(
     const double p0 = tl ? p[t0] : 0.0;
     // (CH independent chains each: one lane sums NX - t0 columns)
     constexpr int CH = (NX > 16) ? 4 : 2;
-    double sq[CH], dt[CH];
+    double dt[CH];
 #pragma unroll
-    for (int c = 0; c < CH; ++c) sq[c] = dt[c] = 0.0;
-    sq[0] = p0 * p0;
+    for (int c = 0; c < CH; ++c) dt[c] = 0.0;
     dt[0] = a[t0] * p0;
 #pragma unroll
-    for (int t = t0 + 1; t < te; ++t) {
-      sq[(t - t0) % CH] = fma(p[t], p[t], sq[(t - t0) % CH]);
-      dt[(t - t0) % CH] = fma(a[t], p[t], dt[(t - t0) % CH]);
-    }
-    double sqs = sq[0], dts = dt[0];
-    if constexpr (CH == 4) {
-      sqs = (sq[0] + sq[1]) + (sq[2] + sq[3]);
-      dts = (dt[0] + dt[1]) + (dt[2] + dt[3]);
-    } else {
-      sqs = sq[0] + sq[1];
-      dts = dt[0] + dt[1];
-    }
+    for (int t = t0 + 1; t < te; ++t) dt[(t - t0) % CH] = fma(a[t], p[t], dt[(t - t0) % CH]);
+    const double dts = (CH == 4) ? (dt[0] + dt[1]) + (dt[2] + dt[3]) : dt[0] + dt[1];
     const double ai = from_sub_k<S>(a[t0], k0);  // this row's entry in the pivot column
-    const double sig = rowsum<S>(sqs);
     const double dot = rowsum<S>(dts);
+    double sig;
+    if constexpr (kFullWave) {
+      // the pivot row's own dot is its squared tail (p is a copy of its tail), summed in the
+      // order below: read it from the row's first lane instead of a chain in every lane
+      const double s0 = readlane_d(dot, i * S);
+      if constexpr (NG == 2) {
+        const double s1 = readlane_d(dot, (NR + i) * S);
+        sig = g ? s1 : s0;
+      } else {
+        sig = s0;
+      }
+    } else {
+      double sq[CH];
+#pragma unroll
+      for (int c = 0; c < CH; ++c) sq[c] = 0.0;
+      sq[0] = p0 * p0;
+#pragma unroll
+      for (int t = t0 + 1; t < te; ++t) sq[(t - t0) % CH] = fma(p[t], p[t], sq[(t - t0) % CH]);
+      const double sqs = (CH == 4) ? (sq[0] + sq[1]) + (sq[2] + sq[3]) : sq[0] + sq[1];
+      sig = rowsum<S>(sqs);
+    }
     double nrm, rnrm;
     sqrt_rsqrt(fma(alpha, alpha, sig), nrm, rnrm);
     const double bc_ = -copysign(nrm, alpha);  // Fortran SIGN: honours the sign of -0.0
@@ -783,11 +809,11 @@ __device__ __forceinline__ void lq_virtual(double (&a)[B1 + B2], int row, double
 // lq() with the reduction order of a split over V lanes: lq<S> itself when the
 // row is split over V = S lanes, lq_virtual<V> on one lane per row.
 template <int S, int V, int NR, int B1, int B2, int NG = 1, int I0 = 0, int I1 = 1 << 20, bool FINISH = true,
-          int W2 = 1 << 20>
+          int W2 = 1 << 20, int WV = 0>
 __device__ __forceinline__ void lqv(double (&a)[(padded<B1, S>() + padded<B2, S>()) / S], int row, int sub,
                                     double* bc) {
   if constexpr (S == V) {
-    lq<S, NR, B1, B2, NG, I0, I1, FINISH, W2>(a, row, sub, bc);
+    lq<S, NR, B1, B2, NG, I0, I1, FINISH, W2, WV>(a, row, sub, bc);
   } else {
     static_assert(S == 1, "a reduction order of V lanes is reproduced on one lane per row");
     lq_virtual<V, NR, B1, B2, NG, I0, I1, FINISH, W2>(a, row, bc);
@@ -1154,7 +1180,7 @@ __device__ void filter_element(const Ctx<D, SS>& cx, const ModelArgs& a, int k,
         row[NL + t] = (r < D) ? lq_r[t] : 0.0;
       }
     }
-    lq<S, D, D, D>(row, r, s, cx.bc());
+    lq<S, D, D, D, 1, 0, 1 << 20, true, 1 << 20, L::W>(row, r, s, cx.bc());
     if (r < D) {
 #pragma unroll
       for (int t = 0; t < NL; ++t) T0[r * LD + t * S + s] = row[t];
@@ -1177,7 +1203,7 @@ __device__ void filter_element(const Ctx<D, SS>& cx, const ModelArgs& a, int k,
 #pragma unroll
     for (int t = 0; t < NLd; ++t) psi[NL + t] = (r < d) ? ob.r_at(r, t * S + s) : 0.0;
   }
-  lq<S, d + D, D, d>(psi, r, s, cx.bc());
+  lq<S, d + D, D, d, 1, 0, 1 << 20, true, 1 << 20, L::W>(psi, r, s, cx.bc());
 
   // Psi11 -> T1 (compact columns < d lie in the first block)
   if (r < d) store_dist<S, d>(T1 + r * LD, psi, s);
@@ -1251,7 +1277,7 @@ __device__ void filter_element(const Ctx<D, SS>& cx, const ModelArgs& a, int k,
     // nx <= ny: Z = tria(Zr) with Zr of shape D x d (D == d here)
     double zd[NLd];
     own<S, d>(z, s, zd);
-    lq<S, D, d, 0>(zd, r, s, cx.bc());
+    lq<S, D, d, 0, 1, 0, 1 << 20, true, 1 << 20, L::W>(zd, r, s, cx.bc());
 #pragma unroll
     for (int t = 0; t < NL; ++t) Z[t] = zd[t];
   }
@@ -1355,20 +1381,20 @@ __device__ void filter_combine(const Ctx<D, SS>& cx, const FiltArrays& x1, const
   if constexpr (state_only) {
     // Xi11 and Xi21 are final after the first D pivots; the last D pivots only
     // form Xi22, which enters Z alone
-    lqv<S, VS, 2 * D, D, D, 1, 0, D / 2, false, 0>(xr, r, s, cx.bc());
+    lqv<S, VS, 2 * D, D, D, 1, 0, D / 2, false, 0, L::W>(xr, r, s, cx.bc());
     load_dist<S, D>(a2, x2.A + (i2 * D + rr) * D, s);
     load_dist<S, D>(u2, x2.U + (i2 * D + rr) * D, s);
     b2r = x2.b[i2 * D + rr];
     zero(z1);
-    lqv<S, VS, 2 * D, D, D, 1, D / 2, D, true, 0>(xr, r, s, cx.bc());
+    lqv<S, VS, 2 * D, D, D, 1, D / 2, D, true, 0, L::W>(xr, r, s, cx.bc());
   } else {
-    lqv<S, VS, 2 * D, D, D, 1, 0, D, false, 0>(xr, r, s, cx.bc());
+    lqv<S, VS, 2 * D, D, D, 1, 0, D, false, 0, L::W>(xr, r, s, cx.bc());
     load_dist<S, D>(a2, x2.A + (i2 * D + rr) * D, s);
     load_dist<S, D>(u2, x2.U + (i2 * D + rr) * D, s);
     load_dist<S, D>(z1, x1.Z + (i1 * D + rz) * D, s);  // Z1 rows feed the rows D..2D-1
     b2r = x2.b[i2 * D + rr];
     eta1r = x1.eta[i1 * D + rr];
-    lqv<S, VS, 2 * D, D, D, 1, D>(xr, r, s, cx.bc());
+    lqv<S, VS, 2 * D, D, D, 1, D, 1 << 20, true, 1 << 20, L::W>(xr, r, s, cx.bc());
   }
   if (act) {
 #pragma unroll
@@ -1469,7 +1495,8 @@ __device__ void filter_combine(const Ctx<D, SS>& cx, const FiltArrays& x1, const
   }
   // bands: U2 is lower triangular; Z1 too, except a leaf element's [Zr | 0]
   // (nonzero in its first d columns only)
-  lqv<S, VS, D, D, D, state_only ? 1 : 2, 0, 1 << 20, true, state_only ? 0 : max_obs_dim(D) - 1>(jr, r, s, cx.bc());
+  lqv<S, VS, D, D, D, state_only ? 1 : 2, 0, 1 << 20, true, state_only ? 0 : max_obs_dim(D) - 1, L::W>(jr, r, s,
+                                                                                                        cx.bc());
   // output offsets formed here, not hoisted above the factorisation (where
   // they would hold registers through it)
   o = opaque_sz(o);
@@ -1547,7 +1574,7 @@ __device__ void smoother_combine(const Ctx<D, SS>& cx, const SmoArrays& x1, cons
       dr[t] = t1[t];
       dr[NL + t] = d2[t];
     }
-    lqv<S, VS, D, D, D>(dr, r, s, cx.bc());
+    lqv<S, VS, D, D, D, 1, 0, 1 << 20, true, 1 << 20, L::W>(dr, r, s, cx.bc());
   }
   if (act) {
     if (s == 0) (state_only ? so.m : y.g)[o * D + r] = gg;

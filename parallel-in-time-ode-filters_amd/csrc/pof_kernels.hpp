@@ -306,14 +306,14 @@ __global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK), (post_min_waves<D, EK1>(
   // second block [LQ; 0]: banded when LQ is lower triangular (the IWP prior's
   // Cholesky factor; a caller's model is checked on the host)
   if (g.lq_lower) {
-    lq<S, 2 * D, D, D, 1, 0, D, false, 0>(ph, r, sb, cx.bc());
+    lq<S, 2 * D, D, D, 1, 0, D, false, 0, L::W>(ph, r, sb, cx.bc());
   } else {
-    lq<S, 2 * D, D, D, 1, 0, D, false>(ph, r, sb, cx.bc());
+    lq<S, 2 * D, D, D, 1, 0, D, false, 1 << 20, L::W>(ph, r, sb, cx.bc());
   }
   if (wc) {
-    lq<S, 2 * D, D, D, 1, D>(ph, r, sb, cx.bc());  // Phi22
+    lq<S, 2 * D, D, D, 1, D, 1 << 20, true, 1 << 20, L::W>(ph, r, sb, cx.bc());  // Phi22
   } else {
-    lq<S, 2 * D, D, D, 1, D, D>(ph, r, sb, cx.bc());  // no pivot: clear the strict upper part
+    lq<S, 2 * D, D, D, 1, D, D, true, 1 << 20, L::W>(ph, r, sb, cx.bc());  // no pivot: clear the strict upper part
   }
   if (dm_only) {
     if (r >= D && r < 2 * D) {
@@ -365,7 +365,7 @@ __global__ __launch_bounds__((Geo<D, kLdsPost>::BLOCK), (post_min_waves<D, EK1>(
 #pragma unroll
     for (int t = 0; t < NLd; ++t) sr[NL + t] = (r < d) ? ob.r_at(r, t * S + sb) : 0.0;
   }
-  lq<S, d, D, d>(sr, r, sb, cx.bc());
+  lq<S, d, D, d, 1, 0, 1 << 20, true, 1 << 20, L::W>(sr, r, sb, cx.bc());
   if (r < d) store_dist<S, d>(TS + r * LD, sr, sb);
   wsync();
   if (r == 0 && sb == 0) {
