@@ -107,6 +107,7 @@ constexpr int kNB = 16;  // rows per panel of the blocked LQ
 // k < kNB < 64 lives in slot 0 of lane k, so only slot 0 needs the test c > k;
 // columns c >= n - i0 are loaded as zeros and stay zero (every update adds a
 // multiple of the pivot row, zero there too), so they need no test.
+typedef double lq_panel_d4 __attribute__((ext_vector_type(4)));
 template <int NQ>
 __global__ __launch_bounds__(64) void k_lq_panel(double* A, long long sA, int n, int ld, int i0, int np, double* Vb,
                                                  long long sV, int ldV, double* Tb, long long sT, int ldT) {
@@ -116,7 +117,7 @@ __global__ __launch_bounds__(64) void k_lq_panel(double* A, long long sA, int n,
   __shared__ double red[kNB][65];
   __shared__ double tot[kNB];
   __shared__ double colk[kNB];  // column k of the panel rows (held by lane k, slot 0)
-  __shared__ double G[kNB][kNB];
+  __shared__ double G[kNB][kNB + 1];
   __shared__ double taus[kNB];
   const int lane = threadIdx.x;
   const int rv = lane >> 2, rs = lane & 3;  // value and quarter this lane reduces
@@ -222,38 +223,42 @@ __global__ __launch_bounds__(64) void k_lq_panel(double* A, long long sA, int n,
       }
     }
   }
-  // G[l][k] = v_l . v_k (l < k) = v_l[k] + sum_{c > k} v_l[c] v_k[c], one batch per k
+  // T (dlarft, forward / rowwise): G = V V^T on the matrix cores, with V read back from the
+  // rows just stored (L2; the panel's registers are free by then), 64 columns per batch of
+  // loads and four accumulation chains (a Gram matrix sums over every column, so the order
+  // of the columns is free); then lane j forms row j of T:
+  // T[j][k] = -tau_k sum_{l = j}^{k - 1} T[j][l] G[l][k], T[k][k] = tau_k
+  __threadfence_block();
+  __syncthreads();
+  const int gj = lane & 15, gg = lane >> 4;
+  const double* vr = Vo + (size_t)gj * ldV;
+  lq_panel_d4 ga[4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
 #pragma unroll
-  for (int k = 1; k < kNB; ++k) {
-    if (k < np) {
-      __syncthreads();
-      const double pk0 = (lane > k) ? P[k][0] : 0.0;
+  for (int q = 0; q < NQ; ++q) {
+    if (64 * q < nc) {
+      double x[16];
 #pragma unroll
-      for (int l = 0; l < k; ++l) {
-        double w = P[l][0] * pk0;
-#pragma unroll
-        for (int q = 1; q < NQ; ++q) w = fma(P[l][q], P[k][q], w);
-        red[l][lane] = w;
+      for (int e = 0; e < 16; ++e) {
+        const int c = 64 * q + 16 * (e >> 2) + 4 * gg + (e & 3);
+        x[e] = (gj < np && c < nc) ? vr[c] : 0.0;
       }
-      if (lane == k) {
 #pragma unroll
-        for (int l = 0; l < kNB; ++l) colk[l] = P[l][0];
-      }
-      reduce();
-      if (lane < k) G[lane][k] = tot[lane] + colk[lane];
+      for (int e = 0; e < 16; ++e) ga[e & 3] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[e], x[e], ga[e & 3], 0, 0, 0);
     }
   }
+  // the FP64 accumulator holds element (l / 16 + 4 v, l % 16) in value v of lane l
+#pragma unroll
+  for (int v = 0; v < 4; ++v) G[gg + 4 * v][gj] = (ga[0][v] + ga[1][v]) + (ga[2][v] + ga[3][v]);
   __syncthreads();
-  // T (dlarft): lane j holds row j; T[j][k] = sum_{l = j}^{k - 1} T[j][l] (-tau_k G[l][k]), T[k][k] = tau_k
-  double trow[kNB];
-#pragma unroll
-  for (int k = 0; k < kNB; ++k) {
-    double acc = 0.0;
-#pragma unroll
-    for (int l = 0; l < k; ++l) acc = fma(trow[l], -taus[k] * G[l][k], acc);  // trow[l] = 0 for l < lane
-    trow[k] = (k < np) ? ((lane < k) ? acc : ((lane == k) ? taus[k] : 0.0)) : 0.0;
-  }
   if (lane < np) {
+    double trow[kNB];
+#pragma unroll
+    for (int k = 0; k < kNB; ++k) {
+      double acc = 0.0;
+#pragma unroll
+      for (int l = 0; l < k; ++l) acc = fma(trow[l], -taus[k] * G[l][k], acc);  // trow[l] = 0 for l < lane
+      trow[k] = (k < np) ? ((lane < k) ? acc : ((lane == k) ? taus[k] : 0.0)) : 0.0;
+    }
     double* T = Tb + (long long)blockIdx.x * sT + (size_t)lane * ldT;
 #pragma unroll
     for (int k = 0; k < kNB; ++k)
@@ -303,18 +308,16 @@ __global__ __launch_bounds__(64 * kLqUpdWaves, 2) void k_lq_update(double* A, lo
     const int p = e / (16 * NT), c = e % (16 * NT);
     Vs[p * LDV + c] = (p < np && c < nc) ? vsrc[(size_t)p * ldV + c] : 0.0;
   }
-  const double* tsrc = Tb + item * sT;
   {
     const int p = tid / 16, c = tid % 16;
-    Ts[p * 17 + c] = (p < np && c < np) ? tsrc[(size_t)p * ldT + c] : 0.0;
+    Ts[p * 17 + c] = (p < np && c < np) ? Tb[item * sT + (size_t)p * ldT + c] : 0.0;
   }
   __syncthreads();
-  const int strip = sb * kLqUpdWaves + wave;
-  const int r0 = 16 * strip;
+  const int r0 = 16 * (sb * kLqUpdWaves + wave);
   if (r0 >= mr) return;  // no barrier follows
   const int j = lane & 15, g = lane >> 4;
-  const int nfull = nc / 16;           // tiles without a column beyond nc
   const int ntr = (nc + 15) / 16;
+  const int nfull = nc / 16;  // tiles without a column beyond nc
   const bool rowok = r0 + j < mr;
   // a row past the trailing block reads the block's last row: every product below keeps
   // the strip's rows apart (row j of the strip only meets column j of W^T), and it is not stored

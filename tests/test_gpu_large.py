@@ -38,12 +38,14 @@ def _ssq_tol(oivp, ts, q, init, s_ref, fixed_iters):
     from oracle import problems as op
     from oracle import solver as osolver
 
+    # sigma_squared of d >= 2 depends on factor signs that rounding decides: its tolerance is
+    # twice the oracle's own spread over six relative perturbations of y0 at 1e-15 .. 1e-14
     spread = 0.0
-    for eps in (1e-15, -1e-15, 3e-15):
+    for eps in (1e-15, -1e-15, 3e-15, -3e-15, 1e-14, -1e-14):
         pp = op.Problem(oivp.name, oivp.params, oivp.y0 * (1 + eps), oivp.t0, oivp.tmax)
         _, inf = osolver.solve(pp, ts, q, init=init, maxiters=200, fixed_iters=fixed_iters)
         spread = max(spread, abs(inf["sigma_squared"] - s_ref))
-    return max(1e-8 * abs(s_ref), 1.5 * spread)
+    return max(1e-8 * abs(s_ref), 2.0 * spread)
 
 
 @pytest.mark.parametrize("name,kw,N,q,init", [
