@@ -153,8 +153,9 @@ int pof_tuning_get(const pof_ctx* ctx, int D, int32_t* widths);
 int pof_solve(pof_ctx* ctx, const pof_problem* prob, const pof_opts* opts, pof_out* out, void* stream);
 
 /* The same solve on the large-state engine: every scan level as batched
- * primitives (a Householder LQ kernel, one workgroup per matrix, and rocBLAS
- * strided-batched DGEMM and a triangular-solve kernel), for state dimensions beyond the per-worker
+ * primitives of this library (a blocked Householder LQ whose trailing updates
+ * run on the FP64 matrix cores, a strided-batched FP64 MFMA GEMM and a
+ * triangular-solve kernel), for state dimensions beyond the per-worker
  * kernels (Lorenz-96 with d = 32, q = 3: D = 128).  pof_solve uses it by itself
  * for such problems; it runs any (D, d) whose field it can linearise (every
  * pof_supported pair, and Lorenz-96 for D <= 128).  Parallel IEKS only;

@@ -1690,7 +1690,7 @@ static int solve_large(pof_ctx* ctx, const pof_problem* prob, const pof_opts* op
   HIP_TRY(hipSetDevice(ctx->device));
   if (!ctx->large) {
     ctx->large = pof::large::engine_create(ctx->device);
-    if (!ctx->large) return set_err(ctx, POF_E_HIP, "rocBLAS handle creation failed");
+    if (!ctx->large) return set_err(ctx, POF_E_HIP, "large-state engine creation failed");
   }
   pof::large::Problem lp{};
   lp.D = s.D;

@@ -1,9 +1,10 @@
 // Large-state IEKS engine: the same solve as the per-element kernels, for state
 // dimensions whose elements do not fit one worker's registers (Lorenz-96 with
 // d = 32, q = 3: D = 128).  Every scan level runs as batched primitives over all
-// its combines: a hand-written Householder LQ kernel (one workgroup per matrix),
-// rocBLAS strided-batched GEMM for the plain products, a triangular-solve kernel, and
-// small element-wise kernels.  The host drives JAX's odd/even recursion.
+// its combines: a blocked Householder LQ (panel kernel plus a one-pass trailing
+// update on the FP64 matrix cores), a strided-batched MFMA GEMM for the plain
+// products, a triangular-solve kernel, and small element-wise kernels.  The host
+// drives JAX's odd/even recursion.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -29,7 +30,7 @@ struct Problem {
   const double* m0;   // [Ns + 1, D] initial trajectory (preconditioned)
 };
 
-struct Engine;  // per-context state (rocBLAS handle, workspace)
+struct Engine;  // per-context state (device, workspace)
 
 Engine* engine_create(int device);
 void engine_destroy(Engine* e);
