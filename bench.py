@@ -21,7 +21,8 @@ dominant kernel class (algorithmic flops per launch over its mean launch
 time, measured with HIP events on the launch stream in an instrumented
 solve after the timed region), and the CPU baseline: the FP64 NumPy
 restatement of the reference algorithm (the oracle) timed on the host on
-one full-size IEKS iteration (solves/s = 1 / (10 x that time)).
+one full-size IEKS iteration in each of as many processes as the host has
+cores (solves/s = processes / (10 x the mean iteration time)).
 """
 import argparse
 import json
@@ -834,7 +835,7 @@ def main():
             print(json.dumps({"converged": converged, "accuracy": accuracy}), file=sys.stderr, flush=True)
     batched = served = None
     if not args.no_batch:
-        batched = batched_run(args.batch_ivps, maxiters=100, plain_sample=32, cpu=not args.no_cpu_baseline)
+        batched = batched_run(args.batch_ivps, maxiters=100, plain_sample=256, cpu=not args.no_cpu_baseline)
         if args.verbose:
             print(json.dumps(batched), file=sys.stderr, flush=True)
         # mean Ns of log-uniform [256, 65536] over [256, 16384]: 3.04x more work per request
