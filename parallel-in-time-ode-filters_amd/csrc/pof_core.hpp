@@ -132,17 +132,29 @@ constexpr int kind_width() {
   return is_throughput_kind(K) ? pow2_at_least(2 * D, 8) : 0;
 }
 
+// State dimensions whose scan combines form their D x D tile products on the FP64 matrix cores
+// (v_mfma_f64_16x16x4f64, one 16 x 16 tile): a whole-wavefront worker (rows over two lanes,
+// D > 8) whose LDS takes one more tile without losing occupancy.  (D = 9 .. 11 keep their
+// throughput forms, which the autotuner picks on wide levels and whose results must stay bitwise
+// equal to the latency form's; D = 15 would lose a block per CU to the extra tile.)
+template <int D>
+constexpr bool mfma_products() {
+  return D == 12 || D == 16;
+}
+
 // 1 when the throughput kinds differ from the latency-oriented ones (rows split over several lanes)
 template <int D>
 constexpr bool has_throughput_kinds() {
-  return split_of<D>() > 1;
+  return split_of<D>() > 1 && !mfma_products<D>();
 }
 
 template <int D_, int K = kLdsFull>
 struct Geo : Lay<D_, kind_split<D_, K>(), kind_width<D_, K>()> {
   using L = Lay<D_, kind_split<D_, K>(), kind_width<D_, K>()>;
   static constexpr int TILE = D_ * L::LD;
-  static constexpr int NT = LdsNeeds<K>::NT;        // tiles per worker
+  // tiles per worker; the combines with matrix-core products stage operands and results in one more
+  static constexpr int NT =
+      LdsNeeds<K>::NT + ((mfma_products<D_>() && (base_kind(K) == kLdsFull || base_kind(K) == kLdsState)) ? 1 : 0);
   static constexpr int NV = LdsNeeds<K>::NV;        // broadcast vectors per worker (2D entries each)
   // pivot-row broadcast of two matrices of 2 DP columns; the throughput kinds broadcast rows in the
   // padded layout of the latency kinds' split (whose reduction order they reproduce)
@@ -562,6 +574,47 @@ __device__ __forceinline__ void tile_t_times_tile(int r, const double* T1, const
       }
     }
     pin(out);
+  }
+}
+
+// Out = op(A) op(B) for D x D tiles in LDS (row stride LD) on the FP64 matrix
+// cores, D <= 16: one v_mfma_f64_16x16x4f64 per four summation indices, every
+// lane of the wavefront (the worker) taking part.  Lane l supplies A[l % 16][k]
+// and B[k][l % 16] for k = 4 step + l / 16, and its accumulator value v is
+// element (l / 16 + 4 v, l % 16); rows and columns past D are zero and not
+// written.  TA: A is stored transposed (A[i][k] = Ap[k LD + i]); TB likewise.
+// Out may be an operand's tile: the operands' reads feed the products, so they
+// have completed before the first write.  The caller syncs the worker before
+// reading Out.
+typedef double f64x4_t __attribute__((ext_vector_type(4)));
+template <int D, int LD, bool TA, bool TB>
+__device__ __forceinline__ void tile_mfma(const double* Ap, const double* Bp, double* Out) {
+  static_assert(D <= 16, "one 16 x 16 tile");
+  const int lane = threadIdx.x & 63;
+  const int i = lane & 15, kq = lane >> 4;
+  f64x4_t acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int st = 0; st < (D + 3) / 4; ++st) {
+    const int k = 4 * st + kq;
+    const bool ok = k < D && i < D;
+    const int kc = ok ? k : 0, ic = ok ? i : 0;
+    const double a = TA ? Ap[kc * LD + ic] : Ap[ic * LD + kc];
+    const double b = TB ? Bp[ic * LD + kc] : Bp[kc * LD + ic];
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ok ? a : 0.0, ok ? b : 0.0, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int row = kq + 4 * v;
+    if (row < D && i < D) Out[row * LD + i] = acc[v];
+  }
+}
+
+// The row (split) x of each lane's row into tile T (rows < n of the worker's lanes that hold them).
+template <int S, int n, int LD, int NX>
+__device__ __forceinline__ void stage_row(double* T, int r, int sub, const double (&x)[NX]) {
+  if (r < n) {
+#pragma unroll
+    for (int t = 0; t < NX; ++t) T[r * LD + t * S + sub] = x[t];  // padded columns < LD hold zeros
   }
 }
 
@@ -1362,10 +1415,20 @@ __device__ void filter_combine(const Ctx<D, SS>& cx, const FiltArrays& x1, const
     }
   }
   // Xi_ = [[U1^T Z2, I], [Z2, 0]]: rows 0..2D-1, column blocks (D, D)
+  // the combines of mfma_products<D>() (latency form) form their tile products on the matrix
+  // cores, staging row operands and results in one more LDS tile
+  constexpr bool MP = mfma_products<D>() && S == split_of<D>();
+  double* TS = MP ? cx.tile(state_only ? 4 : 5) : nullptr;
   double xr[2 * NL];
   {
     double t1[NL];
-    tile_t_times_tile<S, D, D, LD>(rr, TU1, TZ2, s, t1);
+    if constexpr (MP) {
+      tile_mfma<D, LD, true, false>(TU1, TZ2, TX11);  // U1^T Z2 (TX11 is free until Xi11 is stored)
+      wsync();
+      load_dist<S, D>(t1, TX11 + rr * LD, s);
+    } else {
+      tile_t_times_tile<S, D, D, LD>(rr, TU1, TZ2, s, t1);
+    }
     double zr[NL];
     const int r2 = (r >= D && r < 2 * D) ? r - D : 0;
     load_dist<S, D>(zr, TZ2 + r2 * LD, s);
@@ -1410,13 +1473,32 @@ __device__ void filter_combine(const Ctx<D, SS>& cx, const FiltArrays& x1, const
 
   // MT = A2 U1 Xi11^-T ; W = A2 - MT Xi21^T
   double mt[NL], wr[NL];
-  {
+  if constexpr (MP) {
+    stage_row<S, D, LD>(TS, r, s, a2);
+    wsync();
+    tile_mfma<D, LD, false, false>(TS, TU1, TS);  // A2 U1
+    wsync();
+    load_dist<S, D>(mt, TS + rr * LD, s);
+  } else {
     double a2f[D];
     gather<S, D>(a2, a2f);
     row_times_tile<S, D, D, LD>(a2f, TU1, s, mt);
   }
   rsolve_lowerT_s<S, D, LD>(mt, TX11, s);
-  {
+  if constexpr (MP) {
+    wsync();  // every row has read A2 U1
+    stage_row<S, D, LD>(TS, r, s, mt);
+    wsync();
+    tile_mfma<D, LD, false, true>(TS, TX21, TS);  // MT Xi21^T
+    wsync();
+    double p[NL];
+    load_dist<S, D>(p, TS + rr * LD, s);
+#pragma unroll
+    for (int t = 0; t < NL; ++t) {
+      wr[t] = (t * S + s < D) ? a2[t] - p[t] : 0.0;
+      pin1(wr[t]);
+    }
+  } else {
     double mtf[D];
     gather<S, D>(mt, mtf);
 #pragma unroll
@@ -1465,7 +1547,13 @@ __device__ void filter_combine(const Ctx<D, SS>& cx, const FiltArrays& x1, const
     wsync();
     eo = eta1r + lds_dot<S, D, LD, VS>(TA1 + rr, Vs3, s);
     pin1(eo);
-    {
+    if constexpr (MP) {
+      wsync();  // every row has read MT Xi21^T
+      stage_row<S, D, LD>(TS, r, s, wr);
+      wsync();
+      tile_mfma<D, LD, false, false>(TS, TA1, TU1);  // A = W A1 into U1's tile: U1's last reads were above
+      wsync();
+    } else {
       double wf[D];
       gather<S, D>(wr, wf);
       double ar[NL];
@@ -1482,7 +1570,11 @@ __device__ void filter_combine(const Ctx<D, SS>& cx, const FiltArrays& x1, const
   double jr[2 * NL];
   {
     double tz[NL];
-    if constexpr (!state_only) {
+    if constexpr (!state_only && MP) {
+      tile_mfma<D, LD, true, false>(TA1, TX22, TS);  // A1^T Xi22
+      wsync();
+      load_dist<S, D>(tz, TS + rz * LD, s);
+    } else if constexpr (!state_only) {
       tile_t_times_tile<S, D, D, LD>(rz, TA1, TX22, s, tz);
     } else {
       zero(tz);

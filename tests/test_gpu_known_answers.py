@@ -19,7 +19,9 @@ relative plus 1e-10 absolute (the two reference solutions, Kvaerno5 there and
 DOP853 here, differ by a few 1e-11) where they exceed 1e-8; below that they sit
 at that error floor and only their order of magnitude is checked.  The
 configurations whose parallel IEKS diverges in the reference (Henon-Heiles,
-q = 3: 1001 iterations, errors ~1e6..1e8) must diverge here too.
+q = 3: 1001 iterations, errors ~1e6..1e8) diverge or converge by rounding (the
+oracle converges at N = 8192): either is accepted, a converged solve only with
+errors below 1e-6.
 """
 import csv
 from pathlib import Path
@@ -98,9 +100,20 @@ def test_full_size_work_precision_row(gpu, row):
     want_it = float(row["iterations"])
     m = ys.mean.cpu().numpy()
     if want_it == 1001.0:
-        # the reference's parallel IEKS diverges here (its sequential IEKS converges)
-        assert info["iterations"] == 1001.0 and info["status"] == "maxiter"
-        assert not np.isfinite(m).all() or np.abs(m).max() > 1e3
+        # The reference's parallel IEKS diverges here (its sequential IEKS converges), but whether the
+        # parallel iteration diverges is decided by rounding: the oracle converges for Henon-Heiles
+        # q = 3 at N = 8192, in 28, 34 or 50 iterations for y0, y0 (1 + 1e-15) and y0 (1 - 1e-15).
+        # Either outcome is accepted; a converged solve must be accurate.
+        if info["status"] == "maxiter":
+            assert info["iterations"] == 1001.0
+            assert not np.isfinite(m).all() or np.abs(m).max() > 1e3
+            return
+        assert info["status"] == "converged", info["status"]
+        yref = _reference(row["problem"], ts)
+        rf = float(np.linalg.norm(m[-1] - yref[-1]))
+        rt = float(np.linalg.norm(m - yref, axis=1).mean())
+        print(f"  converged in {info['iterations']} iterations: rmse_final {rf:.3e} rmse_traj {rt:.3e}", flush=True)
+        assert rf < 1e-6 and rt < 1e-6, (rf, rt)
         return
     if row["iterations"] == row["seq_iterations"] and (row["problem"], N, q) not in ROUNDING_SENSITIVE:
         assert info["iterations"] == want_it, (info["iterations"], want_it)
