@@ -154,7 +154,10 @@ struct Geo : Lay<D_, kind_split<D_, K>(), kind_width<D_, K>()> {
   static constexpr int TILE = D_ * L::LD;
   // tiles per worker; the combines with matrix-core products stage operands and results in one more
   static constexpr int NT =
-      LdsNeeds<K>::NT + ((mfma_products<D_>() && (base_kind(K) == kLdsFull || base_kind(K) == kLdsState)) ? 1 : 0);
+      LdsNeeds<K>::NT + ((mfma_products<D_>() && (base_kind(K) == kLdsFull || base_kind(K) == kLdsState ||
+                                                   base_kind(K) == kLdsSmooth))
+                             ? 1
+                             : 0);
   static constexpr int NV = LdsNeeds<K>::NV;        // broadcast vectors per worker (2D entries each)
   // pivot-row broadcast of two matrices of 2 DP columns; the throughput kinds broadcast rows in the
   // padded layout of the latency kinds' split (whose reduction order they reproduce)
@@ -1653,14 +1656,30 @@ __device__ void smoother_combine(const Ctx<D, SS>& cx, const SmoArrays& x1, cons
   // results stay in registers; every global store is issued at the end so
   // that no wait on outstanding memory operations lands inside the LQ below
   const double gg = g2r + row_dot<S, D, VS>(e2, Vg1, s);
-  double e2f[D];
-  gather<S, D>(e2, e2f);
+  // E2 E1 and E2 D1: on the matrix cores for mfma_products<D>() (latency form), E2's rows staged in
+  // the extra tile and each product written over its right operand's tile
+  constexpr bool MP = mfma_products<D>() && S == split_of<D>();
   double er[state_only ? 1 : NL];
-  if constexpr (!state_only) row_times_tile<S, D, D, LD>(e2f, TE1, s, er);
+  double t1[NL];
+  if constexpr (MP) {
+    if (!state_only || with_chol) {
+      double* TS = cx.tile(2);
+      stage_row<S, D, LD>(TS, r, s, e2);
+      wsync();
+      if constexpr (!state_only) tile_mfma<D, LD, false, false>(TS, TE1, TE1);
+      if (with_chol) tile_mfma<D, LD, false, false>(TS, TD1, TD1);
+      wsync();
+      if constexpr (!state_only) load_dist<S, D>(er, TE1 + rr * LD, s);
+      if (with_chol) load_dist<S, D>(t1, TD1 + rr * LD, s);
+    }
+  } else {
+    double e2f[D];
+    gather<S, D>(e2, e2f);
+    if constexpr (!state_only) row_times_tile<S, D, D, LD>(e2f, TE1, s, er);
+    if (with_chol) row_times_tile<S, D, D, LD>(e2f, TD1, s, t1);
+  }
   double dr[2 * NL];
   if (with_chol) {
-    double t1[NL];
-    row_times_tile<S, D, D, LD>(e2f, TD1, s, t1);
 #pragma unroll
     for (int t = 0; t < NL; ++t) {
       dr[t] = t1[t];
