@@ -133,13 +133,15 @@ constexpr int kind_width() {
 }
 
 // State dimensions whose scan combines form their D x D tile products on the FP64 matrix cores
-// (v_mfma_f64_16x16x4f64, one 16 x 16 tile): a whole-wavefront worker (rows over two lanes,
-// D > 8) whose LDS takes one more tile without losing occupancy.  (D = 9 .. 11 keep their
-// throughput forms, which the autotuner picks on wide levels and whose results must stay bitwise
-// equal to the latency form's; D = 15 would lose a block per CU to the extra tile.)
+// (v_mfma_f64_16x16x4f64, one 16 x 16 tile): whole-wavefront workers (rows over two lanes,
+// D > 8) that stage operands in one more LDS tile.  D = 9 .. 11 keep their throughput forms,
+// which the autotuner picks on wide levels and whose results must stay bitwise equal to the
+// latency form's (the matrix core sums in an order of its own).  At D = 15 the extra tile costs
+// one of twelve resident up-sweep workers per CU (LDS), and the products save 14% of the
+// up-sweep's instructions: the batch's D = 15 group gained (6.26 -> 6.16 s for the batch).
 template <int D>
 constexpr bool mfma_products() {
-  return D == 12 || D == 16;
+  return D == 12 || D == 15 || D == 16;
 }
 
 // 1 when the throughput kinds differ from the latency-oriented ones (rows split over several lanes)
@@ -169,7 +171,21 @@ struct Geo : Lay<D_, kind_split<D_, K>(), kind_width<D_, K>()> {
   // workers per block: as many as fit in 64 KiB of LDS, at most 256 threads
   // (throughput kinds: the largest power of two up to 256 threads that fits)
   static constexpr int wpb_fit(int w) { return (w <= 1 || bytes_for(w) <= 65536) ? w : wpb_fit(w / 2); }
+  // resident workers per CU allowed by its 160 KiB of LDS with w workers per block (counted up to
+  // 16, four waves per SIMD, beyond which these kernels' registers limit them)
+  static constexpr int lds_workers(int w) {
+    return ((163840 / (int)bytes_for(w)) * w < 16) ? (163840 / (int)bytes_for(w)) * w : 16;
+  }
+  // the kinds with the extra tile of the matrix-core products: the block size (4, 2 or 1 whole-
+  // wavefront workers) that keeps the most workers resident by LDS
+  static constexpr bool EXTRA = NT > LdsNeeds<K>::NT;
+  static constexpr int wpb_lds() {
+    return (lds_workers(4) >= lds_workers(2) && lds_workers(4) >= lds_workers(1)) ? 4
+           : (lds_workers(2) >= lds_workers(1))                                  ? 2
+                                                                                 : 1;
+  }
   static constexpr int WPB = is_throughput_kind(K)                   ? wpb_fit(256 / L::W)
+                             : EXTRA                                 ? wpb_lds()
                              : (L::W == 32 && bytes_for(8) <= 65536) ? 8
                              : (bytes_for(256 / L::W) <= 65536)      ? 256 / L::W
                              : (bytes_for(2) <= 65536)               ? 2
