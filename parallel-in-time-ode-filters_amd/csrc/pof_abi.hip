@@ -2058,46 +2058,11 @@ int pof_solve_batch(pof_ctx* ctx, const pof_problem* probs, const pof_opts* opts
   }
   // iterate every group on its own stream, each at its own pace: a group's next
   // iteration is enqueued as soon as its previous one has been finished, so a
-  // group with short iterations never waits for one with long ones.  Without
-  // per-launch timing (which shares the context's timer) every group runs on a
-  // host thread of its own: the blocking waits of one group (an iteration's
-  // convergence data, the factor pass and output of retiring IVPs) then never
-  // delay the next iteration of another.  The groups share no mutable host
-  // state but the context's error message (locked).
-  int n_run = 0;
-  for (auto& g : groups) n_run += pof_batch_active(g.b) > 0 ? 1 : 0;
-  if (n_run > 1 && !ctx->profiling) {
-    std::vector<int> rcs(groups.size(), POF_OK);
-    std::atomic<bool> stop{false};
-    std::vector<std::thread> th;
-    th.reserve(groups.size());
-    for (size_t k = 0; k < groups.size(); ++k) {
-      if (pof_batch_active(groups[k].b) == 0) continue;
-      th.emplace_back([&, k]() {
-        pof_batch* b = groups[k].b;
-        const hipError_t he = hipSetDevice(ctx->device);
-        if (he != hipSuccess) {
-          rcs[k] = set_err(ctx, POF_E_HIP, "hipSetDevice: %s", hipGetErrorString(he));
-          stop = true;
-          return;
-        }
-        while (!stop && pof_batch_active(b) > 0) {
-          int rc = batch_launch(b);
-          if (rc == POF_OK && !b->launched) break;  // nothing to run
-          if (rc == POF_OK) rc = batch_finish(b, 0, nullptr, nullptr);
-          if (rc != POF_OK) {
-            rcs[k] = rc;
-            stop = true;
-            return;
-          }
-        }
-      });
-    }
-    for (auto& t : th) t.join();
-    for (int rc : rcs)
-      if (rc != POF_OK) return fail(rc);
-    return POF_OK;
-  }
+  // group with short iterations never waits for one with long ones.  (One host
+  // thread per group, so that one group's blocking waits would not delay another's
+  // launches, gained 2-3% on a 2,048-IVP batch but crashed once inside the HIP
+  // runtime from a worker thread under rocprofv3's kernel trace; one host thread
+  // drives every group.)
   for (auto& g : groups) {
     if (pof_batch_active(g.b) == 0) continue;
     int rc = batch_launch(g.b);
