@@ -1053,7 +1053,10 @@ void filter_fn(Ops& o, Arena& ar, int D, const FE& e1, const FE& e2, const FE& o
   const size_t mark = ar.off;
   const long long DD = (long long)D * D;
   V X{ar.take((size_t)cnt * 4 * DD), 4 * DD, 2 * D};
-  V Y{ar.take((size_t)cnt * 2 * DD), 2 * DD, 2 * D};
+  // [M^T | U2] for items 0 .. cnt - 1 and [A1^T Xi22 | Z1] for items cnt .. 2 cnt - 1: both are
+  // factored by one LQ call over 2 cnt items
+  V Y{ar.take((size_t)cnt * 4 * DD), 2 * DD, 2 * D};
+  const V YZ = Y.item(cnt);
   V T2{ar.take((size_t)cnt * DD), DD, D};
   V T3{ar.take((size_t)cnt * DD), DD, D}, T4{ar.take((size_t)cnt * DD), DD, D};
   V v1{ar.take((size_t)cnt * D), D, 1}, v2{ar.take((size_t)cnt * D), D, 1}, v3{ar.take((size_t)cnt * D), D, 1};
@@ -1085,10 +1088,8 @@ void filter_fn(Ops& o, Arena& ar, int D, const FE& e1, const FE& e2, const FE& o
   o.gemm(true, false, D, 1, D, -1.0, T3, v3, 1.0, v2, cnt);
   o.copy(out.b, e2.b, D, 1, cnt);
   o.gemm(false, false, D, 1, D, 1.0, e2.A, v2, 1.0, out.b, cnt);
-  // U = tria([M^T | U2])
+  // [M^T | U2]
   o.copy(Y.at(0, D), e2.U, D, D, cnt);
-  o.lq(Y, D, 2 * D, cnt);
-  o.copy(out.U, Y.at(0, 0), D, D, cnt);
   // _e = Xi11^-T Xi21^T -> T4; eta = A1^T (w2 - _e^T U1^T w2) + eta1, w2 = eta2 - Z2 (Z2^T b1)
   o.copy(T4, Xi21, D, D, cnt, true);
   o.trsm(true, D, D, Xi11, T4, cnt);
@@ -1099,11 +1100,12 @@ void filter_fn(Ops& o, Arena& ar, int D, const FE& e1, const FE& e2, const FE& o
   o.gemm(true, false, D, 1, D, -1.0, T4, v3, 1.0, v2, cnt);
   o.copy(out.eta, e1.eta, D, 1, cnt);
   o.gemm(true, false, D, 1, D, 1.0, e1.A, v2, 1.0, out.eta, cnt);
-  // Z = tria([A1^T Xi22 | Z1])
-  o.gemm(true, false, D, D, D, 1.0, e1.A, Xi22, 0.0, Y.at(0, 0), cnt);
-  o.copy(Y.at(0, D), e1.Z, D, D, cnt);
-  o.lq(Y, D, 2 * D, cnt);
-  o.copy(out.Z, Y.at(0, 0), D, D, cnt);
+  // [A1^T Xi22 | Z1]; then U = tria([M^T | U2]) and Z = tria([A1^T Xi22 | Z1]) in one call
+  o.gemm(true, false, D, D, D, 1.0, e1.A, Xi22, 0.0, YZ.at(0, 0), cnt);
+  o.copy(YZ.at(0, D), e1.Z, D, D, cnt);
+  o.lq(Y, D, 2 * D, 2 * cnt);
+  o.copy(out.U, Y.at(0, 0), D, D, cnt);
+  o.copy(out.Z, YZ.at(0, 0), D, D, cnt);
   ar.off = mark;
 }
 
@@ -1212,7 +1214,7 @@ size_t workspace_doubles(int D, int d, long long Ns, bool f32) {
   w += n1 * (D + DD);                       // smoothed natural order
   // per-call temporaries at the widest level (the filter combine), and the post step
   const size_t h = (size_t)(Ns / 2 + 1);
-  w += h * (4 * DD + 2 * DD + 5 * DD + 3 * D) + 4096;
+  w += h * (4 * DD + 4 * DD + 5 * DD + 3 * D) + 4096;
   w += (size_t)(Ns + 1) * (2 * D * kNB + kNB * kNB) + 4096;  // blocked-LQ panels (V, T)
   w += (size_t)Ns * (5 * DD + (size_t)d * D + 2 * D + 3 * d + 4) + 4096;
   w += 8 * n1 + 64 * 1024;
