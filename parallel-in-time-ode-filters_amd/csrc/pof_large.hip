@@ -15,12 +15,14 @@
 #include "pof_large.hpp"
 
 #include "pof_problems.hpp"
+#include "pof_wave.hpp"
 
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 namespace pof {
@@ -98,6 +100,13 @@ __global__ __launch_bounds__(kLqThreads) void k_lq(double* A, long long stride, 
 
 constexpr int kNB = 16;  // rows per panel of the blocked LQ
 
+// f(integral_constant<int, K>) for K = 0, 1, ... in order: a loop whose index
+// is a compile-time constant in every step
+template <typename F, int... K>
+__device__ __forceinline__ void unroll_steps(F& f, std::integer_sequence<int, K...>) {
+  (f(std::integral_constant<int, K>()), ...);
+}
+
 // One panel of the blocked LQ (LAPACK dgelqf: dgelq2 on the panel, dlarft
 // forward / rowwise): rows i0 .. i0 + np - 1 of item blockIdx.x, columns
 // i0 .. n - 1, factored by one wave with the panel in registers (lane holds
@@ -106,22 +115,26 @@ constexpr int kNB = 16;  // rows per panel of the blocked LQ
 // triangular factor T (np x np) with H_1 ... H_np = I - V^T T V.  Pivot column
 // k < kNB < 64 lives in slot 0 of lane k, so only slot 0 needs the test c > k;
 // columns c >= n - i0 are loaded as zeros and stay zero (every update adds a
-// multiple of the pivot row, zero there too), so they need no test.
+// multiple of the pivot row, zero there too), so they need no test.  Rows
+// np .. kNB - 1 are zeros too: their updates leave them so and they are never
+// written, so the steps need no row test either.
+//
+// Step k sums 16 - k values over the wave at once (the squared tail of pivot
+// row k and the products of rows k + 1 .. kNB - 1 with it) in registers: a
+// transposed wave sum leaves each total in a group of lanes, and scalar reads
+// of one lane broadcast it, so a step has no LDS round trip and no barrier.
 typedef double lq_panel_d4 __attribute__((ext_vector_type(4)));
+// Three waves per SIMD at four column slots: the Gram matrix accumulates in
+// ordinary VGPRs then, and the panel fits in 156 registers.
 template <int NQ>
-__global__ __launch_bounds__(64) void k_lq_panel(double* A, long long sA, int n, int ld, int i0, int np, double* Vb,
-                                                 long long sV, int ldV, double* Tb, long long sT, int ldT) {
-  // Cross-lane sums go through LDS in batches: each lane writes its partial sum
-  // of every value of a step (red[value][lane]), then lanes 4v .. 4v + 3 add up
-  // value v (16 partials each, then two butterfly steps) and publish it in tot.
-  __shared__ double red[kNB][65];
-  __shared__ double tot[kNB];
-  __shared__ double colk[kNB];  // column k of the panel rows (held by lane k, slot 0)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_lq_panel(
+    double* A, long long sA, int n, int ld, int i0, int np, double* Vb, long long sV, int ldV, double* Tb,
+    long long sT, int ldT) {
   __shared__ double G[kNB][kNB + 1];
   __shared__ double taus[kNB];
   const int lane = threadIdx.x;
-  const int rv = lane >> 2, rs = lane & 3;  // value and quarter this lane reduces
   double* a = A + (long long)blockIdx.x * sA + (size_t)i0 * ld + i0;
+  double* Vo = Vb + (long long)blockIdx.x * sV;
   const int nc = n - i0;
   double P[kNB][NQ];
 #pragma unroll
@@ -132,97 +145,78 @@ __global__ __launch_bounds__(64) void k_lq_panel(double* A, long long sA, int n,
       P[r][q] = (r < np && c < nc) ? a[(size_t)r * ld + c] : 0.0;
     }
   }
-  auto reduce = [&]() {
-    __syncthreads();
-    double h[16];
+  // Householder step k (a compile-time constant); NV, a power of two >= kNB - k,
+  // values per wave sum
+  auto step = [&](auto kc) __attribute__((always_inline)) {
+    constexpr int k = decltype(kc)::value;
+    constexpr int NV = k < 8 ? 16 : (k < 12 ? 8 : (k < 14 ? 4 : (k < 15 ? 2 : 1)));
+    constexpr int G1 = 64 / NV;  // lanes per total after the wave sum
+    if (k >= np) return;
+    // value 0: the squared tail of pivot row k; value j >= 1: row k + j . tail of row k.
+    // Slot 0 holds a tail column only in lanes beyond k: masked once, here.
+    const double pk0 = (lane > k) ? P[k][0] : 0.0;
+    double x[NV];
+    {
+      double sq = pk0 * pk0;
 #pragma unroll
-    for (int t = 0; t < 16; ++t) h[t] = red[rv][rs * 16 + t];
+      for (int q = 1; q < NQ; ++q) sq = fma(P[k][q], P[k][q], sq);
+      x[0] = sq;
+    }
 #pragma unroll
-    for (int w = 8; w > 0; w >>= 1)  // pairwise: four dependent adds instead of sixteen
-#pragma unroll
-      for (int t = 0; t < w; ++t) h[t] += h[t + w];
-    double s = h[0];
-    s += __shfl_xor(s, 1, 64);
-    s += __shfl_xor(s, 2, 64);
-    if (rs == 0) tot[rv] = s;
-    __syncthreads();
-  };
-  double betas[kNB];
-#pragma unroll
-  for (int k = 0; k < kNB; ++k) {
-    betas[k] = 0.0;
-    if (k < np) {
-      __syncthreads();  // the previous step has read red, tot and colk
-      // value 0: the squared tail of pivot row k; value r > k: row r . tail of row k.
-      // Slot 0 holds a tail column only in lanes beyond k: masked once, here.
-      const double pk0 = (lane > k) ? P[k][0] : 0.0;
-      {
-        double sq = pk0 * pk0;
-#pragma unroll
-        for (int q = 1; q < NQ; ++q) sq = fma(P[k][q], P[k][q], sq);
-        red[0][lane] = sq;
-      }
-#pragma unroll
-      for (int r = k + 1; r < kNB; ++r) {
-        double w = P[r][0] * pk0;
+    for (int j = 1; j < NV; ++j) {
+      const int r = k + j;
+      double w = 0.0;
+      if (r < kNB) {
+        w = P[r][0] * pk0;
 #pragma unroll
         for (int q = 1; q < NQ; ++q) w = fma(P[r][q], P[k][q], w);
-        red[r][lane] = w;
       }
-      if (lane == k) {
+      x[j] = w;
+    }
+    wave::sum_scatter<NV>(x, lane);
+    const double sig = wave::lane_value(x[0], 0);
+    const double alpha = wave::lane_value(P[k][0], k);
+    double tau = 0.0, beta = alpha, scal = 0.0;
+    if (sig != 0.0) {
+      const double nrm = sqrt(fma(alpha, alpha, sig));
+      beta = -copysign(nrm, alpha);
+      tau = (beta - alpha) / beta;
+      scal = 1.0 / (alpha - beta);
+    }
+    // the reflector in slot 0: 1 in the pivot lane, the scaled tail beyond it, 0 before
+    const double v0 = (lane > k) ? P[k][0] * scal : ((lane == k) ? 1.0 : 0.0);
+    P[k][0] = (lane > k) ? v0 : P[k][0];
 #pragma unroll
-        for (int r = 0; r < kNB; ++r) colk[r] = P[r][0];
-      }
-      reduce();
-      const double sig = tot[0];
-      const double alpha = colk[k];
-      double tau = 0.0, beta = alpha, scal = 0.0;
-      if (sig != 0.0) {
-        const double nrm = sqrt(fma(alpha, alpha, sig));
-        beta = -copysign(nrm, alpha);
-        tau = (beta - alpha) / beta;
-        scal = 1.0 / (alpha - beta);
-      }
-      // the reflector in slot 0: 1 in the pivot lane, the scaled tail beyond it, 0 before
-      const double v0 = (lane > k) ? P[k][0] * scal : ((lane == k) ? 1.0 : 0.0);
-      P[k][0] = (lane > k) ? v0 : P[k][0];
+    for (int q = 1; q < NQ; ++q) P[k][q] *= scal;
+    // row k is final: L row k back to A, V row k out (slots q >= 1 hold columns
+    // 64 q + lane > k: zero in L, tail in V)
+    if (lane < nc) {
+      a[(size_t)k * ld + lane] = (lane < k) ? P[k][0] : ((lane == k) ? beta : 0.0);
+      Vo[(size_t)k * ldV + lane] = (lane < k) ? 0.0 : ((lane == k) ? 1.0 : P[k][0]);
+    }
 #pragma unroll
-      for (int q = 1; q < NQ; ++q) P[k][q] *= scal;
-      betas[k] = beta;
-      if (lane == 0) taus[k] = tau;
-      if (tau != 0.0) {
-#pragma unroll
-        for (int r = k + 1; r < kNB; ++r) {
-          if (r < np) {
-            // row r . v with v = [1, scal * tail]
-            const double tw = tau * fma(scal, tot[r], colk[r]);
-            P[r][0] = fma(-tw, v0, P[r][0]);
-#pragma unroll
-            for (int q = 1; q < NQ; ++q) P[r][q] = fma(-tw, P[k][q], P[r][q]);
-          }
-        }
+    for (int q = 1; q < NQ; ++q) {
+      const int c = lane + 64 * q;
+      if (c < nc) {
+        a[(size_t)k * ld + c] = 0.0;
+        Vo[(size_t)k * ldV + c] = P[k][q];
       }
     }
-  }
-  // L rows back to A; V rows out (slots q >= 1 hold columns 64 q + lane > k: zero in L, tail in V)
-  double* Vo = Vb + (long long)blockIdx.x * sV;
+    if (lane == 0) taus[k] = tau;
+    if (tau != 0.0) {
 #pragma unroll
-  for (int k = 0; k < kNB; ++k) {
-    if (k < np) {
-      if (lane < nc) {
-        a[(size_t)k * ld + lane] = (lane < k) ? P[k][0] : ((lane == k) ? betas[k] : 0.0);
-        Vo[(size_t)k * ldV + lane] = (lane < k) ? 0.0 : ((lane == k) ? 1.0 : P[k][0]);
-      }
+      for (int r = k + 1; r < kNB; ++r) {
+        // row r . v with v = [1, scal * tail]; the pivot column of row r sits in lane k
+        const double tr = wave::lane_value(x[0], (r - k) * G1);
+        const double ck = wave::lane_value(P[r][0], k);
+        const double tw = tau * fma(scal, tr, ck);
+        P[r][0] = fma(-tw, v0, P[r][0]);
 #pragma unroll
-      for (int q = 1; q < NQ; ++q) {
-        const int c = lane + 64 * q;
-        if (c < nc) {
-          a[(size_t)k * ld + c] = 0.0;
-          Vo[(size_t)k * ldV + c] = P[k][q];
-        }
+        for (int q = 1; q < NQ; ++q) P[r][q] = fma(-tw, P[k][q], P[r][q]);
       }
     }
-  }
+  };
+  unroll_steps(step, std::make_integer_sequence<int, kNB>());
   // T (dlarft, forward / rowwise): G = V V^T on the matrix cores, with V read back from the
   // rows just stored (L2; the panel's registers are free by then), 64 columns per batch of
   // loads and four accumulation chains (a Gram matrix sums over every column, so the order
@@ -901,7 +895,8 @@ struct Ops {
     for (int i0 = 0; i0 < kmax && !err; i0 += kNB) {
       const int np = (kmax - i0) < kNB ? (kmax - i0) : kNB;
       const int nc = n - i0;
-      const int nq = (n + 63) / 64;
+      // column slots for this panel's own width: later panels are narrower
+      const int nq = (nc + 63) / 64;
       if (nq <= 1) {
         hipLaunchKernelGGL(k_lq_panel<1>, dim3(cnt), dim3(64), 0, st, A.p, A.s, n, A.ld, i0, np, Vb.p, Vb.s, Vb.ld,
                            Tb.p, Tb.s, Tb.ld);
