@@ -379,66 +379,116 @@ __global__ __launch_bounds__(64 * kLqUpdWaves, 2) void k_lq_update(double* A, lo
   }
 }
 
-constexpr int kTB = 16;        // rows per block of the triangular solve
-constexpr int kTrsmMaxN = 256;  // order of the triangular matrix at most
+constexpr int kTrsmMaxN = 128;  // order of the triangular matrix at most (D <= 128)
+constexpr int kTrsmWaves = 8;
 
-// op(L) X = B in place for item blockIdx.y: L lower n x n, B n x m, op = L
-// (forward substitution) or L^T (backward).  One thread per right-hand side
-// column; blocks of kTB rows: the block's rows (or columns, for L^T) of L are
-// staged in LDS, the earlier blocks' solutions are subtracted, then the
-// diagonal block is solved in registers.
-__global__ __launch_bounds__(256) void k_trsm(const double* L, long long sL, int ldL, double* B, long long sB,
-                                              int ldB, int n, int m, int trans) {
-  __shared__ double Ls[kTB][kTrsmMaxN + 1];
-  const double* l = L + (long long)blockIdx.y * sL;
-  double* b = B + (long long)blockIdx.y * sB;
+// op(L) X = B in place for item blockIdx.x: L lower n x n (n <= 128), B n x m
+// (m <= 128), op = L (forward substitution) or L^T (TRANS, backward), on the FP64
+// matrix cores (v_mfma_f64_16x16x4f64) in blocks of 16 rows:
+//   X_K = inv(L_KK) (B_K - sum_J op(L)_KJ X_J)
+// over the earlier blocks J (later ones for L^T).  The lower block triangle of L
+// (36 blocks of 16 x 16, zero beyond n) is staged in LDS in one coalesced pass;
+// the diagonal blocks are then inverted in place, one thread per column of an
+// inverse (forward substitution; rows beyond n act as rows of the identity).
+// Wave w takes columns 16 w .. 16 w + 15 of B: its blocks are loaded once and
+// solved in registers, in the accumulator layout (lane j + 16 g value
+// v holds row 4 v + g, column j of a 16 x 16 block), which is also the layout
+// of the B operand of the products with later blocks.
+typedef double trsm_d4 __attribute__((ext_vector_type(4)));
+constexpr int kTrsmNB = kTrsmMaxN / 16;
+constexpr int kTrsmBS = 16 * 17;  // one padded 16 x 16 block
+__host__ __device__ constexpr int trsm_tri(int K, int J) { return K * (K + 1) / 2 + J; }
+// Four waves per SIMD (128 VGPRs): two blocks per CU, 2 x 78 KiB of LDS.
+template <bool TRANS>
+__global__ __launch_bounds__(64 * kTrsmWaves) __attribute__((amdgpu_waves_per_eu(4))) void k_trsm(
+    const double* L, long long sL, int ldL, double* B, long long sB, int ldB, int n, int m) {
+  constexpr int NB = kTrsmNB;
+  __shared__ double Lt[trsm_tri(NB, 0) * kTrsmBS];
+  const double* l = L + (long long)blockIdx.x * sL;
+  double* b = B + (long long)blockIdx.x * sB;
   const int tid = threadIdx.x;
-  const int c = blockIdx.x * 256 + tid;
-  const int nb = (n + kTB - 1) / kTB;
-  for (int kb = 0; kb < nb; ++kb) {
-    const int K = trans ? (nb - 1 - kb) : kb;
-    const int r0 = K * kTB;
-    const int rn = (n - r0) < kTB ? (n - r0) : kTB;
-    __syncthreads();
-    for (int e = tid; e < rn * n; e += 256) {
-      const int r = e / n, t = e % n;
-      Ls[r][t] = trans ? l[(size_t)t * ldL + r0 + r] : l[(size_t)(r0 + r) * ldL + t];
+  const int wave = tid >> 6, lane = tid & 63;
+  const int li = lane & 15, lg = lane >> 4;
+  const int nb = (n + 15) >> 4;
+  const int c = 16 * wave + li;
+  const bool cok = c < m;
+  for (int e = tid; e < trsm_tri(nb, 0) * 256; e += 64 * kTrsmWaves) {
+    const int bi = e >> 8, i = (e >> 4) & 15, j = e & 15;
+    int K = 0;
+    while (trsm_tri(K + 1, 0) <= bi) ++K;
+    const int J = bi - trsm_tri(K, 0);
+    const int r = 16 * K + i, cc = 16 * J + j;
+    const bool ok = r < n && cc < n;
+    Lt[bi * kTrsmBS + i * 17 + j] = ok ? l[(size_t)(ok ? r : 0) * ldL + (ok ? cc : 0)] : 0.0;
+  }
+  __syncthreads();
+  {
+    double y[16];  // column j of inv(L_KK)
+    const int K = tid >> 4, j = tid & 15;
+    double* Lk = Lt + trsm_tri(K < nb ? K : 0, K < nb ? K : 0) * kTrsmBS;
+    if (K < nb) {
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        double acc = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+        for (int t = 0; t < i; ++t) acc = fma(-Lk[i * 17 + t], y[t], acc);
+        y[i] = (i < j) ? 0.0 : acc / ((16 * K + i < n) ? Lk[i * 17 + i] : 1.0);
+      }
     }
     __syncthreads();
-    if (c < m) {
-      double acc[kTB];
+    if (K < nb) {
 #pragma unroll
-      for (int r = 0; r < kTB; ++r) acc[r] = (r < rn) ? b[(size_t)(r0 + r) * ldB + c] : 0.0;
-      if (!trans) {
-        for (int t = 0; t < r0; ++t) {
-          const double x = b[(size_t)t * ldB + c];
+      for (int i = 0; i < 16; ++i) Lk[i * 17 + j] = y[i];
+    }
+  }
+  __syncthreads();
+  if (16 * wave >= m) return;  // uniform over the wave; no barrier follows
+  // this wave's blocks of B (two blocks per CU: the other block's work covers the latency)
+  trsm_d4 X[NB];
 #pragma unroll
-          for (int r = 0; r < kTB; ++r) acc[r] = fma(-Ls[r][t], x, acc[r]);
-        }
+  for (int K = 0; K < NB; ++K) {
 #pragma unroll
-        for (int r = 0; r < kTB; ++r) {
+    for (int v = 0; v < 4; ++v) {
+      const int r = 16 * K + 4 * v + lg;
+      const bool ok = K < nb && r < n && cok;
+      X[K][v] = ok ? b[(size_t)(ok ? r : 0) * ldB + (ok ? c : 0)] : 0.0;
+    }
+  }
 #pragma unroll
-          for (int t = 0; t < kTB; ++t)
-            if (t < r) acc[r] = fma(-Ls[r][r0 + t], acc[t], acc[r]);
-          acc[r] = (r < rn) ? acc[r] / Ls[r][r0 + r] : 0.0;
-        }
-      } else {
-        for (int t = r0 + rn; t < n; ++t) {
-          const double x = b[(size_t)t * ldB + c];
+  for (int q = 0; q < NB; ++q) {
+    const int K = TRANS ? NB - 1 - q : q;
+    if (K >= nb) continue;  // uniform
+    // two accumulation chains, so that consecutive products do not wait on each other
+    trsm_d4 acc0 = X[K], acc1 = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-          for (int r = 0; r < kTB; ++r) acc[r] = fma(-Ls[r][t], x, acc[r]);
-        }
+    for (int J = 0; J < NB; ++J) {
+      if (TRANS ? (J > K && J < nb) : (J < K)) {
 #pragma unroll
-        for (int r = kTB - 1; r >= 0; --r) {
-#pragma unroll
-          for (int t = 0; t < kTB; ++t)
-            if (t > r && t < rn) acc[r] = fma(-Ls[r][r0 + t], acc[t], acc[r]);
-          acc[r] = (r < rn) ? acc[r] / Ls[r][r0 + r] : 0.0;
+        for (int s = 0; s < 4; ++s) {
+          // A = op(L)_KJ: row li, column 4 s + lg of the block
+          const double a = TRANS ? Lt[trsm_tri(J, K) * kTrsmBS + (4 * s + lg) * 17 + li]
+                                 : Lt[trsm_tri(K, J) * kTrsmBS + li * 17 + 4 * s + lg];
+          if (J & 1)
+            acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(-a, X[J][s], acc1, 0, 0, 0);
+          else
+            acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(-a, X[J][s], acc0, 0, 0, 0);
         }
       }
+    }
+    const trsm_d4 rhs = acc0 + acc1;
+    trsm_d4 x = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int r = 0; r < kTB; ++r)
-        if (r < rn) b[(size_t)(r0 + r) * ldB + c] = acc[r];
+    for (int s = 0; s < 4; ++s) {
+      // inv(L_KK) (or its transpose): row li, column 4 s + lg
+      const double a = TRANS ? Lt[trsm_tri(K, K) * kTrsmBS + (4 * s + lg) * 17 + li]
+                             : Lt[trsm_tri(K, K) * kTrsmBS + li * 17 + 4 * s + lg];
+      x = __builtin_amdgcn_mfma_f64_16x16x4f64(a, rhs[s], x, 0, 0, 0);
+    }
+    X[K] = x;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int r = 16 * K + 4 * v + lg;
+      if (r < n && cok) b[(size_t)r * ldB + c] = x[v];
     }
   }
 }
@@ -948,12 +998,15 @@ struct Ops {
   // op(L) X = X in place (L lower n x n, X n x m)
   void trsm(bool trans, int n, int m, V L, V X, int cnt) {
     if (err || cnt <= 0) return;
-    if (n > kTrsmMaxN) {
-      fail("trsm: order too large", -1);
+    if (n > kTrsmMaxN || m > 16 * kTrsmWaves) {
+      fail("trsm: order or right-hand sides beyond 128", -1);
       return;
     }
-    hipLaunchKernelGGL(k_trsm, dim3((m + 255) / 256, cnt), dim3(256), 0, st, L.p, L.s, L.ld, X.p, X.s, X.ld, n, m,
-                       trans ? 1 : 0);
+    const dim3 grid(cnt), block(64 * kTrsmWaves);
+    if (trans)
+      hipLaunchKernelGGL(k_trsm<true>, grid, block, 0, st, L.p, L.s, L.ld, X.p, X.s, X.ld, n, m);
+    else
+      hipLaunchKernelGGL(k_trsm<false>, grid, block, 0, st, L.p, L.s, L.ld, X.p, X.s, X.ld, n, m);
     check("k_trsm");
   }
   void copy(V dst, V src, int rows, int cols, int cnt, bool trans = false, bool rev = false, double alpha = 1.0) {
@@ -1189,7 +1242,8 @@ void engine_destroy(Engine* e) {
 }
 
 bool supports(int kind, int D, int d) {
-  // the LQ panels hold kNB rows of up to 2 D columns in LDS: D <= 128 keeps them within 32 KiB
+  // the LQ panel kernel holds up to 256 columns (2 D) in registers and the triangular solves
+  // take orders up to 128: D <= 128
   return kind == kLorenz96 && d >= 4 && D % d == 0 && D / d >= 2 && D <= 128;
 }
 
