@@ -138,16 +138,17 @@ k_filter_leaf(GroupArgs g, LaunchMap lm) {
   }
 }
 
-// Waves per SIMD the up-sweep combine is compiled for: at D = 12 the register
-// budget of 4 waves (128 VGPRs) costs a 12-byte spill and lifts the occupancy
-// from 3 waves; elsewhere the natural allocation is kept.
+// Waves per SIMD the up- and down-sweep combines are compiled for: 5 at D = 10,
+// the natural allocation elsewhere.  A 4-wave budget (128 VGPRs) for the D = 12
+// up-sweep or the D = 15 down-sweep spills (24 and 44 bytes per lane) and is no
+// faster than their natural 3 waves.
 template <int D>
 constexpr int up_min_waves() {
-  return D == 12 ? 4 : (D == 10) ? 5 : 1;
+  return (D == 10) ? 5 : 1;
 }
 template <int D>
 constexpr int down_min_waves() {
-  return D == 15 ? 4 : (D == 10) ? 5 : 1;
+  return (D == 10) ? 5 : 1;
 }
 
 // Waves per SIMD of a scan kernel of kind K: the tuned bound of the
