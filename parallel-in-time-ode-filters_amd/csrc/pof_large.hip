@@ -865,6 +865,7 @@ struct Ops {
   std::string msg;
   bool f32 = false;          // gemm() forms matrix-matrix products as FP32 GEMMs (set inside the combines)
   bool f32_combines = false;  // the solve asked for FP32 combine products
+  int zw2 = 1 << 20;  // band of the filter elements' Z blocks (d - 1: leaf Z = [Zr | 0]; none by default)
 
   void fail(const char* what, int code) {
     if (!err) {
@@ -920,9 +921,14 @@ struct Ops {
                          alpha, A.p, A.s, A.ld, B.p, B.s, B.ld, beta, C.p, C.s, C.ld);
     }
   }
-  // tria in place: the blocked LQ (panels of kNB rows in LDS, trailing rows updated
-  // by three GEMMs per panel), or k_lq for matrices of at most one panel
-  void lq(V A, int m, int n, int cnt) {
+  // tria in place: the blocked LQ (panels of kNB rows, trailing rows updated in one
+  // pass per panel), or k_lq for matrices of at most one panel.  b1 > 0 declares a
+  // banded second block: in columns b1 .. n - 1, row r is zero beyond column
+  // b1 + max(r, w2).  Pivot i's reflector is then zero beyond column b1 + max(i, w2)
+  // (the reflectors before it only reach that far), so a panel of first-block pivots
+  // leaves the columns beyond its last pivot's band as they are and skips them: the
+  // same factor, with the panel and its trailing update over fewer columns.
+  void lq(V A, int m, int n, int cnt, int b1 = 0, int w2 = 0) {
     if (err || cnt <= 0) return;
     if (n > 256) {
       fail("lq: more than 256 columns", -1);
@@ -944,20 +950,25 @@ struct Ops {
     }
     for (int i0 = 0; i0 < kmax && !err; i0 += kNB) {
       const int np = (kmax - i0) < kNB ? (kmax - i0) : kNB;
-      const int nc = n - i0;
+      int ce = n;  // end (exclusive) of the columns this panel's reflectors reach
+      if (b1 > 0 && i0 + np <= b1) {
+        const int last = i0 + np - 1;
+        ce = std::min(n, b1 + std::max(last, w2) + 1);
+      }
+      const int nc = ce - i0;
       // column slots for this panel's own width: later panels are narrower
       const int nq = (nc + 63) / 64;
       if (nq <= 1) {
-        hipLaunchKernelGGL(k_lq_panel<1>, dim3(cnt), dim3(64), 0, st, A.p, A.s, n, A.ld, i0, np, Vb.p, Vb.s, Vb.ld,
+        hipLaunchKernelGGL(k_lq_panel<1>, dim3(cnt), dim3(64), 0, st, A.p, A.s, ce, A.ld, i0, np, Vb.p, Vb.s, Vb.ld,
                            Tb.p, Tb.s, Tb.ld);
       } else if (nq == 2) {
-        hipLaunchKernelGGL(k_lq_panel<2>, dim3(cnt), dim3(64), 0, st, A.p, A.s, n, A.ld, i0, np, Vb.p, Vb.s, Vb.ld,
+        hipLaunchKernelGGL(k_lq_panel<2>, dim3(cnt), dim3(64), 0, st, A.p, A.s, ce, A.ld, i0, np, Vb.p, Vb.s, Vb.ld,
                            Tb.p, Tb.s, Tb.ld);
       } else if (nq == 3) {
-        hipLaunchKernelGGL(k_lq_panel<3>, dim3(cnt), dim3(64), 0, st, A.p, A.s, n, A.ld, i0, np, Vb.p, Vb.s, Vb.ld,
+        hipLaunchKernelGGL(k_lq_panel<3>, dim3(cnt), dim3(64), 0, st, A.p, A.s, ce, A.ld, i0, np, Vb.p, Vb.s, Vb.ld,
                            Tb.p, Tb.s, Tb.ld);
       } else {
-        hipLaunchKernelGGL(k_lq_panel<4>, dim3(cnt), dim3(64), 0, st, A.p, A.s, n, A.ld, i0, np, Vb.p, Vb.s, Vb.ld,
+        hipLaunchKernelGGL(k_lq_panel<4>, dim3(cnt), dim3(64), 0, st, A.p, A.s, ce, A.ld, i0, np, Vb.p, Vb.s, Vb.ld,
                            Tb.p, Tb.s, Tb.ld);
       }
       check("k_lq_panel");
@@ -1117,7 +1128,8 @@ void filter_fn(Ops& o, Arena& ar, int D, const FE& e1, const FE& e2, const FE& o
   o.fill(X.at(0, D), D, D, cnt, 1.0);
   o.copy(X.at(D, 0), e2.Z, D, D, cnt);
   o.fill(X.at(D, D), D, D, cnt, 0.0);
-  o.lq(X, 2 * D, 2 * D, cnt);
+  // second block [I; 0]: row r is zero beyond column D + r
+  o.lq(X, 2 * D, 2 * D, cnt, D, 0);
   const V Xi11 = X.at(0, 0), Xi21 = X.at(D, 0), Xi22 = X.at(D, D);
   // m_ = Xi11^-1 U1^T -> T3; M^T = (m_ A2^T)^T = A2 m_^T, formed in place as the left
   // block of [M^T | U2] (Y)
@@ -1151,7 +1163,8 @@ void filter_fn(Ops& o, Arena& ar, int D, const FE& e1, const FE& e2, const FE& o
   // [A1^T Xi22 | Z1]; then U = tria([M^T | U2]) and Z = tria([A1^T Xi22 | Z1]) in one call
   o.gemm(true, false, D, D, D, 1.0, e1.A, Xi22, 0.0, YZ.at(0, 0), cnt);
   o.copy(YZ.at(0, D), e1.Z, D, D, cnt);
-  o.lq(Y, D, 2 * D, 2 * cnt);
+  // second blocks U2 (lower triangular) and Z1 (lower triangular, or [Zr | 0] at the leaves)
+  o.lq(Y, D, 2 * D, 2 * cnt, D, o.zw2);
   o.copy(out.U, Y.at(0, 0), D, D, cnt);
   o.copy(out.Z, YZ.at(0, 0), D, D, cnt);
   ar.off = mark;
@@ -1174,7 +1187,7 @@ void smoother_fn(Ops& o, Arena& ar, int D, const SE& e1, const SE& e2, const SE&
     }
     o.gemm(false, false, D, D, D, 1.0, e2.E, e1.Dm, 0.0, Y.at(0, 0), cnt);
     o.copy(Y.at(0, D), e2.Dm, D, D, cnt);
-    o.lq(Y, D, 2 * D, cnt);
+    o.lq(Y, D, 2 * D, cnt, D, 0);  // second block D2: lower triangular
     o.copy(out.Dm, Y.at(0, 0), D, D, cnt);
   }
   ar.off = mark;
@@ -1311,6 +1324,7 @@ int solve(Engine* e, const Problem& p, const pof_opts& opts, pof_out* out, const
   Arena ar{e->buf, e->cap, 0};
   Ops o{st, &ar};
   o.f32_combines = f32;
+  o.zw2 = d - 1;  // leaf Z = [Zr | 0]: nonzero in its first d columns only
   // constants: F, QL, N1z, P, x0 (one block)
   double* cst = ar.take(5 * DD + 2 * D);
   double* mA = ar.take((size_t)(Ns + 1) * D);
@@ -1357,6 +1371,11 @@ int solve(Engine* e, const Problem& p, const pof_opts& opts, pof_out* out, const
       qli[(size_t)i * D + j] = v / p.QL[(size_t)i * D + i];
     }
   up(dQLi, qli.data(), DD);
+  // a lower-triangular QL makes the second blocks [QL] and [QL; 0] banded (the LQs skip past the band)
+  bool ql_lower = true;
+  for (int i = 0; i < D; ++i)
+    for (int j = i + 1; j < D; ++j) ql_lower = ql_lower && p.QL[(size_t)i * D + j] == 0.0;
+  const int ql_band = ql_lower ? D : 0;
   up(dm0, p.x0m, D);
   up(mA, p.m0, (size_t)(Ns + 1) * D);
   if (he != hipSuccess) {
@@ -1409,7 +1428,7 @@ int solve(Engine* e, const Problem& p, const pof_opts& opts, pof_out* out, const
       o.gemm(false, false, D, 1, D, 1.0, Fv, V{dm0, 0, 1}, 0.0, m1, 1);
       o.gemm(false, false, D, D, D, 1.0, Fv, V{dL0, 0, D}, 0.0, N10.at(0, 0), 1);
       o.copy(N10.at(0, D), QLv, D, D, 1);
-      o.lq(N10, D, 2 * D, 1);
+      o.lq(N10, D, 2 * D, 1, ql_band, 0);
       for (int part = 0; part < 2; ++part) {
         const int k0 = part == 0 ? 0 : 1;
         const int cnt = part == 0 ? 1 : n - 1;
@@ -1483,7 +1502,7 @@ int solve(Engine* e, const Problem& p, const pof_opts& opts, pof_out* out, const
       o.copy(Phi.at(0, D), QLv, D, D, n);
       o.copy(Phi.at(D, 0), fLv, D, D, n);
       o.fill(Phi.at(D, D), D, D, n, 0.0);
-      o.lq(Phi, 2 * D, 2 * D, n);
+      o.lq(Phi, 2 * D, 2 * D, n, ql_band, 0);  // second block [QL; 0]
       // E = Phi21 Phi11^-1, D = Phi22, g = m - E F m
       // E^T = Phi11^-T Phi21^T
       o.copy(Phin, Phi.at(D, 0), D, D, n, true);
