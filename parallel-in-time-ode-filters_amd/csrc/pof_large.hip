@@ -986,10 +986,18 @@ struct Ops {
     const dim3 block(64 * kLqUpdWaves);
     // 16-byte loads and stores when every row of the block starts on a 16-byte boundary
     const bool vec = ((uintptr_t)A.p % 16 == 0) && (A.s % 2 == 0) && (A.ld % 2 == 0) && (i0 % 2 == 0);
+    // tiles of 16 columns: the fewest that cover nc (registers and LDS scale with them; the banded
+    // panels of the D = 128 combines have 144 or 160 columns)
     if (nc <= 64) {
       launch_update<4>(vec, grid, block, A, i0, np, mr, nc, Vb, Tb, cnt, sbk);
     } else if (nc <= 128) {
       launch_update<8>(vec, grid, block, A, i0, np, mr, nc, Vb, Tb, cnt, sbk);
+    } else if (nc <= 144) {
+      launch_update<9>(vec, grid, block, A, i0, np, mr, nc, Vb, Tb, cnt, sbk);
+    } else if (nc <= 160) {
+      launch_update<10>(vec, grid, block, A, i0, np, mr, nc, Vb, Tb, cnt, sbk);
+    } else if (nc <= 192) {
+      launch_update<12>(vec, grid, block, A, i0, np, mr, nc, Vb, Tb, cnt, sbk);
     } else {
       launch_update<16>(vec, grid, block, A, i0, np, mr, nc, Vb, Tb, cnt, sbk);
     }
