@@ -390,8 +390,8 @@ constexpr int kTrsmWaves = 8;
 // (36 blocks of 16 x 16, zero beyond n) is staged in LDS in one coalesced pass;
 // the diagonal blocks are then inverted in place, one thread per column of an
 // inverse (forward substitution; rows beyond n act as rows of the identity).
-// Wave w takes columns 16 w .. 16 w + 15 of B: its blocks are loaded once and
-// solved in registers, in the accumulator layout (lane j + 16 g value
+// Wave w takes columns 16 w .. 16 w + 15 of B: its blocks are loaded once (from
+// B itself, or from a separate source R: B = R, or R^T when RT) and solved in registers, in the accumulator layout (lane j + 16 g value
 // v holds row 4 v + g, column j of a 16 x 16 block), which is also the layout
 // of the B operand of the products with later blocks.
 typedef double trsm_d4 __attribute__((ext_vector_type(4)));
@@ -401,11 +401,13 @@ __host__ __device__ constexpr int trsm_tri(int K, int J) { return K * (K + 1) / 
 // Four waves per SIMD (128 VGPRs): two blocks per CU, 2 x 78 KiB of LDS.
 template <bool TRANS>
 __global__ __launch_bounds__(64 * kTrsmWaves) __attribute__((amdgpu_waves_per_eu(4))) void k_trsm(
-    const double* L, long long sL, int ldL, double* B, long long sB, int ldB, int n, int m) {
+    const double* L, long long sL, int ldL, double* B, long long sB, int ldB, int n, int m, const double* R,
+    long long sR, int ldR, int RT) {
   constexpr int NB = kTrsmNB;
   __shared__ double Lt[trsm_tri(NB, 0) * kTrsmBS];
   const double* l = L + (long long)blockIdx.x * sL;
   double* b = B + (long long)blockIdx.x * sB;
+  const double* rb = R + (long long)blockIdx.x * sR;
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int li = lane & 15, lg = lane >> 4;
@@ -451,7 +453,8 @@ __global__ __launch_bounds__(64 * kTrsmWaves) __attribute__((amdgpu_waves_per_eu
     for (int v = 0; v < 4; ++v) {
       const int r = 16 * K + 4 * v + lg;
       const bool ok = K < nb && r < n && cok;
-      X[K][v] = ok ? b[(size_t)(ok ? r : 0) * ldB + (ok ? c : 0)] : 0.0;
+      const int rc = ok ? r : 0, cc = ok ? c : 0;
+      X[K][v] = ok ? (RT ? rb[(size_t)cc * ldR + rc] : rb[(size_t)rc * ldR + cc]) : 0.0;
     }
   }
 #pragma unroll
@@ -1014,18 +1017,25 @@ struct Ops {
                          Vb.ld, Tb.p, Tb.s, Tb.ld, cnt, sbk);
     }
   }
-  // op(L) X = X in place (L lower n x n, X n x m)
-  void trsm(bool trans, int n, int m, V L, V X, int cnt) {
+  // op(L) X = R (L lower n x n, X n x m): R is X itself (in place), or the separate R when given
+  // (its transpose when rt)
+  void trsm(bool trans, int n, int m, V L, V X, int cnt, V R = V{}, bool rt = false) {
     if (err || cnt <= 0) return;
     if (n > kTrsmMaxN || m > 16 * kTrsmWaves) {
       fail("trsm: order or right-hand sides beyond 128", -1);
       return;
     }
+    if (!R.p) {
+      R = X;
+      rt = false;
+    }
     const dim3 grid(cnt), block(64 * kTrsmWaves);
     if (trans)
-      hipLaunchKernelGGL(k_trsm<true>, grid, block, 0, st, L.p, L.s, L.ld, X.p, X.s, X.ld, n, m);
+      hipLaunchKernelGGL(k_trsm<true>, grid, block, 0, st, L.p, L.s, L.ld, X.p, X.s, X.ld, n, m, R.p, R.s, R.ld,
+                         rt ? 1 : 0);
     else
-      hipLaunchKernelGGL(k_trsm<false>, grid, block, 0, st, L.p, L.s, L.ld, X.p, X.s, X.ld, n, m);
+      hipLaunchKernelGGL(k_trsm<false>, grid, block, 0, st, L.p, L.s, L.ld, X.p, X.s, X.ld, n, m, R.p, R.s, R.ld,
+                         rt ? 1 : 0);
     check("k_trsm");
   }
   void copy(V dst, V src, int rows, int cols, int cnt, bool trans = false, bool rev = false, double alpha = 1.0) {
@@ -1141,8 +1151,7 @@ void filter_fn(Ops& o, Arena& ar, int D, const FE& e1, const FE& e2, const FE& o
   const V Xi11 = X.at(0, 0), Xi21 = X.at(D, 0), Xi22 = X.at(D, D);
   // m_ = Xi11^-1 U1^T -> T3; M^T = (m_ A2^T)^T = A2 m_^T, formed in place as the left
   // block of [M^T | U2] (Y)
-  o.copy(T3, e1.U, D, D, cnt, true);
-  o.trsm(false, D, D, Xi11, T3, cnt);
+  o.trsm(false, D, D, Xi11, T3, cnt, e1.U, true);
   o.gemm(false, true, D, D, D, 1.0, e2.A, T3, 0.0, Y.at(0, 0), cnt);
   // A = A2 A1 - M^T (Xi21^T A1)
   o.gemm(true, false, D, D, D, 1.0, Xi21, e1.A, 0.0, T2, cnt);
@@ -1159,8 +1168,7 @@ void filter_fn(Ops& o, Arena& ar, int D, const FE& e1, const FE& e2, const FE& o
   // [M^T | U2]
   o.copy(Y.at(0, D), e2.U, D, D, cnt);
   // _e = Xi11^-T Xi21^T -> T4; eta = A1^T (w2 - _e^T U1^T w2) + eta1, w2 = eta2 - Z2 (Z2^T b1)
-  o.copy(T4, Xi21, D, D, cnt, true);
-  o.trsm(true, D, D, Xi11, T4, cnt);
+  o.trsm(true, D, D, Xi11, T4, cnt, Xi21, true);
   o.gemm(true, false, D, 1, D, 1.0, e2.Z, e1.b, 0.0, v1, cnt);
   o.copy(v2, e2.eta, D, 1, cnt);
   o.gemm(false, false, D, 1, D, -1.0, e2.Z, v1, 1.0, v2, cnt);
@@ -1454,8 +1462,7 @@ int solve(Engine* e, const Problem& p, const pof_opts& opts, pof_out* out, const
         o.copy(ok.U, P0.at(d, d), D, D - d, cnt);
         o.fill(ok.U.at(0, D - d), D, d, cnt, 0.0);
         // K^T = Psi11^-T Psi21^T (K = Psi21 Psi11^-1)
-        o.copy(KT0, P0.at(d, 0), d, D, cnt, true);
-        o.trsm(true, d, D, P0.at(0, 0), KT0, cnt);
+        o.trsm(true, d, D, P0.at(0, 0), KT0, cnt, P0.at(d, 0), true);
         // A = F - K (H F)
         o.gemm(false, false, d, D, D, 1.0, Hk, Fv, 0.0, HF0, cnt);
         o.copy(ok.A, Fv, D, D, cnt);
@@ -1470,12 +1477,10 @@ int solve(Engine* e, const Problem& p, const pof_opts& opts, pof_out* out, const
           o.gemm(true, false, D, 1, d, -1.0, KT0, ck, 0.0, ok.b, cnt);
         }
         // Z = [(Psi11^-1 H F)^T | 0]; eta = -(Psi11^-T Psi11^-1 H F)^T c
-        o.copy(Z0, HF0, d, D, cnt);
-        o.trsm(false, d, D, P0.at(0, 0), Z0, cnt);
+        o.trsm(false, d, D, P0.at(0, 0), Z0, cnt, HF0, false);
         o.copy(ok.Z, Z0, D, d, cnt, true);
         o.fill(ok.Z.at(0, d), D, D - d, cnt, 0.0);
-        o.copy(X0, Z0, d, D, cnt);
-        o.trsm(true, d, D, P0.at(0, 0), X0, cnt);
+        o.trsm(true, d, D, P0.at(0, 0), X0, cnt, Z0, false);
         o.gemm(true, false, D, 1, d, -1.0, X0, ck, 0.0, ok.eta, cnt);
       }
       ar.off = mark;
@@ -1513,8 +1518,7 @@ int solve(Engine* e, const Problem& p, const pof_opts& opts, pof_out* out, const
       o.lq(Phi, 2 * D, 2 * D, n, ql_band, 0);  // second block [QL; 0]
       // E = Phi21 Phi11^-1, D = Phi22, g = m - E F m
       // E^T = Phi11^-T Phi21^T
-      o.copy(Phin, Phi.at(D, 0), D, D, n, true);
-      o.trsm(true, D, D, Phi.at(0, 0), Phin, n);
+      o.trsm(true, D, D, Phi.at(0, 0), Phin, n, Phi.at(D, 0), true);
       o.copy(snat.E, Phin, D, D, n, true);
       o.copy(snat.Dm, Phi.at(D, D), D, D, n);
       o.gemm(false, false, D, 1, D, 1.0, Fv, fmv, 0.0, Fm, n);
