@@ -693,14 +693,16 @@ __global__ void k_project(const double* m, const double* L, const double* P, int
 // 16 x 16 x 4 MFMAs: A[i][k] from lane i + 16 k, B[k][j] from lane j + 16 k;
 // value v of lane l holds result element (4 (l / 16) + v, l % 16) for FP32 but
 // (l / 16 + 4 v, l % 16) for FP64 (both checked on gfx950 with an in-bounds
-// probe of the builtins).
+// probe of the builtins).  Three blocks per CU: the 128 x 64 FP64 tile then
+// keeps its accumulators in 153 VGPRs (2 blocks with 156 VGPRs + 64 AGPRs
+// otherwise).
 typedef double gemm_d4 __attribute__((ext_vector_type(4)));
 typedef float gemm_f4 __attribute__((ext_vector_type(4)));
 
 constexpr int kGemmKc = 16;
 
 template <bool F32, int BM, int BN, int WM, int WN>
-__global__ __launch_bounds__(256) void k_gemm(int tA, int tB, int M, int N, int K, double alpha, const double* A,
+__global__ __launch_bounds__(256, 3) void k_gemm(int tA, int tB, int M, int N, int K, double alpha, const double* A,
                                               long long sA, int lda, const double* B, long long sB, int ldb,
                                               double beta, double* C, long long sC, int ldc) {
   static_assert(WM * WN == 4, "four waves");
