@@ -857,6 +857,85 @@ __global__ __launch_bounds__(64 * kGemvRowWaves) void k_gemv_rows(int tB, int M,
   }
 }
 
+// A chain of five matrix-vector products over one item (blockIdx.x), D <= 128:
+//   t1 = op1(M1) x;  t2 = y + s2 op2(M2) t1;  t3 = op3(M3) t2;  t2 -= op4(M4) t3;
+//   out = z + op5(M5) t2
+// (op = M or M^T), the vectors in LDS: one launch for what the combine's b and eta
+// would take seven (five products, two vector copies).  M x: a wave takes 16 rows at
+// once, lanes along the rows (coalesced), and a transposed wave sum finishes them;
+// M^T x: a thread per column over half the rows (coalesced along the rows), the two
+// halves added in LDS.
+struct ChainArgs {
+  const double* m[5];
+  long long ms[5];
+  int mld[5];
+  int mt[5];
+  const double *x, *y, *z;
+  long long xs, ys, zs;
+  double s2;
+  double* out;
+  long long outs;
+  int D;
+};
+constexpr int kChainMaxD = 128;
+template <typename Put>
+__device__ __forceinline__ void chain_mv(const double* M, int ld, int trans, const double* xs, double* red, int D,
+                                         Put put) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (!trans) {
+    for (int r0 = 16 * wave; r0 < D; r0 += 64) {  // uniform per wave
+      double acc[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int r = r0 + j;
+        double sum = 0.0;
+        if (r < D) {
+          const double* mr = M + (size_t)r * ld;
+          if (lane < D) sum = mr[lane] * xs[lane];
+          if (lane + 64 < D) sum = fma(mr[lane + 64], xs[lane + 64], sum);
+        }
+        acc[j] = sum;
+      }
+      wave::sum_scatter<16>(acc, lane);
+      const int r = r0 + (lane >> 2);
+      if ((lane & 3) == 0 && r < D) put(r, acc[0]);
+    }
+  } else {
+    const int c = tid & 127, h = tid >> 7;
+    const int half = (D + 1) / 2;
+    const int rb = h * half, re = (h == 0) ? half : D;
+    double sum = 0.0;
+    if (c < D) {
+#pragma unroll 8
+      for (int r = rb; r < re; ++r) sum = fma(M[(size_t)r * ld + c], xs[r], sum);
+    }
+    red[tid] = sum;
+    __syncthreads();
+    if (tid < D) put(tid, red[tid] + red[tid + 128]);
+  }
+  __syncthreads();
+}
+__global__ __launch_bounds__(256) void k_vec_chain(ChainArgs a) {
+  __shared__ double v0[kChainMaxD], v1[kChainMaxD], v2[kChainMaxD], v3[kChainMaxD];
+  __shared__ double red[256];
+  const long long it = blockIdx.x;
+  const int tid = threadIdx.x, D = a.D;
+  const double* M[5];
+#pragma unroll
+  for (int q = 0; q < 5; ++q) M[q] = a.m[q] + it * a.ms[q];
+  if (tid < D) v0[tid] = a.x[it * a.xs + tid];
+  __syncthreads();
+  const double* y = a.y + it * a.ys;
+  const double* z = a.z + it * a.zs;
+  double* out = a.out + it * a.outs;
+  const double s2 = a.s2;
+  chain_mv(M[0], a.mld[0], a.mt[0], v0, red, D, [&](int i, double t) { v1[i] = t; });
+  chain_mv(M[1], a.mld[1], a.mt[1], v1, red, D, [&](int i, double t) { v2[i] = fma(s2, t, y[i]); });
+  chain_mv(M[2], a.mld[2], a.mt[2], v2, red, D, [&](int i, double t) { v3[i] = t; });
+  chain_mv(M[3], a.mld[3], a.mt[3], v3, red, D, [&](int i, double t) { v2[i] -= t; });
+  chain_mv(M[4], a.mld[4], a.mt[4], v2, red, D, [&](int i, double t) { out[i] = z[i] + t; });
+}
+
 // ------------------------------------------------------------- host helpers
 struct Arena;
 double* arena_take(Arena* ar, size_t n);
@@ -888,6 +967,36 @@ struct Ops {
   }
   void gemm64(bool tA, bool tB, int M, int N, int K, double alpha, V A, V B, double beta, V C, int cnt) {
     gemm_impl(false, tA, tB, M, N, K, alpha, A, B, beta, C, cnt);
+  }
+  // out = z + op5(M5) (t2 - op4(M4) op3(M3) t2), t2 = y + s2 op2(M2) op1(M1) x, per item (k_vec_chain)
+  void vec_chain(int D, V m1, bool t1, V m2, bool t2, V m3, bool t3, V m4, bool t4, V m5, bool t5, V x, V y,
+                 double s2, V z, V out, int cnt) {
+    if (err || cnt <= 0) return;
+    if (D > kChainMaxD) {
+      fail("vector chain: D beyond 128", -1);
+      return;
+    }
+    ChainArgs a{};
+    const V mm[5] = {m1, m2, m3, m4, m5};
+    const bool tt[5] = {t1, t2, t3, t4, t5};
+    for (int q = 0; q < 5; ++q) {
+      a.m[q] = mm[q].p;
+      a.ms[q] = mm[q].s;
+      a.mld[q] = mm[q].ld;
+      a.mt[q] = tt[q] ? 1 : 0;
+    }
+    a.x = x.p;
+    a.xs = x.s;
+    a.y = y.p;
+    a.ys = y.s;
+    a.z = z.p;
+    a.zs = z.s;
+    a.s2 = s2;
+    a.out = out.p;
+    a.outs = out.s;
+    a.D = D;
+    hipLaunchKernelGGL(k_vec_chain, dim3(cnt), dim3(256), 0, st, a);
+    check("k_vec_chain");
   }
   void gemm_impl(bool use32, bool tA, bool tB, int M, int N, int K, double alpha, V A, V B, double beta, V C,
                  int cnt) {
@@ -1138,8 +1247,7 @@ void filter_fn(Ops& o, Arena& ar, int D, const FE& e1, const FE& e2, const FE& o
   const V YZ = Y.item(cnt);
   V T2{ar.take((size_t)cnt * DD), DD, D};
   V T3{ar.take((size_t)cnt * DD), DD, D}, T4{ar.take((size_t)cnt * DD), DD, D};
-  V v1{ar.take((size_t)cnt * D), D, 1}, v2{ar.take((size_t)cnt * D), D, 1}, v3{ar.take((size_t)cnt * D), D, 1};
-  if (!X.p || !Y.p || !T2.p || !T3.p || !T4.p || !v1.p || !v2.p || !v3.p) {
+  if (!X.p || !Y.p || !T2.p || !T3.p || !T4.p) {
     o.fail("workspace", -2);
     return;
   }
@@ -1160,24 +1268,13 @@ void filter_fn(Ops& o, Arena& ar, int D, const FE& e1, const FE& e2, const FE& o
   o.gemm(false, false, D, D, D, 1.0, e2.A, e1.A, 0.0, out.A, cnt);
   o.gemm(false, false, D, D, D, -1.0, Y.at(0, 0), T2, 1.0, out.A, cnt);
   // b = A2 (v2 - m_^T Xi21^T v2) + b2, v2 = b1 + U1 (U1^T eta2)
-  o.gemm(true, false, D, 1, D, 1.0, e1.U, e2.eta, 0.0, v1, cnt);
-  o.copy(v2, e1.b, D, 1, cnt);
-  o.gemm(false, false, D, 1, D, 1.0, e1.U, v1, 1.0, v2, cnt);
-  o.gemm(true, false, D, 1, D, 1.0, Xi21, v2, 0.0, v3, cnt);
-  o.gemm(true, false, D, 1, D, -1.0, T3, v3, 1.0, v2, cnt);
-  o.copy(out.b, e2.b, D, 1, cnt);
-  o.gemm(false, false, D, 1, D, 1.0, e2.A, v2, 1.0, out.b, cnt);
+  o.vec_chain(D, e1.U, true, e1.U, false, Xi21, true, T3, true, e2.A, false, e2.eta, e1.b, 1.0, e2.b, out.b, cnt);
   // [M^T | U2]
   o.copy(Y.at(0, D), e2.U, D, D, cnt);
   // _e = Xi11^-T Xi21^T -> T4; eta = A1^T (w2 - _e^T U1^T w2) + eta1, w2 = eta2 - Z2 (Z2^T b1)
   o.trsm(true, D, D, Xi11, T4, cnt, Xi21, true);
-  o.gemm(true, false, D, 1, D, 1.0, e2.Z, e1.b, 0.0, v1, cnt);
-  o.copy(v2, e2.eta, D, 1, cnt);
-  o.gemm(false, false, D, 1, D, -1.0, e2.Z, v1, 1.0, v2, cnt);
-  o.gemm(true, false, D, 1, D, 1.0, e1.U, v2, 0.0, v3, cnt);
-  o.gemm(true, false, D, 1, D, -1.0, T4, v3, 1.0, v2, cnt);
-  o.copy(out.eta, e1.eta, D, 1, cnt);
-  o.gemm(true, false, D, 1, D, 1.0, e1.A, v2, 1.0, out.eta, cnt);
+  o.vec_chain(D, e2.Z, true, e2.Z, false, e1.U, true, T4, true, e1.A, true, e1.b, e2.eta, -1.0, e1.eta, out.eta,
+              cnt);
   // [A1^T Xi22 | Z1]; then U = tria([M^T | U2]) and Z = tria([A1^T Xi22 | Z1]) in one call
   o.gemm(true, false, D, D, D, 1.0, e1.A, Xi22, 0.0, YZ.at(0, 0), cnt);
   o.copy(YZ.at(0, D), e1.Z, D, D, cnt);
@@ -1294,7 +1391,7 @@ size_t workspace_doubles(int D, int d, long long Ns, bool f32) {
   w += n1 * (D + DD);                       // smoothed natural order
   // per-call temporaries at the widest level (the filter combine), and the post step
   const size_t h = (size_t)(Ns / 2 + 1);
-  w += h * (4 * DD + 4 * DD + 5 * DD + 3 * D) + 4096;
+  w += h * (4 * DD + 4 * DD + 5 * DD) + 4096;
   w += (size_t)(Ns + 1) * (2 * D * kNB + kNB * kNB) + 4096;  // blocked-LQ panels (V, T)
   w += (size_t)Ns * (5 * DD + (size_t)d * D + 2 * D + 3 * d + 4) + 4096;
   w += 8 * n1 + 64 * 1024;
