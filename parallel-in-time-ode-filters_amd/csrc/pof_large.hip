@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <initializer_list>
 #include <type_traits>
 #include <utility>
 #include <vector>
@@ -539,6 +540,37 @@ __global__ void k_fill(double* dst, long long ds, int dld, int rows, int cols, i
   if (b >= cnt || e >= rows * cols) return;
   const int r = e / cols, c = e % cols;
   dst[(long long)b * ds + (size_t)r * dld + c] = (r == c) ? dv : 0.0;
+}
+
+// Up to kMaxCopyOps block copies (or identity fills, src = nullptr: dst = fill I) in one
+// launch: op blockIdx.z of item blockIdx.y.  grid (ceil(max rows x cols / 256), cnt, ops)
+struct CopyOp {
+  double* dst;
+  long long ds;
+  int dld;
+  const double* src;
+  long long ss;
+  int sld;
+  int rows, cols;
+  double fill;
+};
+constexpr int kMaxCopyOps = 5;
+struct CopyOps {
+  CopyOp op[kMaxCopyOps];
+};
+__global__ void k_copy_multi(CopyOps a) {
+  const int z = blockIdx.z;
+  // the op by compile-time indices (uniform selects; no dynamic index into the arguments)
+  CopyOp o = a.op[0];
+#pragma unroll
+  for (int q = 1; q < kMaxCopyOps; ++q)
+    if (z == q) o = a.op[q];
+  const long long b = blockIdx.y;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= o.rows * o.cols) return;
+  const int r = e / o.cols, c = e % o.cols;
+  const double x = o.src ? o.src[b * o.ss + (size_t)r * o.sld + c] : ((r == c) ? o.fill : 0.0);
+  o.dst[b * o.ds + (size_t)r * o.dld + c] = x;
 }
 
 // EK1 of the Lorenz-96 residual at n states (rows of H as in the per-worker
@@ -1169,6 +1201,28 @@ struct Ops {
                        dv);
     check("k_fill");
   }
+  // several independent copies (src.p == nullptr: dst = fill I) over cnt items in one launch
+  struct CopySpec {
+    V dst, src;
+    int rows, cols;
+    double fill;
+  };
+  void copies(std::initializer_list<CopySpec> ops, int cnt) {
+    if (err || cnt <= 0) return;
+    if (ops.size() > (size_t)kMaxCopyOps) {
+      fail("copies: too many", -1);
+      return;
+    }
+    CopyOps a{};
+    int n = 0, emax = 0;
+    for (const CopySpec& c : ops) {
+      a.op[n++] = CopyOp{c.dst.p, c.dst.s, c.dst.ld, c.src.p, c.src.s, c.src.ld, c.rows, c.cols, c.fill};
+      emax = std::max(emax, c.rows * c.cols);
+    }
+    if (n == 0 || emax == 0) return;
+    hipLaunchKernelGGL(k_copy_multi, dim3((emax + 255) / 256, cnt, n), dim3(256), 0, st, a);
+    check("k_copy_multi");
+  }
   void check(const char* what) {
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) fail(what, (int)e);
@@ -1253,9 +1307,7 @@ void filter_fn(Ops& o, Arena& ar, int D, const FE& e1, const FE& e2, const FE& o
   }
   // Xi = tria([[U1^T Z2, I], [Z2, 0]])
   o.gemm(true, false, D, D, D, 1.0, e1.U, e2.Z, 0.0, X.at(0, 0), cnt);
-  o.fill(X.at(0, D), D, D, cnt, 1.0);
-  o.copy(X.at(D, 0), e2.Z, D, D, cnt);
-  o.fill(X.at(D, D), D, D, cnt, 0.0);
+  o.copies({{X.at(0, D), V{}, D, D, 1.0}, {X.at(D, 0), e2.Z, D, D, 0.0}, {X.at(D, D), V{}, D, D, 0.0}}, cnt);
   // second block [I; 0]: row r is zero beyond column D + r
   o.lq(X, 2 * D, 2 * D, cnt, D, 0);
   const V Xi11 = X.at(0, 0), Xi21 = X.at(D, 0), Xi22 = X.at(D, D);
@@ -1269,19 +1321,16 @@ void filter_fn(Ops& o, Arena& ar, int D, const FE& e1, const FE& e2, const FE& o
   o.gemm(false, false, D, D, D, -1.0, Y.at(0, 0), T2, 1.0, out.A, cnt);
   // b = A2 (v2 - m_^T Xi21^T v2) + b2, v2 = b1 + U1 (U1^T eta2)
   o.vec_chain(D, e1.U, true, e1.U, false, Xi21, true, T3, true, e2.A, false, e2.eta, e1.b, 1.0, e2.b, out.b, cnt);
-  // [M^T | U2]
-  o.copy(Y.at(0, D), e2.U, D, D, cnt);
   // _e = Xi11^-T Xi21^T -> T4; eta = A1^T (w2 - _e^T U1^T w2) + eta1, w2 = eta2 - Z2 (Z2^T b1)
   o.trsm(true, D, D, Xi11, T4, cnt, Xi21, true);
   o.vec_chain(D, e2.Z, true, e2.Z, false, e1.U, true, T4, true, e1.A, true, e1.b, e2.eta, -1.0, e1.eta, out.eta,
               cnt);
-  // [A1^T Xi22 | Z1]; then U = tria([M^T | U2]) and Z = tria([A1^T Xi22 | Z1]) in one call
+  // A1^T Xi22 and the right blocks U2, Z1; then U = tria([M^T | U2]) and Z = tria([A1^T Xi22 | Z1]) in one call
   o.gemm(true, false, D, D, D, 1.0, e1.A, Xi22, 0.0, YZ.at(0, 0), cnt);
-  o.copy(YZ.at(0, D), e1.Z, D, D, cnt);
+  o.copies({{Y.at(0, D), e2.U, D, D, 0.0}, {YZ.at(0, D), e1.Z, D, D, 0.0}}, cnt);
   // second blocks U2 (lower triangular) and Z1 (lower triangular, or [Zr | 0] at the leaves)
   o.lq(Y, D, 2 * D, 2 * cnt, D, o.zw2);
-  o.copy(out.U, Y.at(0, 0), D, D, cnt);
-  o.copy(out.Z, YZ.at(0, 0), D, D, cnt);
+  o.copies({{out.U, Y.at(0, 0), D, D, 0.0}, {out.Z, YZ.at(0, 0), D, D, 0.0}}, cnt);
   ar.off = mark;
 }
 
@@ -1337,16 +1386,15 @@ void scan(Ops& o, Arena& ar, int D, const E& el, const E& out, long long n, Fn f
 }
 
 void fe_copy1(Ops& o, int D, const FE& dst, const FE& src) {
-  o.copy(dst.A, src.A, D, D, 1);
-  o.copy(dst.b, src.b, D, 1, 1);
-  o.copy(dst.U, src.U, D, D, 1);
-  o.copy(dst.eta, src.eta, D, 1, 1);
-  o.copy(dst.Z, src.Z, D, D, 1);
+  o.copies({{dst.A, src.A, D, D, 0.0},
+            {dst.b, src.b, D, 1, 0.0},
+            {dst.U, src.U, D, D, 0.0},
+            {dst.eta, src.eta, D, 1, 0.0},
+            {dst.Z, src.Z, D, D, 0.0}},
+           1);
 }
 void se_copy1(Ops& o, int D, const SE& dst, const SE& src) {
-  o.copy(dst.g, src.g, D, 1, 1);
-  o.copy(dst.E, src.E, D, D, 1);
-  o.copy(dst.Dm, src.Dm, D, D, 1);
+  o.copies({{dst.g, src.g, D, 1, 0.0}, {dst.E, src.E, D, D, 0.0}, {dst.Dm, src.Dm, D, D, 0.0}}, 1);
 }
 
 }  // namespace
@@ -1611,9 +1659,7 @@ int solve(Engine* e, const Problem& p, const pof_opts& opts, pof_out* out, const
       if (!Phi.p || !Fm.p || !Sb.p || !yv.p || !Phin.p) o.fail("workspace", -2);
       const V fmv{fm, D, 1}, fLv{fL, DD, D};
       o.gemm(false, false, D, D, D, 1.0, Fv, fLv, 0.0, Phi.at(0, 0), n);
-      o.copy(Phi.at(0, D), QLv, D, D, n);
-      o.copy(Phi.at(D, 0), fLv, D, D, n);
-      o.fill(Phi.at(D, D), D, D, n, 0.0);
+      o.copies({{Phi.at(0, D), QLv, D, D, 0.0}, {Phi.at(D, 0), fLv, D, D, 0.0}, {Phi.at(D, D), V{}, D, D, 0.0}}, n);
       o.lq(Phi, 2 * D, 2 * D, n, ql_band, 0);  // second block [QL; 0]
       // E = Phi21 Phi11^-1, D = Phi22, g = m - E F m
       // E^T = Phi11^-T Phi21^T
