@@ -149,6 +149,9 @@ def gpu_run(steps, warmup):
             "algorithmic_bytes_per_launch": roofline.bytes_per_unit(dom, D, d)
             * work[roofline.KERNEL_CLASSES.index(dom)] / c["launches"],
             "traffic_source": TRAFFIC_SUMMARY.name if traffic is not None else None,
+            "peak_note": "MI355X's FP64 peak (78.6 TFLOP/s) is the same on the matrix cores and the vector ALUs; "
+                         "the D = 12 combines form their tile products on the matrix cores and the Householder "
+                         "pivots on the vector ALUs",
         },
         "instrumented_iteration_ms": total_ms / ITERS,
         "iteration_tflops_algorithmic": iter_flops / (total_ms / ITERS * 1e-3) / 1e12,
