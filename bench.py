@@ -327,7 +327,7 @@ def small_state_run(ns=65536, iters=10):
     }
 
 
-def _cfg2_iteration_job(_):
+def _lorenz63_iteration_job(_):
     """One full-size oracle IEKS iteration of the Lorenz-63 workload in a worker process (seconds)."""
     import numpy as np
 
@@ -358,10 +358,10 @@ def cpu_baseline():
     try:
         ctx = mp.get_context("spawn")
         with ctx.Pool(1) as pool:
-            single = pool.map(_cfg2_iteration_job, [0])[0]
+            single = pool.map(_lorenz63_iteration_job, [0])[0]
         t0 = time.perf_counter()
         with ctx.Pool(cores) as pool:
-            per = pool.map(_cfg2_iteration_job, range(cores), chunksize=1)
+            per = pool.map(_lorenz63_iteration_job, range(cores), chunksize=1)
         wall = time.perf_counter() - t0
     finally:
         for k, v in env.items():
