@@ -86,10 +86,12 @@ def test_large_engine_fixed_iterations_lorenz63(gpu):
     assert rel(ys.mean.cpu().numpy(), ref.mean) < 1e-6
 
 
-@pytest.mark.parametrize("d,q,N,fixed", [(32, 3, 17, 3), (8, 2, 33, None)])
+@pytest.mark.parametrize("d,q,N,fixed", [(32, 3, 17, 3), (8, 2, 33, None), (5, 4, 17, 3)])
 def test_lorenz96_matches_oracle(gpu, d, q, N, fixed):
     """d = 32, q = 3 is the D = 128 configuration (short horizon, same dt as t in [0, 5] with 8192
-    steps); d = 8, q = 2 (D = 24) is beyond the per-worker kernels too and runs to convergence."""
+    steps); d = 8, q = 2 (D = 24) is beyond the per-worker kernels too and runs to convergence;
+    d = 5, q = 4 (D = 25) leaves partial 16-row blocks in the LQ panels and triangular solves and an
+    odd vector length in the matrix-vector chains."""
     import torch
 
     import pof.solver
