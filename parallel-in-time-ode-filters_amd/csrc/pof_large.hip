@@ -2,14 +2,16 @@
 // e.g. Lorenz-96 d = 32, q = 3: D = 128).
 //
 // Each combine of a scan level is a sequence of batched primitives over every
-// combine of that level: a Householder LQ kernel (one workgroup per matrix,
-// LAPACK dlarfg's reflector convention, the same as the per-worker lq()), a
+// combine of that level: the blocked Householder LQ (a panel kernel, one wave
+// per matrix, and a one-pass trailing update on the matrix cores; LAPACK
+// dlarfg's reflector convention, the same as the per-worker lq()), a
 // strided-batched GEMM kernel on the matrix cores (v_mfma_f64_16x16x4f64, or
-// v_mfma_f32_16x16x4f32 for the FP32-product option), a blocked
-// triangular-solve kernel, and small copy / fill / reduction kernels.  The
-// host drives JAX's odd/even associative_scan recursion (pof/parallel_filtsmooth
-// filter.py:35, smoother.py:15-17) over strided views, so every factor is built
-// by the same sequence of operations as in the reference.
+// v_mfma_f32_16x16x4f32 for the FP32-product option), blocked triangular solves
+// on the matrix cores, fused chains of matrix-vector products, and small copy /
+// fill / reduction kernels.  The host drives JAX's odd/even associative_scan
+// recursion (pof/parallel_filtsmooth filter.py:35, smoother.py:15-17) over
+// strided views, so every factor is built by the same sequence of operations as
+// in the reference.
 //
 // Matrices are row-major float64.
 #include "pof_large.hpp"
